@@ -1,0 +1,176 @@
+"""SpMV benchmark on one MI355X: FP32 SpMV on the 2M x 2M power-law matrix.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload powerlaw|uniform|laplacian]
+
+A step is one y = A*x over the whole matrix with A, x, y resident in HBM.
+Every step reads a different copy of (A, x, y) (enough copies that the
+others evict it from the 256 MiB Infinity Cache), so each step is a cold
+read. The kernel and its tile size are picked by the library's autotune
+sweep over the candidate kernels (the sweep runs before the timed region).
+
+Prints one JSON line: value = algorithmic GB/s over the K timed steps
+(bytes = N*(4+4) + 4*(R+1) + 4*C + 4*R per step); roofline = the main
+kernel's average duration from HIP events recorded around it in every
+timed step; cpu_baseline = the oracle's FP32 CSR loop on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+CANDIDATES = [("merge", 4), ("merge", 8), ("merge", 16), ("csr5", 256), ("csr5", 512), ("csr5", 1024),
+              ("vwave", 4), ("vwave", 8), ("vector", 0), ("scalar", 0)]
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload(name: str):
+    from muxi_spmv_amd import generators as G
+    import numpy as np
+
+    if name == "powerlaw":
+        return G.cfg_powerlaw_2m(), np.float32, "powerlaw_2m_fp32"
+    if name == "uniform":
+        return G.cfg_uniform_100k(), np.float32, "uniform_100k_fp32"
+    if name == "laplacian":
+        return G.cfg_laplacian_4096(), np.float64, "laplacian_4096_fp64"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(h, x, budget_s: float = 12.0) -> dict:
+    """The oracle's FP32 CSR loop (the reference's CPU loop) on all host cores."""
+    import oracle
+    from muxi_spmv_amd.benchmark import algorithmic_bytes
+
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(threads, 64))
+    vals = h.values.astype("float32")
+    xx = x.astype("float32")
+    oracle.spmv_csr_f32_mt(h.rowPtr, h.colIdx, vals, xx, threads)  # warm
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        oracle.spmv_csr_f32_mt(h.rowPtr, h.colIdx, vals, xx, threads)
+        reps += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    b = algorithmic_bytes(h.numRows, h.numCols, h.nnz, 4, 4)
+    return {"value": round(b / ms / 1e6, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"full matrix, {reps} SpMVs of the FP32 CSR loop ({ms:.2f} ms each), OpenMP rows"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="powerlaw")
+    ap.add_argument("--kernel", default="sweep", help="sweep, or kernel:param (e.g. merge:8)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep-steps", type=int, default=30)
+    a = ap.parse_args(argv)
+
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the SpMV path here is single-GPU; "
+              f"row-block sharding across GPUs is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from muxi_spmv_amd import api
+    from muxi_spmv_amd import generators as G
+    from muxi_spmv_amd.benchmark import (HBM_PEAK_GBS, algorithmic_bytes, calculate_gflops, copies_for_cold,
+                                         measure_plans)
+
+    torch.cuda.set_device(0)
+    t = time.perf_counter()
+    h, vdt, wname = make_workload(a.workload)
+    x = G.uniform_vector(h.numCols, 12345, vdt)
+    log(f"workload {wname}: R={h.numRows} nnz={h.nnz} built in {time.perf_counter() - t:.1f}s")
+    tdt = torch.float64 if vdt == np.float64 else torch.float32
+    esz = 8 if vdt == np.float64 else 4
+    bytes_step = algorithmic_bytes(h.numRows, h.numCols, h.nnz, esz, esz)
+    ncopy = copies_for_cold(bytes_step)
+    A0 = api.CSRMatrix(h.rowPtr, h.colIdx, torch.from_numpy(h.values).to(tdt), h.numCols)
+    mats = [A0] + [api.CSRMatrix(A0.d_rowPtr.clone(), A0.d_colIdx.clone(), A0.d_values.clone(), h.numCols)
+                   for _ in range(ncopy - 1)]
+    xs = [torch.from_numpy(x).cuda() for _ in range(ncopy)]
+    ys = [torch.zeros(h.numRows, dtype=tdt, device="cuda") for _ in range(ncopy)]
+    log(f"{ncopy} rotating copies of {bytes_step / 1e6:.1f} MB")
+
+    sweep = {}
+    if a.kernel == "sweep":
+        cands = [c for c in CANDIDATES if not (c[0] in ("vector", "scalar") and h.nnz > 50_000_000)]
+        for k, p in cands:
+            plans = [api.Plan(m, k, p, xtype=tdt) for m in mats]
+            m = measure_plans(plans, xs, ys, a.sweep_steps, 3, record_kernel=False)
+            sweep[f"{k}:{p}"] = round(m.ms_per_step, 5)
+            log(f"sweep {k}:{p} {m.ms_per_step * 1e3:.1f} us/step")
+            del plans
+        best = min(sweep, key=sweep.get)
+    else:
+        best = a.kernel
+    kname, kparam = best.split(":")
+    plans = [api.Plan(m, kname, int(kparam), xtype=tdt) for m in mats]
+    info = plans[0].info()
+
+    torch.cuda.synchronize()
+    meas = measure_plans(plans, xs, ys, a.steps, a.warmup)
+    ms = meas.ms_per_step
+    value = bytes_step / ms / 1e6
+    kms = meas.kernel_ms_mean
+    achieved = bytes_step / kms / 1e6
+    log(f"{best}: {ms * 1e3:.2f} us/step, main kernel {kms * 1e3:.2f} us (median {meas.kernel_ms_median * 1e3:.2f})")
+
+    # correctness of the measured configuration against the device FP64 oracle
+    y = torch.zeros(h.numRows, dtype=tdt, device="cuda")
+    plans[0].exec(xs[0], y)
+    y64 = api.oracle_f64(A0, xs[0])
+    torch.cuda.synchronize()
+    rel_l2 = float(torch.linalg.vector_norm(y.double() - y64) / torch.linalg.vector_norm(y64))
+
+    out = {
+        "metric": "SpMV GB/s (frac of HBM peak) + GFLOP/s; learned-pick regret vs sweep",
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp64" if vdt == np.float64 else "fp32",
+        "data": "synthetic (seeded generator, cold: each step reads one of "
+                f"{ncopy} rotating copies)",
+        "config": {"workload": wname, "rows": h.numRows, "cols": h.numCols, "nnz": h.nnz,
+                   "kernel": info.kernel, "param": info.param0, "parallelism": "single GPU"},
+        "gflops": round(calculate_gflops(h.nnz, ms), 3),
+        "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
+        "rel_l2_vs_fp64": rel_l2,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_us_mean": round(kms * 1e3, 3), "bytes_per_launch": bytes_step},
+        "sweep_ms": sweep,
+    }
+    if not a.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(h, x)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,145 @@
+"""ctypes binding of libmuxi_spmv_amd.so (the C-ABI in include/muxi_spmv/muxi_spmv.h).
+
+Loading never falls back to anything: a missing or stale library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("MUXI_SPMV_LIB", _PKG / "libmuxi_spmv_amd.so"))
+
+# ---- enums (src/utils/common.h:32-71)
+SPMV_SUCCESS = 0
+STATUS_NAMES = {
+    0: "SPMV_SUCCESS", -1: "SPMV_ERROR_INVALID_HANDLE", -2: "SPMV_ERROR_INVALID_MATRIX",
+    -3: "SPMV_ERROR_INVALID_VECTOR", -4: "SPMV_ERROR_MEMORY_ALLOC", -5: "SPMV_ERROR_MEMORY_COPY",
+    -6: "SPMV_ERROR_DEVICE", -7: "SPMV_ERROR_UNSUPPORTED_FORMAT", -8: "SPMV_ERROR_UNSUPPORTED_TYPE",
+    -9: "SPMV_ERROR_INTERNAL",
+}
+SPMV_FORMAT_CSR, SPMV_FORMAT_COO, SPMV_FORMAT_CSR2, SPMV_FORMAT_BSR, SPMV_FORMAT_CSR5 = 0, 1, 2, 3, 4
+SPMV_TYPE_FLOAT, SPMV_TYPE_DOUBLE, SPMV_TYPE_HALF, SPMV_TYPE_BF16 = 0, 1, 2, 3
+SPMV_OP_NORMAL, SPMV_OP_TRANSPOSE = 0, 1
+
+KERNELS = {"auto": 0, "scalar": 1, "vector": 2, "vwave": 3, "adaptive": 4, "merge": 5, "csr5": 6, "ell": 7}
+KERNEL_NAMES = {v: k for k, v in KERNELS.items()}
+
+
+class spmv_opts_t(C.Structure):
+    _fields_ = [("operation", C.c_int), ("sync", C.c_int), ("check_inputs", C.c_int),
+                ("use_tensor_core", C.c_int), ("use_streaming", C.c_int), ("stream_id", C.c_int),
+                ("stream", C.c_void_p)]
+
+
+class muxi_csr_t(C.Structure):
+    _fields_ = [("numRows", C.c_int), ("numCols", C.c_int), ("nnz", C.c_int),
+                ("rowPtr", C.c_void_p), ("colIdx", C.c_void_p), ("values", C.c_void_p),
+                ("d_rowPtr", C.c_void_p), ("d_colIdx", C.c_void_p), ("d_values", C.c_void_p),
+                ("ownsHostMemory", C.c_bool), ("ownsDeviceMemory", C.c_bool)]
+
+
+class muxi_csr5_t(C.Structure):
+    _fields_ = [("numRows", C.c_int), ("numCols", C.c_int), ("nnz", C.c_int),
+                ("rowPtr", C.c_void_p), ("colIdx", C.c_void_p), ("values", C.c_void_p),
+                ("sigma", C.c_int), ("numTiles", C.c_int),
+                ("tile_row_ptr", C.POINTER(C.c_int)), ("tile_nnz_offset", C.POINTER(C.c_int)),
+                ("d_rowPtr", C.c_void_p), ("d_colIdx", C.c_void_p), ("d_values", C.c_void_p),
+                ("d_tile_row_ptr", C.c_void_p), ("d_tile_nnz_offset", C.c_void_p),
+                ("ownsCsrData", C.c_bool), ("ownsTileData", C.c_bool), ("ownsDeviceMemory", C.c_bool),
+                ("conversionTimeMs", C.c_double)]
+
+
+class muxi_ell_t(C.Structure):
+    _fields_ = [("numRows", C.c_int), ("numCols", C.c_int), ("numElements", C.c_int), ("nnz", C.c_int),
+                ("d_colIdx", C.c_void_p), ("d_values", C.c_void_p)]
+
+
+class muxi_spmv_handle_t(C.Structure):
+    _fields_ = [("deviceId", C.c_int), ("stream", C.c_void_p), ("opts", spmv_opts_t), ("warpSize", C.c_int),
+                ("smCount", C.c_int), ("maxThreadsPerBlock", C.c_int), ("sharedMemPerSM", C.c_size_t)]
+
+
+_VP, _I, _D, _F, _Z = C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_size_t
+_P_CSR, _P_CSR5, _P_ELL = C.POINTER(muxi_csr_t), C.POINTER(muxi_csr5_t), C.POINTER(muxi_ell_t)
+_P_OPTS = C.POINTER(spmv_opts_t)
+_P_H = C.POINTER(muxi_spmv_handle_t)
+
+# name -> (restype, argtypes). Every symbol declared in include/muxi_spmv/*.h appears here.
+SIGNATURES: dict[str, tuple] = {
+    "muxi_spmv_default_opts": (spmv_opts_t, []),
+    "muxi_spmv_create_handle": (_I, [C.POINTER(_P_H)]),
+    "muxi_spmv_destroy_handle": (_I, [_P_H]),
+    "muxi_spmv_set_stream": (_I, [_P_H, _VP]),
+    "muxi_spmv_last_error": (C.c_char_p, []),
+    "muxi_spmv_version": (C.c_char_p, []),
+    "muxi_spmv_csr_f32": (_I, [_P_CSR, _VP, _VP, _F, _F, _P_OPTS]),
+    "muxi_spmv_csr_f64": (_I, [_P_CSR, _VP, _VP, _D, _D, _P_OPTS]),
+    "muxi_spmv_exec_f32": (_I, [_P_H, _I, _VP, _VP, _VP, _F, _F]),
+    "muxi_spmv_exec_f64": (_I, [_P_H, _I, _VP, _VP, _VP, _D, _D]),
+    "muxi_spmv_exec_transpose_f32": (_I, [_P_H, _I, _VP, _VP, _VP, _F, _F]),
+    "muxi_spmv_exec_transpose_f64": (_I, [_P_H, _I, _VP, _VP, _VP, _D, _D]),
+    "muxi_spmv_merge_based_f32": (_I, [_P_CSR, _VP, _VP, _VP]),
+    "muxi_spmv_merge_based_f64": (_I, [_P_CSR, _VP, _VP, _VP]),
+    "muxi_spmv_merge_based_v2_f32": (_I, [_P_CSR, _VP, _VP, _VP, C.c_bool]),
+    "muxi_spmv_merge_based_v2_f64": (_I, [_P_CSR, _VP, _VP, _VP, C.c_bool]),
+    "muxi_spmv_sparse_optimized_f32": (_I, [_P_CSR, _VP, _VP, _VP]),
+    "muxi_spmv_sparse_optimized_f64": (_I, [_P_CSR, _VP, _VP, _VP]),
+    "muxi_merge_partition_count": (_I, [_I, _I, _I, _I, _I]),
+    "muxi_merge_partitions": (_I, [_I, _I, _I, _VP, _VP]),
+    "muxi_merge_path_coords": (_I, [_VP, _I, _I, _VP, _I, _VP, _VP, _VP]),
+    "muxi_csr5_preprocess": (_I, [_P_CSR, _P_CSR5, _I, _VP]),
+    "muxi_csr5_destroy": (_I, [_P_CSR5]),
+    "muxi_spmv_csr5_f32": (_I, [_P_CSR5, _VP, _VP, _F, _F, _P_OPTS]),
+    "muxi_spmv_csr5_f64": (_I, [_P_CSR5, _VP, _VP, _D, _D, _P_OPTS]),
+    "muxi_csr_to_ellpack_f32": (_I, [_P_CSR, _P_ELL, _I]),
+    "muxi_csr_to_ellpack_f64": (_I, [_P_CSR, _P_ELL, _I]),
+    "muxi_spmv_ellpack_f32": (_I, [_P_ELL, _VP, _VP, _VP]),
+    "muxi_spmv_ellpack_f64": (_I, [_P_ELL, _VP, _VP, _VP]),
+    "muxi_ellpack_destroy": (_I, [_P_ELL]),
+    "muxi_plan_create": (_I, [_P_CSR, _I, _I, _I, _I, _I, _VP, C.POINTER(_VP)]),
+    "muxi_plan_destroy": (_I, [_VP]),
+    "muxi_plan_exec": (_I, [_VP, _VP, _VP, _D, _D, _I, _VP]),
+    "muxi_plan_info": (_I, [_VP, C.POINTER(_I), C.POINTER(_I), C.POINTER(_Z)]),
+    "muxi_plan_exec_ev": (_I, [_VP, _VP, _VP, _D, _D, _I, _VP, _VP, _VP]),
+    "muxi_event_create": (_I, [C.POINTER(_VP)]),
+    "muxi_event_destroy": (_I, [_VP]),
+    "muxi_event_record": (_I, [_VP, _VP]),
+    "muxi_event_elapsed_ms": (_I, [_VP, _VP, C.POINTER(C.c_float)]),
+    "muxi_spmv_oracle_f64": (_I, [_P_CSR, _I, _I, _VP, _VP, _VP]),
+    "muxi_convert_values": (_I, [_VP, _I, _VP, _I, _Z, _VP]),
+    "muxi_device_copy": (_I, [_VP, _VP, _Z, _VP]),
+}
+
+_lib = None
+
+
+def load(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Loads the library once; raises if it is missing or lacks a declared symbol."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} not built: run `python -m muxi_spmv_amd.build` (hipcc, gfx950)")
+    lib = C.CDLL(str(p), mode=C.RTLD_GLOBAL)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)  # AttributeError on a missing export
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class SpmvError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str = ""):
+        self.status = status
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)} {detail}".strip())
+
+
+def check(status: int, where: str) -> None:
+    if status != SPMV_SUCCESS:
+        detail = load().muxi_spmv_last_error().decode(errors="replace")
+        raise SpmvError(status, where, detail)

@@ -1,0 +1,131 @@
+"""Benchmark layer: byte/flop model, cold-cache timing and the results JSON.
+
+Counterpart of src/benchmark/performance_benchmark.{h,cu}:
+  calculate_gflops      calculateGFlops (performance_benchmark.h:79-85): 2*N*k / t
+  algorithmic_bytes     calculateBandwidth (:96-111), generalised to value
+                        and vector element sizes and k right-hand sides:
+                        N*(bv + 4) + 4*(R+1) + k*C*bx + k*R*by (+ k*R*by when beta != 0)
+  measure_plan          measureCSRPerformance (performance_benchmark.cu:56-169):
+                        warmups, then timed launches; here every launch reads a
+                        different copy of the matrix so that no launch finds its
+                        data in L2 or the 256 MiB Infinity Cache ("cold").
+  write_results_json    writeResultsJSON (:298-335) schema, extended.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import statistics
+import time
+from dataclasses import asdict, dataclass, field
+
+import torch
+
+from . import _lib
+from .api import Plan, _ptr, _stream
+from ._lib import check, load
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+MALL_BYTES = 256 << 20
+
+
+def calculate_gflops(nnz: int, time_ms: float, k: int = 1) -> float:
+    return 2.0 * nnz * k / time_ms / 1e6
+
+
+def algorithmic_bytes(R: int, C: int, N: int, bv: int, bx: int, by: int | None = None, k: int = 1,
+                      beta_nonzero: bool = False) -> int:
+    by = bx if by is None else by
+    b = N * (bv + 4) + 4 * (R + 1) + k * C * bx + k * R * by
+    if beta_nonzero:
+        b += k * R * by
+    return int(b)
+
+
+def calculate_bandwidth(bytes_: int, time_ms: float) -> float:
+    return bytes_ / time_ms / 1e6
+
+
+class Events:
+    """A pool of library-owned HIP events."""
+
+    def __init__(self, n: int):
+        self.ev = []
+        for _ in range(n):
+            e = C.c_void_p()
+            check(load().muxi_event_create(C.byref(e)), "event_create")
+            self.ev.append(e)
+
+    def elapsed(self, i: int, j: int) -> float:
+        ms = C.c_float()
+        check(load().muxi_event_elapsed_ms(self.ev[i], self.ev[j], C.byref(ms)), "event_elapsed")
+        return ms.value
+
+    def __del__(self):
+        try:
+            for e in self.ev:
+                load().muxi_event_destroy(e)
+        except Exception:
+            pass
+
+
+@dataclass
+class Measurement:
+    ms_per_step: float          # wall clock over the timed launches / steps
+    kernel_ms: list = field(default_factory=list)  # main kernel per launch (HIP events)
+    steps: int = 0
+
+    @property
+    def kernel_ms_mean(self) -> float:
+        return statistics.fmean(self.kernel_ms) if self.kernel_ms else float("nan")
+
+    @property
+    def kernel_ms_median(self) -> float:
+        return statistics.median(self.kernel_ms) if self.kernel_ms else float("nan")
+
+
+def copies_for_cold(bytes_per_copy: int, floor: int = 3) -> int:
+    """Enough rotating copies that the others evict a copy from the MALL."""
+    need = (2 * MALL_BYTES) // max(1, bytes_per_copy) + 2
+    return int(max(floor, min(64, need)))
+
+
+def measure_plans(plans: list[Plan], xs: list[torch.Tensor], ys: list[torch.Tensor], steps: int, warmup: int,
+                  alpha: float = 1.0, beta: float = 0.0, nrhs: int = 1, record_kernel: bool = True) -> Measurement:
+    """Launches plans[i % n] on xs/ys round-robin: `warmup` untimed, then
+    `steps` timed between two device synchronisations. HIP events around the
+    main kernel of every timed launch give the per-kernel durations."""
+    n = len(plans)
+    s = _stream(None)
+    lib = load()
+    for i in range(warmup):
+        check(lib.muxi_plan_exec(plans[i % n]._p, _ptr(xs[i % n]), _ptr(ys[i % n]), alpha, beta, nrhs, s),
+              "plan_exec")
+    ev = Events(2 * steps) if record_kernel else None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        j = i % n
+        if ev is not None:
+            check(lib.muxi_plan_exec_ev(plans[j]._p, _ptr(xs[j]), _ptr(ys[j]), alpha, beta, nrhs, s,
+                                        ev.ev[2 * i], ev.ev[2 * i + 1]), "plan_exec_ev")
+        else:
+            check(lib.muxi_plan_exec(plans[j]._p, _ptr(xs[j]), _ptr(ys[j]), alpha, beta, nrhs, s), "plan_exec")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    m = Measurement(ms_per_step=(t1 - t0) * 1e3 / steps, steps=steps)
+    if ev is not None:
+        m.kernel_ms = [ev.elapsed(2 * i, 2 * i + 1) for i in range(steps)]
+    return m
+
+
+def write_results_json(path: str, results: list[dict]) -> None:
+    """{results:[{matrix{rows,cols,nnz,sparsity}, device{...}, performance{...}, correctnessPassed}]}"""
+    with open(path, "w") as f:
+        json.dump({"results": results}, f, indent=2)
+
+
+def device_record() -> dict:
+    p = torch.cuda.get_device_properties(0)
+    return {"id": 0, "name": p.name, "warpSize": 64, "computeUnits": p.multi_processor_count,
+            "peakBandwidthGBs": HBM_PEAK_GBS}

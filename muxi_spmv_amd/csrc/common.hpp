@@ -1,0 +1,181 @@
+// Shared device/host helpers for the gfx950 SpMV kernels.
+//
+// Element types: values are stored as float, double or bf16 (uint16 bit
+// pattern); x and y are float or double. Products and sums are formed in an
+// accumulator type AT (float or double). With COMP=true the FP32 accumulator
+// carries an error term: products are split exactly with an FMA (TwoProduct)
+// and sums with Knuth's TwoSum, i.e. the compensated dot product "Dot2" of
+// Ogita, Rump and Oishi, "Accurate sum and dot product", SIAM J. Sci.
+// Comput. 26(6), 2005, Algorithm 5.3.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "muxi_spmv/muxi_spmv.h"
+
+namespace muxi {
+
+constexpr int kWave = 64;
+
+// bf16 storage: the upper 16 bits of an IEEE binary32.
+struct bf16_t {
+    uint16_t bits;
+};
+
+template <typename T> struct TypeOf;
+template <> struct TypeOf<float> { static constexpr spmv_type_t v = SPMV_TYPE_FLOAT; };
+template <> struct TypeOf<double> { static constexpr spmv_type_t v = SPMV_TYPE_DOUBLE; };
+template <> struct TypeOf<bf16_t> { static constexpr spmv_type_t v = SPMV_TYPE_BF16; };
+
+__device__ __forceinline__ float to_f32(float v) { return v; }
+__device__ __forceinline__ float to_f32(double v) { return (float)v; }
+__device__ __forceinline__ float to_f32(bf16_t v) { return __uint_as_float(((uint32_t)v.bits) << 16); }
+__device__ __forceinline__ double to_f64(float v) { return (double)v; }
+__device__ __forceinline__ double to_f64(double v) { return v; }
+__device__ __forceinline__ double to_f64(bf16_t v) { return (double)to_f32(v); }
+
+template <typename AT> struct Conv;
+template <> struct Conv<float> {
+    template <typename T> __device__ __forceinline__ static float f(T v) { return to_f32(v); }
+};
+template <> struct Conv<double> {
+    template <typename T> __device__ __forceinline__ static double f(T v) { return to_f64(v); }
+};
+
+// Round-to-nearest-even binary32 -> bf16 (NaN kept quiet).
+__host__ __device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
+    uint32_t u;
+#ifdef __HIP_DEVICE_COMPILE__
+    u = __float_as_uint(f);
+#else
+    __builtin_memcpy(&u, &f, 4);
+#endif
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40u);
+    uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return (uint16_t)(u >> 16);
+}
+
+// ---------------------------------------------------------------- accumulators
+template <typename AT, bool COMP> struct Acc;
+
+template <typename AT> struct Acc<AT, false> {
+    AT s;
+    __device__ __forceinline__ void zero() { s = AT(0); }
+    template <typename VT, typename XT>
+    __device__ __forceinline__ void fma(VT v, XT xv) { s += Conv<AT>::f(v) * Conv<AT>::f(xv); }
+    __device__ __forceinline__ void add(const Acc& o) { s += o.s; }
+    __device__ __forceinline__ AT value() const { return s; }
+    __device__ __forceinline__ Acc shfl_xor(int m) const { Acc r; r.s = __shfl_xor(s, m); return r; }
+    __device__ __forceinline__ Acc shfl_up(int d) const { Acc r; r.s = __shfl_up(s, d); return r; }
+    __device__ __forceinline__ Acc shfl(int src) const { Acc r; r.s = __shfl(s, src); return r; }
+};
+
+// Dot2: s + c carries the sum to about twice the working precision.
+template <> struct Acc<float, true> {
+    float s, c;
+    __device__ __forceinline__ void zero() { s = 0.f; c = 0.f; }
+    __device__ __forceinline__ void two_sum_add(float a) {
+        float t = __fadd_rn(s, a);
+        float bp = __fsub_rn(t, s);
+        float e = __fadd_rn(__fsub_rn(s, __fsub_rn(t, bp)), __fsub_rn(a, bp));
+        s = t;
+        c = __fadd_rn(c, e);
+    }
+    template <typename VT, typename XT>
+    __device__ __forceinline__ void fma(VT v, XT xv) {
+        float a = to_f32(v), b = to_f32(xv);
+        float p = __fmul_rn(a, b);
+        float ep = __fmaf_rn(a, b, -p);  // exact product error
+        two_sum_add(p);
+        c = __fadd_rn(c, ep);
+    }
+    __device__ __forceinline__ void add(const Acc& o) {
+        two_sum_add(o.s);
+        c = __fadd_rn(c, o.c);
+    }
+    __device__ __forceinline__ float value() const { return __fadd_rn(s, c); }
+    __device__ __forceinline__ Acc shfl_xor(int m) const { Acc r; r.s = __shfl_xor(s, m); r.c = __shfl_xor(c, m); return r; }
+    __device__ __forceinline__ Acc shfl_up(int d) const { Acc r; r.s = __shfl_up(s, d); r.c = __shfl_up(c, d); return r; }
+    __device__ __forceinline__ Acc shfl(int src) const { Acc r; r.s = __shfl(s, src); r.c = __shfl(c, src); return r; }
+};
+
+// Product term stored between phases (LDS staging): one AT, or (p, err) when compensated.
+template <typename AT, bool COMP> struct Prod;
+template <typename AT> struct Prod<AT, false> {
+    AT p;
+    template <typename VT, typename XT>
+    __device__ __forceinline__ static Prod make(VT v, XT xv) { Prod r; r.p = Conv<AT>::f(v) * Conv<AT>::f(xv); return r; }
+    __device__ __forceinline__ static Prod zero() { Prod r; r.p = AT(0); return r; }
+    __device__ __forceinline__ void into(Acc<AT, false>& a) const { a.s += p; }
+};
+template <> struct Prod<float, true> {
+    float p, e;
+    template <typename VT, typename XT>
+    __device__ __forceinline__ static Prod make(VT v, XT xv) {
+        float a = to_f32(v), b = to_f32(xv);
+        Prod r;
+        r.p = __fmul_rn(a, b);
+        r.e = __fmaf_rn(a, b, -r.p);
+        return r;
+    }
+    __device__ __forceinline__ static Prod zero() { Prod r; r.p = 0.f; r.e = 0.f; return r; }
+    __device__ __forceinline__ void into(Acc<float, true>& a) const {
+        a.two_sum_add(p);
+        a.c = __fadd_rn(a.c, e);
+    }
+};
+
+// Butterfly sum over groups of W lanes (W a power of two <= 64); every lane of
+// the group ends with the same total, combined in a fixed order.
+template <int W, typename A>
+__device__ __forceinline__ A group_reduce(A a) {
+#pragma unroll
+    for (int m = W / 2; m >= 1; m >>= 1) {
+        A o = a.shfl_xor(m);
+        a.add(o);
+    }
+    return a;
+}
+
+// y[i] = alpha*v + beta*y[i]; beta == 0 never reads y (so y may hold NaN).
+template <typename YT, typename AT>
+__device__ __forceinline__ void store_y(YT* y, long i, AT v, AT alpha, AT beta) {
+    AT r = alpha * v;
+    if (beta != AT(0)) r += beta * (AT)y[i];
+    y[i] = (YT)r;
+}
+
+}  // namespace muxi
+
+// ------------------------------------------------------------------ host side
+namespace muxi {
+
+void set_error(const std::string& msg);
+const char* get_error();
+
+#define MUXI_HIP(call)                                                           \
+    do {                                                                         \
+        hipError_t e_ = (call);                                                  \
+        if (e_ != hipSuccess) {                                                  \
+            ::muxi::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return SPMV_ERROR_DEVICE;                                            \
+        }                                                                        \
+    } while (0)
+
+#define MUXI_LAUNCH_CHECK()                                                        \
+    do {                                                                           \
+        hipError_t e_ = hipGetLastError();                                         \
+        if (e_ != hipSuccess) {                                                    \
+            ::muxi::set_error(std::string("kernel launch: ") + hipGetErrorString(e_)); \
+            return SPMV_ERROR_DEVICE;                                              \
+        }                                                                          \
+    } while (0)
+
+inline unsigned grid_for(long work, int per_block) {
+    return (unsigned)((work + per_block - 1) / per_block);
+}
+
+}  // namespace muxi

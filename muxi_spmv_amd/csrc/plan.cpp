@@ -1,0 +1,285 @@
+// Plan API: build once (partitions, tiles, ELL slots, carries), execute many
+// times without allocating, so an exec may be captured in a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <new>
+
+#include "plan.hpp"
+
+using namespace muxi;
+
+static void plan_free(muxi_plan* p) {
+    if (!p) return;
+    (void)hipFree(p->d_pos);
+    (void)hipFree(p->d_tile_row);
+    (void)hipFree(p->d_carry);
+    (void)hipFree(p->d_ecol);
+    (void)hipFree(p->d_eval);
+    (void)hipFree(p->d_blocks);
+    delete p;
+}
+
+#define PLAN_TRY(expr)                     \
+    do {                                   \
+        spmv_status_t st_ = (expr);        \
+        if (st_ != SPMV_SUCCESS) {         \
+            plan_free(p);                  \
+            return st_;                    \
+        }                                  \
+    } while (0)
+
+#define PLAN_HIP(expr)                                                         \
+    do {                                                                       \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess) {                                                \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));      \
+            plan_free(p);                                                      \
+            return SPMV_ERROR_DEVICE;                                          \
+        }                                                                      \
+    } while (0)
+
+static bool pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; }
+
+extern "C" spmv_status_t muxi_plan_create(const muxi_csr_t* Ain, spmv_type_t vt, spmv_type_t xt, muxi_kernel_t kernel,
+                                          int param0, int compensated, void* stream, muxi_plan** out) {
+    if (!out) {
+        set_error("plan output pointer is NULL");
+        return SPMV_ERROR_INVALID_HANDLE;
+    }
+    *out = nullptr;
+    if (!Ain) {
+        set_error("matrix is NULL");
+        return SPMV_ERROR_INVALID_MATRIX;
+    }
+    if (Ain->numRows < 0 || Ain->numCols < 0 || Ain->nnz < 0 ||
+        (long)Ain->numRows + (long)Ain->nnz > 2147483647L) {
+        set_error("invalid matrix dimensions");
+        return SPMV_ERROR_INVALID_MATRIX;
+    }
+    if (Ain->numRows > 0 && !Ain->d_rowPtr) {
+        set_error("rowPtr device array is NULL");
+        return SPMV_ERROR_INVALID_MATRIX;
+    }
+    if (Ain->nnz > 0 && (!Ain->d_colIdx || !Ain->d_values)) {
+        set_error("matrix device arrays are NULL");
+        return SPMV_ERROR_INVALID_MATRIX;
+    }
+    // type check up front
+    spmv_status_t tst = with_types(vt, xt, compensated != 0, [](auto) { return SPMV_SUCCESS; });
+    if (tst != SPMV_SUCCESS) return tst;
+
+    hipStream_t s = (hipStream_t)stream;
+    muxi_plan* p = new (std::nothrow) muxi_plan();
+    if (!p) return SPMV_ERROR_MEMORY_ALLOC;
+    p->A = DevCsr{Ain->numRows, Ain->numCols, Ain->nnz, Ain->d_rowPtr, Ain->d_colIdx, Ain->d_values};
+    p->vtype = vt;
+    p->xtype = xt;
+    p->comp = compensated != 0;
+    PLAN_HIP(hipGetDevice(&p->device));
+    const int R = Ain->numRows, N = Ain->nnz;
+
+    if (kernel == MUXI_KERNEL_AUTO) {
+        kernel = MUXI_KERNEL_MERGE;
+        param0 = 8;
+    }
+    p->kernel = kernel;
+    p->param0 = param0;
+    const size_t acc_bytes = xt == SPMV_TYPE_DOUBLE ? 8 : (p->comp ? 8 : 4);
+
+    switch (kernel) {
+        case MUXI_KERNEL_SCALAR:
+        case MUXI_KERNEL_VECTOR:
+            break;
+        case MUXI_KERNEL_VWAVE:
+            if (param0 <= 0) param0 = 8;
+            if (!pow2_in(param0, 1, 64)) {
+                set_error("vwave: lanes per row must be a power of two in [1, 64]");
+                plan_free(p);
+                return SPMV_ERROR_INTERNAL;
+            }
+            p->param0 = param0;
+            break;
+        case MUXI_KERNEL_MERGE: {
+            if (param0 <= 0) param0 = 8;
+            if (!(param0 == 2 || param0 == 4 || param0 == 8 || param0 == 16)) {
+                set_error("merge: items per thread must be 2, 4, 8 or 16");
+                plan_free(p);
+                return SPMV_ERROR_INTERNAL;
+            }
+            p->param0 = param0;
+            const int P = merge_default_partitions(R + N, param0);
+            p->P = P;
+            PLAN_HIP(hipMalloc(&p->d_pos, 2 * sizeof(int) * (size_t)(P + 1)));
+            p->d_row = p->d_pos + (P + 1);
+            PLAN_HIP(hipMalloc(&p->d_carry, acc_bytes * (size_t)P));
+            p->scratch_bytes = 2 * sizeof(int) * (size_t)(P + 1) + acc_bytes * (size_t)P;
+            PLAN_TRY(launch_merge_partitions(R, N, P, p->d_pos, s));
+            PLAN_TRY(launch_merge_coords(p->A.rp, R, N, p->d_pos, P + 1, p->d_row, nullptr, s));
+            break;
+        }
+        case MUXI_KERNEL_CSR5: {
+            if (param0 <= 0) param0 = 256;
+            if (!csr5_sigma_supported(param0)) {
+                set_error("csr5: sigma must be 64, 128, 256, 512 or 1024");
+                plan_free(p);
+                return SPMV_ERROR_INTERNAL;
+            }
+            p->param0 = param0;
+            const int T = (int)(((long)N + param0 - 1) / param0);
+            p->T = T;
+            if (T > 0) {
+                PLAN_HIP(hipMalloc(&p->d_tile_row, 2 * sizeof(int) * (size_t)T));
+                p->d_tile_off = p->d_tile_row + T;
+                PLAN_HIP(hipMalloc(&p->d_carry, acc_bytes * (size_t)T));
+                p->scratch_bytes = 2 * sizeof(int) * (size_t)T + acc_bytes * (size_t)T;
+                PLAN_TRY(launch_csr5_tiles(p->A.rp, R, N, param0, T, p->d_tile_row, p->d_tile_off, s));
+            }
+            break;
+        }
+        case MUXI_KERNEL_ELL: {
+            int width = param0;
+            if (width <= 0) {
+                int* d_w = nullptr;
+                PLAN_HIP(hipMalloc(&d_w, sizeof(int)));
+                spmv_status_t st = launch_max_row_len(p->A.rp, R, d_w, s);
+                if (st == SPMV_SUCCESS) {
+                    if (hipMemcpyAsync(&width, d_w, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                        hipStreamSynchronize(s) != hipSuccess)
+                        st = SPMV_ERROR_MEMORY_COPY;
+                }
+                (void)hipFree(d_w);
+                PLAN_TRY(st);
+            }
+            if ((long)R * (long)width > 2147483647L) {
+                set_error("ELLPACK numRows*width exceeds INT32_MAX");
+                plan_free(p);
+                return SPMV_ERROR_INVALID_MATRIX;
+            }
+            p->width = width;
+            p->param0 = width;
+            const size_t slots = (size_t)R * (size_t)width;
+            if (slots > 0) {
+                PLAN_HIP(hipMalloc(&p->d_ecol, sizeof(int) * slots));
+                PLAN_HIP(hipMalloc(&p->d_eval, vsize(vt) * slots));
+                p->scratch_bytes = (sizeof(int) + vsize(vt)) * slots;
+                PLAN_TRY(with_types(vt, xt, p->comp, [&](auto tag) {
+                    using T = decltype(tag);
+                    return launch_csr_to_ell<typename T::VT>(p->A, width, p->d_ecol, (typename T::VT*)p->d_eval, s);
+                }));
+            }
+            break;
+        }
+        default:
+            set_error("unknown kernel id");
+            plan_free(p);
+            return SPMV_ERROR_INTERNAL;
+    }
+    *out = p;
+    return SPMV_SUCCESS;
+}
+
+extern "C" spmv_status_t muxi_plan_destroy(muxi_plan* p) {
+    if (!p) {
+        set_error("plan is NULL");
+        return SPMV_ERROR_INVALID_HANDLE;
+    }
+    plan_free(p);
+    return SPMV_SUCCESS;
+}
+
+extern "C" spmv_status_t muxi_plan_info(const muxi_plan* p, int* kernel, int* param0, size_t* scratch) {
+    if (!p) {
+        set_error("plan is NULL");
+        return SPMV_ERROR_INVALID_HANDLE;
+    }
+    if (kernel) *kernel = (int)p->kernel;
+    if (param0) *param0 = p->param0;
+    if (scratch) *scratch = p->scratch_bytes;
+    return SPMV_SUCCESS;
+}
+
+static spmv_status_t plan_exec(const muxi_plan* p, const void* x, void* y, double alpha, double beta, int nrhs,
+                               hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    if (!p) {
+        set_error("plan is NULL");
+        return SPMV_ERROR_INVALID_HANDLE;
+    }
+    if (!x || !y) {
+        set_error("x or y is NULL");
+        return SPMV_ERROR_INVALID_VECTOR;
+    }
+    if (nrhs != 1) {
+        set_error("nrhs > 1 is served by muxi_spmm_exec");
+        return SPMV_ERROR_UNSUPPORTED_TYPE;
+    }
+    if (ev0) MUXI_HIP(hipEventRecord(ev0, s));
+    spmv_status_t st = with_types(p->vtype, p->xtype, p->comp, [&](auto tag) -> spmv_status_t {
+        using T = decltype(tag);
+        using VT = typename T::VT;
+        using XT = typename T::XT;
+        using AT = typename T::AT;
+        constexpr bool C = T::COMP;
+        const XT* xx = (const XT*)x;
+        XT* yy = (XT*)y;
+        AT a = (AT)alpha, b = (AT)beta;
+        switch (p->kernel) {
+            case MUXI_KERNEL_SCALAR: return launch_scalar<VT, XT, AT, C>(p->A, xx, yy, a, b, s);
+            case MUXI_KERNEL_VECTOR: return launch_vector<VT, XT, AT, C>(p->A, xx, yy, a, b, s);
+            case MUXI_KERNEL_VWAVE: return launch_vwave<VT, XT, AT, C>(p->A, p->param0, xx, yy, a, b, s);
+            case MUXI_KERNEL_MERGE:
+                return launch_merge<VT, XT, AT, C>(p->A, p->param0, p->d_pos, p->d_row, p->P, p->d_carry, xx, yy, a,
+                                                   b, s, ev1);
+            case MUXI_KERNEL_CSR5:
+                if (p->T == 0) {
+                    // no nonzeros: every row is empty
+                    return launch_scalar<VT, XT, AT, C>(p->A, xx, yy, a, b, s);
+                }
+                return launch_csr5<VT, XT, AT, C>(p->A, p->param0, p->d_tile_row, p->T, p->d_carry, xx, yy, a, b, s,
+                                                  ev1);
+            case MUXI_KERNEL_ELL:
+                return launch_ell<VT, XT, AT, C>(p->A.R, p->width, p->d_ecol, (const VT*)p->d_eval, xx, yy, a, b, s);
+            default: set_error("plan kernel not executable"); return SPMV_ERROR_INTERNAL;
+        }
+    });
+    if (st != SPMV_SUCCESS) return st;
+    // single-kernel paths: the main kernel ends with the exec
+    if (ev1 && !((p->kernel == MUXI_KERNEL_MERGE && p->A.R > 0) || (p->kernel == MUXI_KERNEL_CSR5 && p->T > 0)))
+        MUXI_HIP(hipEventRecord(ev1, s));
+    return SPMV_SUCCESS;
+}
+
+extern "C" spmv_status_t muxi_plan_exec(const muxi_plan* p, const void* x, void* y, double alpha, double beta,
+                                        int nrhs, void* stream) {
+    return plan_exec(p, x, y, alpha, beta, nrhs, (hipStream_t)stream, nullptr, nullptr);
+}
+
+extern "C" spmv_status_t muxi_plan_exec_ev(const muxi_plan* p, const void* x, void* y, double alpha, double beta,
+                                           int nrhs, void* stream, void* ev_before, void* ev_after) {
+    return plan_exec(p, x, y, alpha, beta, nrhs, (hipStream_t)stream, (hipEvent_t)ev_before, (hipEvent_t)ev_after);
+}
+
+extern "C" spmv_status_t muxi_event_create(void** ev) {
+    if (!ev) return SPMV_ERROR_INVALID_HANDLE;
+    MUXI_HIP(hipEventCreate((hipEvent_t*)ev));
+    return SPMV_SUCCESS;
+}
+
+extern "C" spmv_status_t muxi_event_destroy(void* ev) {
+    if (!ev) return SPMV_ERROR_INVALID_HANDLE;
+    MUXI_HIP(hipEventDestroy((hipEvent_t)ev));
+    return SPMV_SUCCESS;
+}
+
+extern "C" spmv_status_t muxi_event_record(void* ev, void* stream) {
+    if (!ev) return SPMV_ERROR_INVALID_HANDLE;
+    MUXI_HIP(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+    return SPMV_SUCCESS;
+}
+
+extern "C" spmv_status_t muxi_event_elapsed_ms(void* start, void* end, float* ms) {
+    if (!start || !end || !ms) return SPMV_ERROR_INVALID_HANDLE;
+    MUXI_HIP(hipEventSynchronize((hipEvent_t)end));
+    MUXI_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
+    return SPMV_SUCCESS;
+}

@@ -1,0 +1,70 @@
+// Plan object: one CSR matrix bound to one kernel with its tile parameters
+// and every scratch buffer that kernel needs.
+#pragma once
+
+#include "kernels.hpp"
+
+struct muxi_plan {
+    muxi::DevCsr A;
+    spmv_type_t vtype, xtype;
+    bool comp;
+    muxi_kernel_t kernel;
+    int param0;
+    int device;
+    // merge-path
+    int P = 0;
+    int* d_pos = nullptr;  // P+1 diagonals, then P+1 row coordinates
+    int* d_row = nullptr;
+    // CSR5
+    int T = 0;
+    int* d_tile_row = nullptr;  // T, then T offsets
+    int* d_tile_off = nullptr;
+    // merge / CSR5 carries
+    void* d_carry = nullptr;
+    // ELL
+    int width = 0;
+    int* d_ecol = nullptr;
+    void* d_eval = nullptr;
+    // adaptive row blocks
+    int nblocks = 0;
+    int* d_blocks = nullptr;
+    size_t scratch_bytes = 0;
+};
+
+namespace muxi {
+
+template <typename V, typename X, typename A, bool C>
+struct Tag {
+    using VT = V;
+    using XT = X;
+    using AT = A;
+    static constexpr bool COMP = C;
+};
+
+// Calls f(Tag<...>{}) for the element-type combination, or returns
+// SPMV_ERROR_UNSUPPORTED_TYPE.
+template <typename F>
+spmv_status_t with_types(spmv_type_t vt, spmv_type_t xt, bool comp, F&& f) {
+    if (xt == SPMV_TYPE_FLOAT) {
+        if (vt == SPMV_TYPE_FLOAT)
+            return comp ? f(Tag<float, float, float, true>{}) : f(Tag<float, float, float, false>{});
+        if (vt == SPMV_TYPE_BF16)
+            return comp ? f(Tag<bf16_t, float, float, true>{}) : f(Tag<bf16_t, float, float, false>{});
+        if (vt == SPMV_TYPE_DOUBLE)
+            return comp ? f(Tag<double, float, float, true>{}) : f(Tag<double, float, float, false>{});
+    } else if (xt == SPMV_TYPE_DOUBLE && vt == SPMV_TYPE_DOUBLE && !comp) {
+        return f(Tag<double, double, double, false>{});
+    }
+    set_error("unsupported (value type, x type, compensated) combination");
+    return SPMV_ERROR_UNSUPPORTED_TYPE;
+}
+
+inline int merge_default_partitions(int L, int ipt) {
+    long ch = 256L * ipt;
+    long p = ((long)L + ch - 1) / ch;
+    return (int)(p > 1 ? p : 1);
+}
+
+inline size_t vsize(spmv_type_t t) { return t == SPMV_TYPE_DOUBLE ? 8 : (t == SPMV_TYPE_BF16 ? 2 : 4); }
+
+}  // namespace muxi

@@ -1,0 +1,117 @@
+"""The C-ABI library loads on a CPU host, exports every declared symbol, and
+its host-side error paths return the reference's status codes without
+touching a GPU."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+from muxi_spmv_amd import _lib
+import oracle
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    names = set()
+    for h in (ROOT / "include").rglob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names |= set(re.findall(r"\b(muxi_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    decl = declared_functions()
+    assert len(decl) > 30
+    missing = [n for n in decl if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the ctypes table covers the header exactly
+    assert set(decl) == set(_lib.SIGNATURES), set(decl) ^ set(_lib.SIGNATURES)
+
+
+def test_enum_values_match_reference():
+    # src/utils/common.h:32-71
+    assert _lib.STATUS_NAMES[-1] == "SPMV_ERROR_INVALID_HANDLE"
+    assert _lib.STATUS_NAMES[-9] == "SPMV_ERROR_INTERNAL"
+    assert (_lib.SPMV_FORMAT_CSR, _lib.SPMV_FORMAT_CSR5) == (0, 4)
+    assert (_lib.SPMV_TYPE_FLOAT, _lib.SPMV_TYPE_DOUBLE, _lib.SPMV_TYPE_BF16) == (0, 1, 3)
+
+
+def test_struct_layouts():
+    # field order of CSRMatrix<T> (sparse_formats.h:26-40): 3 ints, 6 pointers, 2 bools
+    assert [f[0] for f in _lib.muxi_csr_t._fields_][:6] == ["numRows", "numCols", "nnz", "rowPtr", "colIdx",
+                                                             "values"]
+    assert C.sizeof(_lib.muxi_csr_t) == 4 * 3 + 4 + 8 * 6 + 8
+    assert C.sizeof(_lib.spmv_opts_t) == 6 * 4 + 8
+
+
+def test_default_opts():
+    o = _lib.load().muxi_spmv_default_opts()
+    assert (o.operation, o.sync, o.check_inputs, o.use_tensor_core, o.use_streaming, o.stream_id) == (0,) * 6
+    assert not o.stream
+
+
+def test_null_handle_errors():
+    lib = _lib.load()
+    assert lib.muxi_spmv_create_handle(None) == -1
+    assert lib.muxi_spmv_destroy_handle(None) == -1
+    assert lib.muxi_spmv_set_stream(None, None) == -1
+    assert b"NULL" in lib.muxi_spmv_last_error()
+    assert lib.muxi_spmv_exec_f32(None, 0, None, None, None, 1.0, 0.0) == -1
+
+
+def test_null_matrix_and_vector_errors():
+    lib = _lib.load()
+    assert lib.muxi_spmv_csr_f32(None, None, None, 1.0, 0.0, None) == -2
+    assert lib.muxi_spmv_merge_based_f64(None, None, None, None) == -2
+    a = _lib.muxi_csr_t()
+    a.numRows, a.numCols, a.nnz = 10, 10, 5  # device arrays missing
+    assert lib.muxi_spmv_csr_f32(C.byref(a), None, None, 1.0, 0.0, None) == -2
+    p = C.c_void_p()
+    assert lib.muxi_plan_create(C.byref(a), 0, 0, 5, 8, 0, None, C.byref(p)) == -2
+    assert lib.muxi_plan_create(None, 0, 0, 5, 8, 0, None, C.byref(p)) == -2
+    assert lib.muxi_plan_exec(None, None, None, 1.0, 0.0, 1, None) == -1
+    assert lib.muxi_csr5_preprocess(None, None, 0, None) == -2
+    assert lib.muxi_spmv_ellpack_f32(None, None, None, None) == -2
+
+
+def test_nnz_zero_is_success_without_device_work():
+    # spmv_csr.cu:61-63, spmv_csr5.cu:229-231 / :285-287: nnz == 0 returns success, y untouched
+    lib = _lib.load()
+    a = _lib.muxi_csr_t()
+    a.numRows, a.numCols, a.nnz = 4, 4, 0
+    assert lib.muxi_spmv_csr_f32(C.byref(a), None, None, 1.0, 0.0, None) == 0
+    assert lib.muxi_spmv_merge_based_f32(C.byref(a), None, None, None) == 0
+    c5 = _lib.muxi_csr5_t()
+    assert lib.muxi_csr5_preprocess(C.byref(a), C.byref(c5), 0, None) == 0
+    assert c5.numTiles == 0
+    assert lib.muxi_spmv_csr5_f32(C.byref(c5), None, None, 1.0, 0.0, None) == 0
+
+
+def test_int32_overflow_guard():
+    lib = _lib.load()
+    a = _lib.muxi_csr_t()
+    a.numRows, a.numCols, a.nnz = 2_000_000_000, 10, 500_000_000
+    a.d_rowPtr = a.d_colIdx = a.d_values = 16  # never dereferenced: rejected first
+    assert lib.muxi_spmv_csr_f32(C.byref(a), C.c_void_p(16), C.c_void_p(16), 1.0, 0.0, None) == -2
+    assert b"INT32_MAX" in lib.muxi_spmv_last_error()
+
+
+def test_unsupported_type_combination():
+    lib = _lib.load()
+    a = _lib.muxi_csr_t()
+    a.numRows, a.numCols, a.nnz = 4, 4, 0
+    a.d_rowPtr = 16
+    p = C.c_void_p()
+    # bf16 values with FP64 x/accumulation is not built
+    assert lib.muxi_plan_create(C.byref(a), _lib.SPMV_TYPE_BF16, _lib.SPMV_TYPE_DOUBLE, 5, 8, 0, None,
+                                C.byref(p)) == -8
+
+
+@pytest.mark.parametrize("R,N,ePP,mult", [(2_000_000, 14_700_000, 16, 8), (100_000, 1_000_000, 16, 8),
+                                          (10, 5, 16, 8), (2_000_000, 14_700_000, 64, 4), (0, 0, 16, 8)])
+def test_partition_count_matches_oracle(R, N, ePP, mult):
+    assert _lib.load().muxi_merge_partition_count(R, N, ePP, mult, 104) == \
+        oracle.merge_partition_count(R, N, ePP, mult, 104, 4)

@@ -1,0 +1,259 @@
+"""GPU parity: every kernel and reference entry point through the C-ABI
+against the CPU oracle. Tolerances: integer outputs bit-exact; FP32 results
+within the per-row bound (n_r + 2) * 2^-24 * sum_j |a_rj x_j| of the FP64
+oracle and a relative L2 error <= 1e-6; FP64 results within
+(n_r + 2) * 2^-53 * sum_j |a_rj x_j|."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from muxi_spmv_amd import api
+from muxi_spmv_amd import generators as G
+from gpu_util import assert_close_fp32, assert_close_fp64, dev_csr, matrix_zoo
+
+pytestmark = pytest.mark.gpu
+
+ZOO = matrix_zoo()
+KCONF = [("scalar", 0), ("vector", 0)] + [("vwave", k) for k in (1, 2, 4, 8, 16, 32, 64)] + \
+        [("merge", k) for k in (2, 4, 8, 16)] + [("csr5", s) for s in (64, 128, 256, 512, 1024)] + [("ell", 0)]
+
+
+def xvec(n, seed=11, dtype=np.float32):
+    return G.uniform_vector(n, seed, dtype)
+
+
+def run_plan(h, kernel, param, vdtype=torch.float32, xdtype=torch.float32, comp=False, x=None, alpha=1.0,
+             beta=0.0, y0=None):
+    A = dev_csr(h, vdtype)
+    xh = xvec(h.numCols, dtype=np.float64 if xdtype == torch.float64 else np.float32) if x is None else x
+    xd = torch.from_numpy(xh).cuda()
+    y = torch.full((h.numRows,), float("nan"), dtype=xdtype, device="cuda") if y0 is None else y0.clone()
+    p = api.Plan(A, kernel, param, xtype=xdtype, compensated=comp)
+    p.exec(xd, y, alpha, beta)
+    torch.cuda.synchronize()
+    return y, xh, A, p
+
+
+@pytest.mark.parametrize("mname", list(ZOO))
+@pytest.mark.parametrize("kernel,param", KCONF)
+def test_plan_fp32(mname, kernel, param):
+    h = ZOO[mname]
+    if kernel == "ell" and h.row_lengths().max(initial=0) * h.numRows > 50_000_000:
+        pytest.skip("ELL padding too large for this structure")
+    y, x, _, _ = run_plan(h, kernel, param)
+    assert_close_fp32(y, h, x)
+
+
+@pytest.mark.parametrize("mname", ["uniform", "powerlaw", "empty_rows", "long_row", "tiny"])
+@pytest.mark.parametrize("kernel,param", [("scalar", 0), ("vwave", 8), ("merge", 8), ("csr5", 256), ("ell", 0)])
+def test_plan_fp64(mname, kernel, param):
+    h = ZOO[mname]
+    if kernel == "ell" and h.row_lengths().max(initial=0) * h.numRows > 50_000_000:
+        pytest.skip("ELL padding too large")
+    h64 = G.HostCSR(h.rowPtr, h.colIdx, h.values.astype(np.float64), h.numRows, h.numCols)
+    y, x, _, _ = run_plan(h64, kernel, param, torch.float64, torch.float64)
+    assert_close_fp64(y, h64, x)
+
+
+@pytest.mark.parametrize("mname", ["uniform", "powerlaw", "long_row", "empty_rows"])
+@pytest.mark.parametrize("kernel,param", [("vector", 0), ("vwave", 4), ("merge", 8), ("csr5", 256), ("ell", 0)])
+@pytest.mark.parametrize("vdtype", [torch.bfloat16, torch.float64])
+@pytest.mark.parametrize("comp", [False, True])
+def test_plan_mixed_precision(mname, kernel, param, vdtype, comp):
+    h = ZOO[mname]
+    if kernel == "ell" and h.row_lengths().max(initial=0) * h.numRows > 50_000_000:
+        pytest.skip("ELL padding too large")
+    y, x, A, _ = run_plan(h, kernel, param, vdtype, torch.float32, comp)
+    stored = A.d_values.double().cpu().numpy()  # the values the kernel actually read
+    assert_close_fp32(y, h, x, values=stored)
+
+
+@pytest.mark.parametrize("kernel,param", [("scalar", 0), ("vwave", 8), ("merge", 8), ("csr5", 256)])
+def test_compensated_beats_plain_on_long_rows(kernel, param):
+    # long rows with huge cancellation: Dot2 must be at least as good, and near FP64
+    rng = np.random.default_rng(99)
+    R, L = 64, 20000
+    lens = np.full(R, L, np.int64)
+    cols = G._distinct_sorted(lens, np.zeros(R, np.int64), np.full(R, 40000, np.int64), rng)
+    vals = (rng.standard_normal(cols.size) * 10.0 ** rng.integers(-3, 4, cols.size)).astype(np.float32)
+    h = G._finish(lens, cols, vals, R, 40000)
+    x = rng.standard_normal(40000).astype(np.float32)
+    yp, *_ = run_plan(h, kernel, param, x=x)
+    yc, *_ = run_plan(h, kernel, param, comp=True, x=x)
+    y64, _ = oracle.spmv_f64(h.rowPtr, h.colIdx, h.values.astype(np.float64), x.astype(np.float64))
+    ep = np.abs(yp.double().cpu().numpy() - y64)
+    ec = np.abs(yc.double().cpu().numpy() - y64)
+    # compensated result is the FP32 rounding of a ~2x-precision sum: within 1 ulp of y64
+    assert np.all(ec <= np.abs(y64) * 2 ** -23 + 1e-30)
+    assert ec.sum() <= ep.sum()
+
+
+@pytest.mark.parametrize("kernel,param", [("scalar", 0), ("vwave", 16), ("merge", 4), ("csr5", 512), ("ell", 0)])
+def test_alpha_beta(kernel, param):
+    h = ZOO["uniform"]
+    y0 = torch.from_numpy(xvec(h.numRows, seed=5)).cuda()
+    y, x, _, _ = run_plan(h, kernel, param, alpha=-0.5, beta=2.0, y0=y0)
+    y64, _ = oracle.spmv_f64(h.rowPtr, h.colIdx, h.values.astype(np.float64), x.astype(np.float64))
+    exp = -0.5 * y64 + 2.0 * y0.double().cpu().numpy()
+    assert np.max(np.abs(y.double().cpu().numpy() - exp)) < 1e-5
+
+
+@pytest.mark.parametrize("kernel,param", [("merge", 8), ("csr5", 256), ("vwave", 8)])
+def test_bitwise_deterministic(kernel, param):
+    h = ZOO["powerlaw"]
+    y1, x, A, p = run_plan(h, kernel, param)
+    y2 = torch.empty_like(y1)
+    for _ in range(3):
+        p.exec(torch.from_numpy(x).cuda(), y2)
+        torch.cuda.synchronize()
+        assert torch.equal(y1, y2)
+
+
+# ---------------------------------------------------------------- reference entry points
+def test_known_answer_diagonal():
+    # tests/correctness/test_spmv.cu: n=1024 diagonal, x[i] = i+1, y == x within 1e-5, via spmv_csr
+    n = 1024
+    A = api.CSRMatrix(np.arange(n + 1), np.arange(n), torch.ones(n), n)
+    x = torch.arange(1, n + 1, dtype=torch.float32, device="cuda")
+    y = torch.zeros(n, device="cuda")
+    api.spmv_csr(A, x, y, 1.0, 0.0, api.default_opts(sync=1))
+    assert torch.max(torch.abs(y - x)).item() <= 1e-5
+    h = api.Handle()
+    assert h.fields.warpSize == 64
+    y2 = torch.zeros(n, device="cuda")
+    h.exec(0, A, x, y2)
+    torch.cuda.synchronize()
+    assert torch.equal(y, y2)
+
+
+@pytest.mark.parametrize("mname", ["uniform", "powerlaw", "empty_rows", "long_row", "tiny", "single_row"])
+def test_reference_entry_points_fp32(mname):
+    h = ZOO[mname]
+    A = dev_csr(h)
+    x = xvec(h.numCols)
+    xd = torch.from_numpy(x).cuda()
+    for fn in (lambda y: api.spmv_csr(A, xd, y), lambda y: api.spmv_merge_based(A, xd, y),
+               lambda y: api.spmv_merge_based_v2(A, xd, y, useOptimized=True),
+               lambda y: api.spmv_merge_based_v2(A, xd, y, useOptimized=False),
+               lambda y: api.spmv_sparse_optimized(A, xd, y)):
+        y = torch.full((h.numRows,), float("nan"), device="cuda")
+        fn(y)
+        torch.cuda.synchronize()
+        assert_close_fp32(y, h, x)
+
+
+@pytest.mark.parametrize("mname", ["uniform", "powerlaw", "empty_rows", "long_row"])
+@pytest.mark.parametrize("sigma", [0, 64, 256, 512, 1024, 100, 777])
+def test_csr5_preprocess_bit_exact_and_spmv(mname, sigma):
+    h = ZOO[mname]
+    A = dev_csr(h)
+    m = api.csr5_preprocess(A, sigma)
+    s = 256 if sigma <= 0 else sigma
+    assert m.sigma == s
+    tr, to = oracle.csr5_tiles(h.rowPtr, h.nnz, s)
+    assert m.numTiles == tr.size
+    assert np.array_equal(m.tile_row_ptr(), tr)
+    assert np.array_equal(m.tile_nnz_offset(), to)
+    assert m.conversionTimeMs >= 0
+    x = xvec(h.numCols)
+    y = torch.full((h.numRows,), float("nan"), device="cuda")
+    api.spmv_csr5(m, torch.from_numpy(x).cuda(), y, 1.0, 0.0, api.default_opts(sync=1))
+    assert_close_fp32(y, h, x)
+
+
+@pytest.mark.parametrize("mname", ["uniform", "empty_rows", "tiny", "banded"])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_ellpack_bit_exact_and_spmv(mname, dtype):
+    h = ZOO[mname]
+    A = dev_csr(h, dtype)
+    E = api.csr_to_ellpack(A)
+    w = oracle.ell_width(h.rowPtr)
+    assert E.numElements == w
+    ec, ev = oracle.csr_to_ell(h.rowPtr, h.colIdx, torch.from_numpy(h.values).to(dtype).double().numpy(), w)
+    col, val = E.arrays()
+    assert np.array_equal(col.cpu().numpy(), ec)
+    assert np.array_equal(val.double().cpu().numpy(), ev)
+    x = xvec(h.numCols, dtype=np.float64 if dtype == torch.float64 else np.float32)
+    y = torch.full((h.numRows,), float("nan"), dtype=dtype, device="cuda")
+    api.spmv_ellpack(E, torch.from_numpy(x).cuda(), y)
+    torch.cuda.synchronize()
+    if dtype == torch.float32:
+        assert_close_fp32(y, h, x)
+    else:
+        assert_close_fp64(y, G.HostCSR(h.rowPtr, h.colIdx, h.values.astype(np.float64), h.numRows, h.numCols), x)
+
+
+def test_ellpack_truncating_width():
+    h = ZOO["uniform"]
+    E = api.csr_to_ellpack(dev_csr(h), 7)
+    ec, ev = oracle.csr_to_ell(h.rowPtr, h.colIdx, h.values.astype(np.float64), 7)
+    col, val = E.arrays()
+    assert np.array_equal(col.cpu().numpy(), ec) and np.array_equal(val.double().cpu().numpy(), ev)
+
+
+@pytest.mark.parametrize("mname", ["uniform", "powerlaw", "empty_rows", "long_row", "tiny"])
+def test_merge_plan_bit_exact(mname):
+    h = ZOO[mname]
+    A = dev_csr(h)
+    for P in {api.merge_partition_count(h.numRows, h.nnz), api.merge_partition_count(h.numRows, h.nnz, 64, 4),
+              1, 3, h.numRows + h.nnz}:
+        P = max(1, P)
+        pos = api.merge_partitions(h.numRows, h.nnz, P)
+        assert np.array_equal(pos.cpu().numpy(), oracle.merge_partitions(h.numRows, h.nnz, P))
+        r, n = api.merge_path_coords(A, pos)
+        er, en = oracle.merge_path_coords(h.rowPtr, h.nnz, pos.cpu().numpy())
+        assert np.array_equal(r.cpu().numpy(), er) and np.array_equal(n.cpu().numpy(), en)
+
+
+def test_fp64_reference_entry_points():
+    h = G.laplacian_2d(64)
+    A = dev_csr(h, torch.float64)
+    x = xvec(h.numCols, dtype=np.float64)
+    xd = torch.from_numpy(x).cuda()
+    for fn in (lambda y: api.spmv_csr(A, xd, y), lambda y: api.spmv_merge_based(A, xd, y),
+               lambda y: api.spmv_csr5(api.csr5_preprocess(A), xd, y)):
+        y = torch.full((h.numRows,), float("nan"), dtype=torch.float64, device="cuda")
+        fn(y)
+        torch.cuda.synchronize()
+        assert_close_fp64(y, h, x)
+
+
+def test_transpose():
+    h = ZOO["uniform"]
+    A = dev_csr(h)
+    x = xvec(h.numRows, seed=3)
+    y = torch.zeros(h.numCols, device="cuda")
+    api.spmv_csr(A, torch.from_numpy(x).cuda(), y, 1.0, 0.0, api.default_opts(sync=1, operation=1))
+    dense = np.zeros((h.numRows, h.numCols))
+    rows = np.repeat(np.arange(h.numRows), h.row_lengths())
+    dense[rows, h.colIdx] = h.values
+    exp = dense.T @ x.astype(np.float64)
+    assert np.max(np.abs(y.double().cpu().numpy() - exp)) < 1e-4
+
+
+def test_sparse_optimized_zero_nnz_zeroes_y():
+    A = api.CSRMatrix(np.zeros(11, np.int32), np.zeros(0, np.int32), torch.zeros(0), 10)
+    y = torch.full((10,), 5.0, device="cuda")
+    api.spmv_sparse_optimized(A, torch.ones(10, device="cuda"), y)
+    torch.cuda.synchronize()
+    assert torch.all(y == 0)
+    y = torch.full((10,), 5.0, device="cuda")
+    api.spmv_csr(A, torch.ones(10, device="cuda"), y)  # spmv_csr leaves y untouched
+    torch.cuda.synchronize()
+    assert torch.all(y == 5.0)
+
+
+def test_bf16_conversion_round_to_nearest_even():
+    f = torch.tensor([1.0, 1.00390625, 1.01171875, -2.5, 3.0e38, 1e-40, 0.0], device="cuda")
+    b = api.convert_values(f, torch.bfloat16)
+    assert torch.equal(b.cpu(), f.cpu().to(torch.bfloat16))
+
+
+def test_device_oracle_matches_cpu_oracle():
+    h = ZOO["powerlaw"]
+    A = dev_csr(h)
+    x = xvec(h.numCols)
+    y = api.oracle_f64(A, torch.from_numpy(x).cuda())
+    y64, ab = oracle.spmv_f64(h.rowPtr, h.colIdx, h.values.astype(np.float64), x.astype(np.float64))
+    assert np.all(np.abs(y.cpu().numpy() - y64) <= 4 * 2.0 ** -53 * (np.abs(y64) + ab * 2.0 ** -50) + 1e-300)
