@@ -230,6 +230,14 @@ typedef enum {
 /* Accumulation: 0 = plain, 1 = compensated (two-sum) accumulation. */
 typedef struct muxi_plan muxi_plan;
 
+/* SpMM plan: Y = alpha*A*X + beta*Y with X (numCols x nrhs) and Y (numRows x
+ * nrhs) row-major, nrhs in [1, 64], on merge-path partitions; ipt = items
+ * per lane group (4, 8, 16 or 32; <= 0 picks 8). Executed by muxi_plan_exec
+ * with the same nrhs. Value types FLOAT/BF16/DOUBLE with x FLOAT, or
+ * DOUBLE with x DOUBLE; plain accumulation. */
+spmv_status_t muxi_plan_create_spmm(const muxi_csr_t* A, spmv_type_t vtype, spmv_type_t xtype, int nrhs, int ipt,
+                                    void* stream, muxi_plan** plan);
+
 /* Creates a plan. vtype is the element type of A.d_values (FLOAT, DOUBLE or
  * BF16), xtype the type of x and y (FLOAT or DOUBLE). With xtype FLOAT the
  * accumulator is FP32 (plain or compensated); with xtype DOUBLE it is FP64. */

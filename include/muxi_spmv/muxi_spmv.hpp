@@ -1,0 +1,223 @@
+// muxi_spmv.hpp — header-only C++ shim that restores the reference's
+// `namespace muxi_spmv` names on top of the C-ABI, so reference callers
+// compile with `cudaStream_t` replaced by `hipStream_t` and nothing else.
+//
+//   CSRMatrix<T>      src/formats/sparse_formats.h:24-103 (move-only here)
+//   CSR5Matrix<T>     src/formats/sparse_formats.h:284-386
+//   ELLPACKMatrix<T>  src/spmv/ellpack/spmv_ellpack.cuh:26-39
+//   spmv_csr<T>, spmv_csr_exec<T>, spmv_csr5<T>, csr5_preprocess<T>,
+//   spmv_merge_based<T>, spmv_merge_based_v2<T>, spmv_sparse_optimized<T>,
+//   csr_to_ellpack<T>, spmv_ellpack<T>, spmv_exec<T>, spmv_exec_transpose<T>,
+//   handle functions (src/api/spmv_api.h:33-92)
+// Instantiated for float and double, as the reference is.
+#ifndef MUXI_SPMV_HPP_
+#define MUXI_SPMV_HPP_
+
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <type_traits>
+
+#include "muxi_spmv.h"
+#include "muxi_spmv_prep.h"
+
+namespace muxi_spmv {
+
+using ::spmv_format_t;
+using ::spmv_operation_t;
+using ::spmv_opts_t;
+using ::spmv_status_t;
+using ::spmv_type_t;
+using spmv_handle_t = ::muxi_spmv_handle_t;
+
+inline spmv_opts_t spmv_default_opts() { return ::muxi_spmv_default_opts(); }
+
+template <typename FloatType>
+struct CSRMatrix {
+    int numRows = 0, numCols = 0, nnz = 0;
+    int* rowPtr = nullptr;
+    int* colIdx = nullptr;
+    FloatType* values = nullptr;
+    int* d_rowPtr = nullptr;
+    int* d_colIdx = nullptr;
+    FloatType* d_values = nullptr;
+    bool ownsHostMemory = false;
+    bool ownsDeviceMemory = false;
+
+    CSRMatrix() = default;
+    CSRMatrix(const CSRMatrix&) = delete;  // the reference's implicit copy double-frees
+    CSRMatrix& operator=(const CSRMatrix&) = delete;
+    CSRMatrix(CSRMatrix&& o) noexcept { *this = static_cast<CSRMatrix&&>(o); }
+    CSRMatrix& operator=(CSRMatrix&& o) noexcept {
+        if (this != &o) {
+            release();
+            numRows = o.numRows; numCols = o.numCols; nnz = o.nnz;
+            rowPtr = o.rowPtr; colIdx = o.colIdx; values = o.values;
+            d_rowPtr = o.d_rowPtr; d_colIdx = o.d_colIdx; d_values = o.d_values;
+            ownsHostMemory = o.ownsHostMemory; ownsDeviceMemory = o.ownsDeviceMemory;
+            o.ownsHostMemory = o.ownsDeviceMemory = false;
+        }
+        return *this;
+    }
+    ~CSRMatrix() { release(); }
+
+    void release() {
+        if (ownsHostMemory) { std::free(rowPtr); std::free(colIdx); std::free(values); }
+        if (ownsDeviceMemory) { (void)hipFree(d_rowPtr); (void)hipFree(d_colIdx); (void)hipFree(d_values); }
+        ownsHostMemory = ownsDeviceMemory = false;
+    }
+    void allocateHost(int rows, int cols, int nonzeros) {
+        numRows = rows; numCols = cols; nnz = nonzeros;
+        rowPtr = (int*)std::malloc((size_t)(numRows + 1) * sizeof(int));
+        colIdx = (int*)std::malloc((size_t)nnz * sizeof(int));
+        values = (FloatType*)std::malloc((size_t)nnz * sizeof(FloatType));
+        ownsHostMemory = true;
+    }
+    void allocateDevice() {
+        (void)hipMalloc(&d_rowPtr, (size_t)(numRows + 1) * sizeof(int));
+        (void)hipMalloc(&d_colIdx, (size_t)nnz * sizeof(int));
+        (void)hipMalloc(&d_values, (size_t)nnz * sizeof(FloatType));
+        ownsDeviceMemory = true;
+    }
+    void copyToDevice(hipStream_t stream = 0) {
+        (void)hipMemcpyAsync(d_rowPtr, rowPtr, (size_t)(numRows + 1) * sizeof(int), hipMemcpyHostToDevice, stream);
+        (void)hipMemcpyAsync(d_colIdx, colIdx, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice, stream);
+        (void)hipMemcpyAsync(d_values, values, (size_t)nnz * sizeof(FloatType), hipMemcpyHostToDevice, stream);
+    }
+    void copyFromDevice(hipStream_t stream = 0) {
+        (void)hipMemcpyAsync(rowPtr, d_rowPtr, (size_t)(numRows + 1) * sizeof(int), hipMemcpyDeviceToHost, stream);
+        (void)hipMemcpyAsync(colIdx, d_colIdx, (size_t)nnz * sizeof(int), hipMemcpyDeviceToHost, stream);
+        (void)hipMemcpyAsync(values, d_values, (size_t)nnz * sizeof(FloatType), hipMemcpyDeviceToHost, stream);
+    }
+    // The C-ABI view (same field order).
+    ::muxi_csr_t abi() const {
+        ::muxi_csr_t c;
+        c.numRows = numRows; c.numCols = numCols; c.nnz = nnz;
+        c.rowPtr = rowPtr; c.colIdx = colIdx; c.values = values;
+        c.d_rowPtr = d_rowPtr; c.d_colIdx = d_colIdx; c.d_values = d_values;
+        c.ownsHostMemory = false; c.ownsDeviceMemory = false;
+        return c;
+    }
+};
+
+template <typename FloatType>
+struct CSR5Matrix {
+    ::muxi_csr5_t c{};
+    CSR5Matrix() { c.sigma = 256; }
+    CSR5Matrix(const CSR5Matrix&) = delete;
+    CSR5Matrix& operator=(const CSR5Matrix&) = delete;
+    ~CSR5Matrix() { ::muxi_csr5_destroy(&c); }
+    int sigma() const { return c.sigma; }
+    int numTiles() const { return c.numTiles; }
+    const int* tile_row_ptr() const { return c.tile_row_ptr; }
+    const int* tile_nnz_offset() const { return c.tile_nnz_offset; }
+    double conversionTimeMs() const { return c.conversionTimeMs; }
+};
+
+template <typename FloatType>
+struct ELLPACKMatrix {
+    int numRows = 0, numCols = 0, numElements = 0, nnz = 0;
+    int* d_colIdx = nullptr;
+    FloatType* d_values = nullptr;  // the caller frees, as in the reference
+};
+
+namespace detail {
+template <typename T> struct is_fp : std::integral_constant<bool, std::is_same<T, float>::value || std::is_same<T, double>::value> {};
+}
+
+template <typename T>
+inline spmv_status_t spmv_csr(const CSRMatrix<T>& A, const T* x, T* y, T alpha, T beta, const spmv_opts_t& opts) {
+    static_assert(detail::is_fp<T>::value, "float or double");
+    ::muxi_csr_t c = A.abi();
+    if constexpr (std::is_same<T, float>::value) return ::muxi_spmv_csr_f32(&c, x, y, alpha, beta, &opts);
+    else return ::muxi_spmv_csr_f64(&c, x, y, alpha, beta, &opts);
+}
+
+template <typename T>
+inline spmv_status_t spmv_csr_exec(const CSRMatrix<T>& A, const T* x, T* y, T alpha = 1.0, T beta = 0.0,
+                                   const spmv_opts_t& opts = spmv_default_opts()) {
+    return spmv_csr<T>(A, x, y, alpha, beta, opts);
+}
+
+template <typename T>
+inline spmv_status_t spmv_merge_based(const CSRMatrix<T>& A, const T* x, T* y, hipStream_t stream) {
+    ::muxi_csr_t c = A.abi();
+    if constexpr (std::is_same<T, float>::value) return ::muxi_spmv_merge_based_f32(&c, x, y, stream);
+    else return ::muxi_spmv_merge_based_f64(&c, x, y, stream);
+}
+
+template <typename T>
+inline spmv_status_t spmv_merge_based_v2(const CSRMatrix<T>& A, const T* x, T* y, hipStream_t stream, bool useOptimized) {
+    ::muxi_csr_t c = A.abi();
+    if constexpr (std::is_same<T, float>::value) return ::muxi_spmv_merge_based_v2_f32(&c, x, y, stream, useOptimized);
+    else return ::muxi_spmv_merge_based_v2_f64(&c, x, y, stream, useOptimized);
+}
+
+template <typename T>
+inline spmv_status_t spmv_sparse_optimized(const CSRMatrix<T>& A, const T* x, T* y, hipStream_t stream) {
+    ::muxi_csr_t c = A.abi();
+    if constexpr (std::is_same<T, float>::value) return ::muxi_spmv_sparse_optimized_f32(&c, x, y, stream);
+    else return ::muxi_spmv_sparse_optimized_f64(&c, x, y, stream);
+}
+
+template <typename T>
+inline spmv_status_t csr5_preprocess(const CSRMatrix<T>& csr, CSR5Matrix<T>& csr5, int sigma, hipStream_t stream) {
+    ::muxi_csr_t c = csr.abi();
+    return ::muxi_csr5_preprocess(&c, &csr5.c, sigma, stream);
+}
+
+template <typename T>
+inline spmv_status_t spmv_csr5(const CSR5Matrix<T>& A, const T* x, T* y, T alpha, T beta, const spmv_opts_t& opts) {
+    if constexpr (std::is_same<T, float>::value) return ::muxi_spmv_csr5_f32(&A.c, x, y, alpha, beta, &opts);
+    else return ::muxi_spmv_csr5_f64(&A.c, x, y, alpha, beta, &opts);
+}
+
+template <typename T>
+inline spmv_status_t csr_to_ellpack(const CSRMatrix<T>& csr, ELLPACKMatrix<T>& ell, int maxElements = 0) {
+    ::muxi_csr_t c = csr.abi();
+    ::muxi_ell_t e{};
+    spmv_status_t st;
+    if constexpr (std::is_same<T, float>::value) st = ::muxi_csr_to_ellpack_f32(&c, &e, maxElements);
+    else st = ::muxi_csr_to_ellpack_f64(&c, &e, maxElements);
+    ell.numRows = e.numRows; ell.numCols = e.numCols; ell.numElements = e.numElements; ell.nnz = e.nnz;
+    ell.d_colIdx = e.d_colIdx; ell.d_values = (T*)e.d_values;
+    return st;
+}
+
+template <typename T>
+inline spmv_status_t spmv_ellpack(const ELLPACKMatrix<T>& ell, const T* d_x, T* d_y, hipStream_t stream) {
+    ::muxi_ell_t e{ell.numRows, ell.numCols, ell.numElements, ell.nnz, ell.d_colIdx, (void*)ell.d_values};
+    if constexpr (std::is_same<T, float>::value) return ::muxi_spmv_ellpack_f32(&e, d_x, d_y, stream);
+    else return ::muxi_spmv_ellpack_f64(&e, d_x, d_y, stream);
+}
+
+inline spmv_status_t spmv_create_handle(spmv_handle_t** h) { return ::muxi_spmv_create_handle(h); }
+inline spmv_status_t spmv_destroy_handle(spmv_handle_t* h) { return ::muxi_spmv_destroy_handle(h); }
+inline spmv_status_t spmv_set_stream(spmv_handle_t* h, hipStream_t s) { return ::muxi_spmv_set_stream(h, s); }
+
+template <typename T>
+inline spmv_status_t spmv_exec(spmv_handle_t* h, spmv_format_t format, const void* matrix, const T* x, T* y,
+                               T alpha = 1.0, T beta = 0.0) {
+    // matrix: CSRMatrix<T>* for SPMV_FORMAT_CSR, CSR5Matrix<T>* for SPMV_FORMAT_CSR5
+    ::muxi_csr_t c;
+    const void* m = matrix;
+    if (format == SPMV_FORMAT_CSR) { c = static_cast<const CSRMatrix<T>*>(matrix)->abi(); m = &c; }
+    else if (format == SPMV_FORMAT_CSR5) m = &static_cast<const CSR5Matrix<T>*>(matrix)->c;
+    if constexpr (std::is_same<T, float>::value) return ::muxi_spmv_exec_f32(h, format, m, x, y, alpha, beta);
+    else return ::muxi_spmv_exec_f64(h, format, m, x, y, alpha, beta);
+}
+
+template <typename T>
+inline spmv_status_t spmv_exec_transpose(spmv_handle_t* h, spmv_format_t format, const void* matrix, const T* x,
+                                         T* y, T alpha = 1.0, T beta = 0.0) {
+    ::muxi_csr_t c;
+    const void* m = matrix;
+    if (format == SPMV_FORMAT_CSR) { c = static_cast<const CSRMatrix<T>*>(matrix)->abi(); m = &c; }
+    else if (format == SPMV_FORMAT_CSR5) m = &static_cast<const CSR5Matrix<T>*>(matrix)->c;
+    if constexpr (std::is_same<T, float>::value) return ::muxi_spmv_exec_transpose_f32(h, format, m, x, y, alpha, beta);
+    else return ::muxi_spmv_exec_transpose_f64(h, format, m, x, y, alpha, beta);
+}
+
+}  // namespace muxi_spmv
+
+#endif  // MUXI_SPMV_HPP_

@@ -100,6 +100,7 @@ SIGNATURES: dict[str, tuple] = {
     "muxi_ellpack_destroy": (_I, [_P_ELL]),
     "muxi_plan_create": (_I, [_P_CSR, _I, _I, _I, _I, _I, _VP, C.POINTER(_VP)]),
     "muxi_plan_destroy": (_I, [_VP]),
+    "muxi_plan_create_spmm": (_I, [_P_CSR, _I, _I, _I, _I, _VP, C.POINTER(_VP)]),
     "muxi_plan_exec": (_I, [_VP, _VP, _VP, _D, _D, _I, _VP]),
     "muxi_plan_info": (_I, [_VP, C.POINTER(_I), C.POINTER(_I), C.POINTER(_Z)]),
     "muxi_plan_exec_ev": (_I, [_VP, _VP, _VP, _D, _D, _I, _VP, _VP, _VP]),
@@ -110,7 +111,26 @@ SIGNATURES: dict[str, tuple] = {
     "muxi_spmv_oracle_f64": (_I, [_P_CSR, _I, _I, _VP, _VP, _VP]),
     "muxi_convert_values": (_I, [_VP, _I, _VP, _I, _Z, _VP]),
     "muxi_device_copy": (_I, [_VP, _VP, _Z, _VP]),
+    # muxi_spmv_prep.h
+    "muxi_reorder_columns": (_I, [_I, _I, _VP, _VP, _VP, _VP, _VP]),
+    "muxi_reorder_x_f32": (_I, [_VP, _VP, _VP, _I]),
+    "muxi_sort_columns_within_rows_f32": (_I, [_I, _VP, _VP, _VP]),
+    "muxi_reorder_columns_minrow": (_I, [_I, _I, _VP, _VP, _VP, _VP]),
+    "muxi_reorder_columns_by_frequency": (_I, [_I, _I, _VP, _VP, _VP, _VP]),
+    "muxi_reorder_rows_by_length_f32": (_I, [_I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "muxi_rcm": (_I, [_I, _VP, _VP, _VP]),
+    "muxi_permute_symmetric": (_I, [_I, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP]),
+    "muxi_bandwidth": (_I, [_I, _VP, _VP]),
+    "muxi_remap_indices_dev": (_I, [_VP, _VP, _VP, _I, _VP]),
+    "muxi_gather_dev": (_I, [_VP, _VP, _VP, _I, _I, _VP]),
+    "muxi_scatter_dev": (_I, [_VP, _VP, _VP, _I, _I, _VP]),
+    "muxi_permute_rows_dev": (_I, [_I, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _VP]),
 }
+
+
+class muxi_column_reorder_stats_t(C.Structure):
+    _fields_ = [("numReorders", C.c_int), ("avgDistanceBefore", C.c_float), ("avgDistanceAfter", C.c_float),
+                ("improvementRatio", C.c_float)]
 
 _lib = None
 

@@ -298,17 +298,32 @@ class Plan:
                  compensated: bool = False, stream=None):
         self.A = A
         self.xtype = xtype
+        self.nrhs = 1
         self._p = C.c_void_p()
         check(load().muxi_plan_create(C.byref(A.c), A.vtype, _TORCH2SPMV[xtype], _lib.KERNELS[kernel], param0,
                                       int(compensated), _stream(stream), C.byref(self._p)), "plan_create")
+
+    @classmethod
+    def spmm(cls, A: CSRMatrix, nrhs: int, ipt: int = 8, xtype=torch.float32, stream=None) -> "Plan":
+        """SpMM plan: Y = alpha*A*X + beta*Y, X (numCols x nrhs) and Y (numRows x nrhs) row-major."""
+        self = cls.__new__(cls)
+        self.A = A
+        self.xtype = xtype
+        self.nrhs = nrhs
+        self._p = C.c_void_p()
+        check(load().muxi_plan_create_spmm(C.byref(A.c), A.vtype, _TORCH2SPMV[xtype], nrhs, ipt, _stream(stream),
+                                           C.byref(self._p)), "plan_create_spmm")
+        return self
 
     def info(self) -> PlanInfo:
         k, p0, sb = C.c_int(), C.c_int(), C.c_size_t()
         check(load().muxi_plan_info(self._p, C.byref(k), C.byref(p0), C.byref(sb)), "plan_info")
         return PlanInfo(_lib.KERNEL_NAMES[k.value], p0.value, sb.value)
 
-    def exec(self, x: torch.Tensor, y: torch.Tensor, alpha=1.0, beta=0.0, nrhs: int = 1, stream=None) -> None:
-        check(load().muxi_plan_exec(self._p, _ptr(x), _ptr(y), float(alpha), float(beta), nrhs, _stream(stream)),
+    def exec(self, x: torch.Tensor, y: torch.Tensor, alpha=1.0, beta=0.0, nrhs: int | None = None,
+             stream=None) -> None:
+        n = self.nrhs if nrhs is None else nrhs
+        check(load().muxi_plan_exec(self._p, _ptr(x), _ptr(y), float(alpha), float(beta), n, _stream(stream)),
               "plan_exec")
 
     def destroy(self):

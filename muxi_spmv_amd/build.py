@@ -69,11 +69,31 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
     return LIB
 
 
+CPP_TESTS = [ROOT / "tests" / "cpp" / "test_spmv_known_answer.cpp"]
+
+
+def build_cpp_tests(verbose: bool = True) -> list[Path]:
+    """Reference-style C++ callers compiled against include/muxi_spmv/muxi_spmv.hpp."""
+    out = []
+    for src in CPP_TESTS:
+        exe = src.with_suffix("")
+        cmd = [HIPCC, "-std=c++17", "-O2", f"--offload-arch={ARCH}", f"-I{ROOT / 'include'}", str(src),
+               f"-L{PKG}", "-lmuxi_spmv_amd", "-Wl,-rpath,$ORIGIN/../../muxi_spmv_amd", "-o", str(exe)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {src.name}\n{r.stdout}\n{r.stderr}")
+        out.append(exe)
+    if verbose:
+        print(f"built {len(out)} C++ caller test(s)", flush=True)
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
     a = ap.parse_args(argv)
     build(a.jobs)
+    build_cpp_tests()
     return 0
 
 

@@ -62,6 +62,13 @@ template <typename VT, typename XT, typename AT, bool COMP>
 spmv_status_t launch_ell(int R, int width, const int* d_col, const VT* d_val, const XT* x, XT* y, AT alpha,
                          AT beta, hipStream_t s);
 
+// ---- SpMM over merge-path partitions (kernels_spmm.hip); carry holds P*k elements.
+int spmm_groups(int k);
+int spmm_chunk_items(int k, int ipt);
+template <typename VT, typename XT>
+spmv_status_t launch_spmm(const DevCsr& A, int k, int ipt, const int* d_pos, const int* d_row, int P, void* carry,
+                          const XT* X, XT* Y, XT alpha, XT beta, hipStream_t s, hipEvent_t ev = nullptr);
+
 // ---- misc (kernels_misc.hip)
 template <typename VT, typename XT>
 spmv_status_t launch_oracle_f64(const DevCsr& A, const XT* x, double* y, hipStream_t s);

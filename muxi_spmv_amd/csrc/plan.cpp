@@ -2,6 +2,7 @@
 // times without allocating, so an exec may be captured in a hipGraph.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <new>
 
@@ -179,6 +180,61 @@ extern "C" spmv_status_t muxi_plan_create(const muxi_csr_t* Ain, spmv_type_t vt,
     return SPMV_SUCCESS;
 }
 
+extern "C" spmv_status_t muxi_plan_create_spmm(const muxi_csr_t* Ain, spmv_type_t vt, spmv_type_t xt, int nrhs,
+                                               int ipt, void* stream, muxi_plan** out) {
+    if (!out) {
+        set_error("plan output pointer is NULL");
+        return SPMV_ERROR_INVALID_HANDLE;
+    }
+    *out = nullptr;
+    if (!Ain || Ain->numRows < 0 || Ain->numCols < 0 || Ain->nnz < 0 ||
+        (long)Ain->numRows + (long)Ain->nnz > 2147483647L || (Ain->numRows > 0 && !Ain->d_rowPtr) ||
+        (Ain->nnz > 0 && (!Ain->d_colIdx || !Ain->d_values))) {
+        set_error("invalid matrix");
+        return SPMV_ERROR_INVALID_MATRIX;
+    }
+    if (nrhs < 1 || nrhs > 64) {
+        set_error("spmm: nrhs must be in [1, 64]");
+        return SPMV_ERROR_INVALID_VECTOR;
+    }
+    if (ipt <= 0) ipt = 8;
+    if (!(ipt == 4 || ipt == 8 || ipt == 16 || ipt == 32)) {
+        set_error("spmm: items per group must be 4, 8, 16 or 32");
+        return SPMV_ERROR_INTERNAL;
+    }
+    const bool ok_types = (xt == SPMV_TYPE_FLOAT && (vt == SPMV_TYPE_FLOAT || vt == SPMV_TYPE_BF16 ||
+                                                     vt == SPMV_TYPE_DOUBLE)) ||
+                          (xt == SPMV_TYPE_DOUBLE && vt == SPMV_TYPE_DOUBLE);
+    if (!ok_types) {
+        set_error("spmm: unsupported type combination");
+        return SPMV_ERROR_UNSUPPORTED_TYPE;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    muxi_plan* p = new (std::nothrow) muxi_plan();
+    if (!p) return SPMV_ERROR_MEMORY_ALLOC;
+    p->A = DevCsr{Ain->numRows, Ain->numCols, Ain->nnz, Ain->d_rowPtr, Ain->d_colIdx, Ain->d_values};
+    p->vtype = vt;
+    p->xtype = xt;
+    p->comp = false;
+    p->kernel = MUXI_KERNEL_MERGE;
+    p->param0 = ipt;
+    p->nrhs = nrhs;
+    PLAN_HIP(hipGetDevice(&p->device));
+    const int R = Ain->numRows, N = Ain->nnz;
+    const long ch = spmm_chunk_items(nrhs, ipt);
+    const int P = (int)std::max(1L, ((long)R + N + ch - 1) / ch);
+    p->P = P;
+    const size_t es = xt == SPMV_TYPE_DOUBLE ? 8 : 4;
+    PLAN_HIP(hipMalloc(&p->d_pos, 2 * sizeof(int) * (size_t)(P + 1)));
+    p->d_row = p->d_pos + (P + 1);
+    PLAN_HIP(hipMalloc(&p->d_carry, es * (size_t)P * nrhs));
+    p->scratch_bytes = 2 * sizeof(int) * (size_t)(P + 1) + es * (size_t)P * nrhs;
+    PLAN_TRY(launch_merge_partitions(R, N, P, p->d_pos, s));
+    PLAN_TRY(launch_merge_coords(p->A.rp, R, N, p->d_pos, P + 1, p->d_row, nullptr, s));
+    *out = p;
+    return SPMV_SUCCESS;
+}
+
 extern "C" spmv_status_t muxi_plan_destroy(muxi_plan* p) {
     if (!p) {
         set_error("plan is NULL");
@@ -209,9 +265,27 @@ static spmv_status_t plan_exec(const muxi_plan* p, const void* x, void* y, doubl
         set_error("x or y is NULL");
         return SPMV_ERROR_INVALID_VECTOR;
     }
+    if (p->nrhs > 0) {
+        if (nrhs != p->nrhs) {
+            set_error("nrhs differs from the SpMM plan's");
+            return SPMV_ERROR_INVALID_VECTOR;
+        }
+        if (ev0) MUXI_HIP(hipEventRecord(ev0, s));
+        if (p->xtype == SPMV_TYPE_DOUBLE)
+            return launch_spmm<double, double>(p->A, nrhs, p->param0, p->d_pos, p->d_row, p->P, p->d_carry,
+                                               (const double*)x, (double*)y, alpha, beta, s, ev1);
+        if (p->vtype == SPMV_TYPE_BF16)
+            return launch_spmm<bf16_t, float>(p->A, nrhs, p->param0, p->d_pos, p->d_row, p->P, p->d_carry,
+                                              (const float*)x, (float*)y, (float)alpha, (float)beta, s, ev1);
+        if (p->vtype == SPMV_TYPE_DOUBLE)
+            return launch_spmm<double, float>(p->A, nrhs, p->param0, p->d_pos, p->d_row, p->P, p->d_carry,
+                                              (const float*)x, (float*)y, (float)alpha, (float)beta, s, ev1);
+        return launch_spmm<float, float>(p->A, nrhs, p->param0, p->d_pos, p->d_row, p->P, p->d_carry,
+                                         (const float*)x, (float*)y, (float)alpha, (float)beta, s, ev1);
+    }
     if (nrhs != 1) {
-        set_error("nrhs > 1 is served by muxi_spmm_exec");
-        return SPMV_ERROR_UNSUPPORTED_TYPE;
+        set_error("nrhs > 1 needs a plan from muxi_plan_create_spmm");
+        return SPMV_ERROR_INVALID_VECTOR;
     }
     if (ev0) MUXI_HIP(hipEventRecord(ev0, s));
     spmv_status_t st = with_types(p->vtype, p->xtype, p->comp, [&](auto tag) -> spmv_status_t {

@@ -29,6 +29,8 @@ struct muxi_plan {
     int nblocks = 0;
     int* d_blocks = nullptr;
     size_t scratch_bytes = 0;
+    // SpMM: right-hand sides (0 for a SpMV plan)
+    int nrhs = 0;
 };
 
 namespace muxi {

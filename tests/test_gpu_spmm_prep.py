@@ -1,0 +1,108 @@
+"""GPU parity for SpMM (k right-hand sides) and the device permutation
+kernels. SpMM tolerance per column: the FP32 bound of the SpMV tests;
+permutation application bit-exact."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from muxi_spmv_amd import api, prep
+from muxi_spmv_amd import generators as G
+from gpu_util import dev_csr, fp32_bound, fp64_bound, matrix_zoo
+
+pytestmark = pytest.mark.gpu
+
+ZOO = matrix_zoo()
+
+
+def spmm_ref(h, X):
+    """Column-by-column FP64 oracle and per-row abs sums."""
+    cols = [oracle.spmv_f64(h.rowPtr, h.colIdx, h.values.astype(np.float64), X[:, j].astype(np.float64))
+            for j in range(X.shape[1])]
+    return np.stack([c[0] for c in cols], 1), np.stack([c[1] for c in cols], 1)
+
+
+@pytest.mark.parametrize("mname", ["uniform", "powerlaw", "empty_rows", "long_row", "tiny", "single_row"])
+@pytest.mark.parametrize("k", [1, 3, 4, 8, 12, 32, 64])
+@pytest.mark.parametrize("ipt", [4, 16])
+def test_spmm_fp32(mname, k, ipt):
+    h = ZOO[mname]
+    A = dev_csr(h)
+    X = np.random.default_rng(k).uniform(-1, 1, (h.numCols, k)).astype(np.float32)
+    Y = torch.full((h.numRows, k), float("nan"), device="cuda")
+    p = api.Plan.spmm(A, k, ipt)
+    p.exec(torch.from_numpy(X).cuda(), Y)
+    torch.cuda.synchronize()
+    y64, ab = spmm_ref(h, X)
+    err = np.abs(Y.double().cpu().numpy() - y64)
+    bound = fp32_bound(h.rowPtr, 1.0)[:, None] * ab + 1e-37
+    assert np.all(err <= bound)
+
+
+@pytest.mark.parametrize("k", [8, 32])
+def test_spmm_alpha_beta_and_bf16(k):
+    h = ZOO["powerlaw"]
+    A = dev_csr(h, torch.bfloat16)
+    X = np.random.default_rng(1).uniform(-1, 1, (h.numCols, k)).astype(np.float32)
+    Y0 = np.random.default_rng(2).uniform(-1, 1, (h.numRows, k)).astype(np.float32)
+    Y = torch.from_numpy(Y0).cuda()
+    api.Plan.spmm(A, k).exec(torch.from_numpy(X).cuda(), Y, alpha=2.0, beta=-1.0)
+    torch.cuda.synchronize()
+    hb = G.HostCSR(h.rowPtr, h.colIdx, A.d_values.double().cpu().numpy(), h.numRows, h.numCols)
+    y64, ab = spmm_ref(hb, X)
+    exp = 2.0 * y64 - Y0
+    err = np.abs(Y.double().cpu().numpy() - exp)
+    assert np.all(err <= 2.0 * fp32_bound(h.rowPtr, 1.0)[:, None] * ab + 4 * 2.0 ** -24 * np.abs(exp) + 1e-30)
+
+
+def test_spmm_fp64():
+    h = G.laplacian_2d(40)
+    A = dev_csr(h, torch.float64)
+    X = np.random.default_rng(3).uniform(-1, 1, (h.numCols, 8))
+    Y = torch.zeros((h.numRows, 8), dtype=torch.float64, device="cuda")
+    api.Plan.spmm(A, 8, xtype=torch.float64).exec(torch.from_numpy(X).cuda(), Y)
+    torch.cuda.synchronize()
+    y64, ab = spmm_ref(h, X)
+    assert np.all(np.abs(Y.cpu().numpy() - y64) <= fp64_bound(h.rowPtr, 1.0)[:, None] * ab)
+
+
+def test_spmm_matches_spmv_columns_bitwise_deterministic():
+    h = ZOO["powerlaw"]
+    A = dev_csr(h)
+    X = torch.from_numpy(np.random.default_rng(4).uniform(-1, 1, (h.numCols, 8)).astype(np.float32)).cuda()
+    p = api.Plan.spmm(A, 8)
+    Y1 = torch.empty((h.numRows, 8), device="cuda")
+    Y2 = torch.empty((h.numRows, 8), device="cuda")
+    p.exec(X, Y1)
+    p.exec(X, Y2)
+    torch.cuda.synchronize()
+    assert torch.equal(Y1, Y2)
+
+
+# ---------------------------------------------------------------- device permutations
+def test_device_permutation_application_bit_exact():
+    h = ZOO["powerlaw"]
+    rm, irm, nrp, nci, nv = oracle.reorder_rows_by_length(h.rowPtr, h.colIdx, h.values)
+    A = dev_csr(h)
+    B = prep.permute_rows_dev(A, torch.from_numpy(rm).cuda(), torch.from_numpy(nrp).cuda())
+    torch.cuda.synchronize()
+    assert np.array_equal(B.d_colIdx.cpu().numpy(), nci)
+    assert np.array_equal(B.d_values.cpu().numpy(), nv)
+    fwd, nc = oracle.reorder_columns_com(h.rowPtr, h.colIdx, h.numCols)
+    inv = np.empty_like(fwd)
+    inv[fwd] = np.arange(fwd.size)
+    got = prep.remap_indices_dev(A.d_colIdx, torch.from_numpy(inv).cuda())
+    assert np.array_equal(got.cpu().numpy(), nc)
+    x = G.uniform_vector(h.numCols, 5)
+    xr = prep.gather_dev(torch.from_numpy(x).cuda(), torch.from_numpy(fwd).cuda())
+    assert np.array_equal(xr.cpu().numpy(), oracle.reorder_x(x, fwd))
+    # y of the row-permuted matrix, restored to the original order, equals y of A
+    xd = torch.from_numpy(x).cuda()
+    yp = torch.empty(h.numRows, device="cuda")
+    api.Plan(B, "merge", 8).exec(xd, yp)
+    y = torch.empty(h.numRows, device="cuda")
+    api.Plan(A, "merge", 8).exec(xd, y)
+    yr = prep.scatter_dev(yp, torch.from_numpy(rm).cuda())
+    torch.cuda.synchronize()
+    assert np.array_equal(yr.cpu().numpy(), oracle.restore_order(yp.cpu().numpy(), rm))
+    assert torch.allclose(yr, y, rtol=0, atol=1e-4)

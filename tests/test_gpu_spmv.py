@@ -257,3 +257,15 @@ def test_device_oracle_matches_cpu_oracle():
     y = api.oracle_f64(A, torch.from_numpy(x).cuda())
     y64, ab = oracle.spmv_f64(h.rowPtr, h.colIdx, h.values.astype(np.float64), x.astype(np.float64))
     assert np.all(np.abs(y.cpu().numpy() - y64) <= 4 * 2.0 ** -53 * (np.abs(y64) + ab * 2.0 ** -50) + 1e-300)
+
+
+def test_cpp_reference_caller():
+    """A reference-style C++ caller (tests/cpp) built against the C++ shim."""
+    import subprocess
+    from pathlib import Path
+
+    exe = Path(__file__).resolve().parent / "cpp" / "test_spmv_known_answer"
+    assert exe.exists(), "build with python -m muxi_spmv_amd.build"
+    r = subprocess.run([str(exe), "4099"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "PASSED" in r.stdout
