@@ -125,6 +125,18 @@ SIGNATURES: dict[str, tuple] = {
     "muxi_gather_dev": (_I, [_VP, _VP, _VP, _I, _I, _VP]),
     "muxi_scatter_dev": (_I, [_VP, _VP, _VP, _I, _I, _VP]),
     "muxi_permute_rows_dev": (_I, [_I, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _VP]),
+    # muxi_spmv_tune.h
+    "muxi_gen_matrix": (_I, [_I, _I, _I, _D, _D, _D, _I, _I, C.c_uint64, _VP, _P_CSR]),
+    "muxi_gen_permute": (_I, [_P_CSR, _I, C.c_uint64, _VP]),
+    "muxi_gen_free": (_I, [_P_CSR]),
+    "muxi_gen_vector": (_I, [_VP, _I, C.c_long, C.c_uint64, _VP]),
+    "muxi_matrix_features": (_I, [_P_CSR, _VP, C.POINTER(C.c_double)]),
+    "muxi_flush_caches": (_I, [_VP, _Z, _VP]),
+    "muxi_mlp_param_count": (C.c_long, [_I, _I, _I]),
+    "muxi_mlp_workspace_bytes": (_Z, [_I, _I, _I, _I]),
+    "muxi_mlp_forward": (_I, [_VP, _I, _I, _I, _VP, _I, _VP, _VP, _VP]),
+    "muxi_mlp_backward": (_I, [_VP, _I, _I, _I, _VP, _I, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "muxi_mlp_adam": (_I, [_VP, _VP, _VP, _VP, C.c_long, _F, _F, _F, _F, _I, _VP]),
 }
 
 
