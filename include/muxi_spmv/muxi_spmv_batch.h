@@ -1,0 +1,57 @@
+/*
+ * muxi_spmv_batch.h — many independent SpMVs in one persistent launch.
+ *
+ * No counterpart in the reference (its "batched" kernels,
+ * src/spmv/csr/spmv_csr.cuh:545-869, batch rows of one matrix). A batch plan
+ * binds `count` CSR matrices and their x / y vectors; every matrix is cut
+ * into merge-path tiles of the same path length and one persistent kernel
+ * (grid sized to the device) works through all tiles of all matrices, then
+ * one fix-up kernel adds the partial sums of rows that span tiles. Exec takes
+ * no pointers, allocates nothing, and can be captured in a hipGraph
+ * (muxi_batch_graph_*).
+ */
+#ifndef MUXI_SPMV_BATCH_H_
+#define MUXI_SPMV_BATCH_H_
+
+#include "muxi_spmv.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct muxi_batch_plan muxi_batch_plan;
+
+/* mats[count] with device arrays; xs[count] / ys[count] device vectors of
+ * numCols / numRows elements. vtype FLOAT (xtype FLOAT) or DOUBLE (xtype
+ * DOUBLE). ipt: merge items per lane (4, 8 or 16; <= 0 picks 8). */
+spmv_status_t muxi_batch_plan_create(const muxi_csr_t* mats, int count, spmv_type_t vtype, spmv_type_t xtype,
+                                     const void* const* xs, void* const* ys, int ipt, void* stream,
+                                     muxi_batch_plan** plan);
+spmv_status_t muxi_batch_plan_destroy(muxi_batch_plan* plan);
+/* ys[m] = alpha * mats[m] * xs[m] + beta * ys[m] for every m. */
+spmv_status_t muxi_batch_exec(const muxi_batch_plan* plan, double alpha, double beta, void* stream);
+/* Like muxi_batch_exec, recording ev_before / ev_after around the whole batch. */
+spmv_status_t muxi_batch_exec_ev(const muxi_batch_plan* plan, double alpha, double beta, void* stream,
+                                 void* ev_before, void* ev_after);
+/* The plain per-matrix path for comparison: one launch of the given row
+ * kernel (MUXI_KERNEL_SCALAR, MUXI_KERNEL_VECTOR or MUXI_KERNEL_VWAVE with
+ * param0 lanes) per matrix, y = A*x. */
+spmv_status_t muxi_batch_exec_per_matrix(const muxi_batch_plan* plan, muxi_kernel_t kernel, int param0,
+                                         void* stream);
+/* Tiles, grid size and scratch bytes of the plan. */
+spmv_status_t muxi_batch_info(const muxi_batch_plan* plan, int* tiles, int* grid, size_t* scratch_bytes);
+
+/* hipGraph of one muxi_batch_exec (per_matrix_kernel = 0) or of one
+ * muxi_batch_exec_per_matrix (its kernel id), captured on `stream` (must not
+ * be the null stream), then replayed by muxi_graph_launch. */
+typedef struct muxi_graph muxi_graph;
+spmv_status_t muxi_batch_graph_create(const muxi_batch_plan* plan, double alpha, double beta,
+                                      int per_matrix_kernel, int param0, void* stream, muxi_graph** graph);
+spmv_status_t muxi_graph_launch(const muxi_graph* graph, void* stream);
+spmv_status_t muxi_graph_destroy(muxi_graph* graph);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MUXI_SPMV_BATCH_H_ */

@@ -125,6 +125,17 @@ SIGNATURES: dict[str, tuple] = {
     "muxi_gather_dev": (_I, [_VP, _VP, _VP, _I, _I, _VP]),
     "muxi_scatter_dev": (_I, [_VP, _VP, _VP, _I, _I, _VP]),
     "muxi_permute_rows_dev": (_I, [_I, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _VP]),
+    # muxi_spmv_batch.h
+    "muxi_batch_plan_create": (_I, [_P_CSR, _I, _I, _I, C.POINTER(_VP), C.POINTER(_VP), _I, _VP,
+                                    C.POINTER(_VP)]),
+    "muxi_batch_plan_destroy": (_I, [_VP]),
+    "muxi_batch_exec": (_I, [_VP, _D, _D, _VP]),
+    "muxi_batch_exec_ev": (_I, [_VP, _D, _D, _VP, _VP, _VP]),
+    "muxi_batch_exec_per_matrix": (_I, [_VP, _I, _I, _VP]),
+    "muxi_batch_info": (_I, [_VP, C.POINTER(_I), C.POINTER(_I), C.POINTER(_Z)]),
+    "muxi_batch_graph_create": (_I, [_VP, _D, _D, _I, _I, _VP, C.POINTER(_VP)]),
+    "muxi_graph_launch": (_I, [_VP, _VP]),
+    "muxi_graph_destroy": (_I, [_VP]),
     # muxi_spmv_tune.h
     "muxi_gen_matrix": (_I, [_I, _I, _I, _D, _D, _D, _I, _I, C.c_uint64, _VP, _P_CSR]),
     "muxi_gen_permute": (_I, [_P_CSR, _I, C.c_uint64, _VP]),

@@ -19,7 +19,7 @@
 // Every sum is formed in a fixed order, so results are bitwise reproducible.
 #include <climits>
 
-#include "kernels.hpp"
+#include "merge_tile.hpp"
 
 namespace muxi {
 
@@ -79,154 +79,17 @@ size_t merge_carry_bytes(int P, spmv_type_t xtype, bool comp) {
     return (size_t)(P > 0 ? P : 1) * e;
 }
 
-// Workgroup-wide inclusive segmented scan over (flag, value) in lane order:
-// (f1,v1) then (f2,v2) -> (f1|f2, f2 ? v2 : v1+v2). Returns the exclusive
-// value (sum carried into this lane) and the workgroup total in *tot.
-template <typename A, int NW>
-__device__ __forceinline__ A block_seg_scan_excl(A v, int f, A* s_agg, int* s_flag, A* tot) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        A vu = v.shfl_up(d);
-        int fu = __shfl_up(f, d);
-        if (lane >= d) {
-            if (!f) {
-                vu.add(v);
-                v = vu;
-            }
-            f |= fu;
-        }
-    }
-    if (lane == 63) {
-        s_agg[w] = v;
-        s_flag[w] = f;
-    }
-    __syncthreads();
-    A pre;
-    pre.zero();
-    for (int u = 0; u < w; ++u) {
-        if (s_flag[u]) {
-            pre = s_agg[u];
-        } else {
-            pre.add(s_agg[u]);
-        }
-    }
-    // inclusive value of this lane including the earlier waves
-    A inc = v;
-    if (!f) {
-        A t = pre;
-        t.add(v);
-        inc = t;
-    }
-    A ex = inc.shfl_up(1);
-    if (lane == 0) ex = pre;
-    if (tot) {
-        A t;
-        t.zero();
-        int ft = 0;
-        for (int u = 0; u < NW; ++u) {
-            if (s_flag[u]) {
-                t = s_agg[u];
-                ft = 1;
-            } else {
-                t.add(s_agg[u]);
-            }
-        }
-        (void)ft;
-        *tot = t;
-    }
-    return ex;
-}
-
 template <typename VT, typename XT, typename AT, bool COMP, int IPT>
 __global__ __launch_bounds__(256) void merge_spmv_kernel(const int* __restrict__ rp, const int* __restrict__ ci,
                                                          const VT* __restrict__ vals, const XT* __restrict__ x,
                                                          XT* __restrict__ y, const int* __restrict__ pdiag,
                                                          const int* __restrict__ prow, int R, AT alpha, AT beta,
                                                          Acc<AT, COMP>* __restrict__ carry) {
-    constexpr int B = 256, CH = B * IPT;
-    using A = Acc<AT, COMP>;
-    using P = Prod<AT, COMP>;
-    __shared__ int s_end[CH];
-    __shared__ P s_prod[CH];
-    __shared__ A s_agg[B / 64];
-    __shared__ int s_flag[B / 64];
-
-    const int p = blockIdx.x, tid = threadIdx.x;
-    const int d0 = pdiag[p], d1 = pdiag[p + 1];
-    const int r0 = prow[p], r1 = prow[p + 1];
-    const int n0 = d0 - r0, n1 = d1 - r1;
-    const int nr = r1 - r0, nn = n1 - n0, tot = nr + nn;
-
-#pragma unroll
-    for (int k = 0; k < IPT; ++k) {
-        int j = tid + k * B;
-        if (j < nn) {
-            int c = ci[n0 + j];
-            s_prod[j] = P::make(vals[n0 + j], x[c]);
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < IPT; ++k) {
-        int i = tid + k * B;
-        if (i < nr) s_end[i] = rp[r0 + 1 + i] - n0;
-    }
-    __syncthreads();
-
-    const int di = min(tid * IPT, tot);
-    const int de = min(di + IPT, tot);
-    int lo = max(0, di - nn), hi = min(di, nr);
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (s_end[mid] <= di - mid - 1)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    const int rs = lo, ns = di - lo;
-
-    // pass 1: does this lane close a row, and what is left open at its end
-    A acc;
-    acc.zero();
-    int fl = 0;
-    {
-        int r = rs, n = ns;
-#pragma unroll
-        for (int k = 0; k < IPT; ++k) {
-            if (di + k < de) {
-                if (r < nr && s_end[r] <= n) {
-                    acc.zero();
-                    fl = 1;
-                    ++r;
-                } else {
-                    s_prod[n].into(acc);
-                    ++n;
-                }
-            }
-        }
-    }
-    A total;
-    A ex = block_seg_scan_excl<A, B / 64>(acc, fl, s_agg, s_flag, &total);
-
-    // pass 2: emit rows, the first one including the sum carried in
-    {
-        A a = ex;
-        int r = rs, n = ns;
-#pragma unroll
-        for (int k = 0; k < IPT; ++k) {
-            if (di + k < de) {
-                if (r < nr && s_end[r] <= n) {
-                    store_y(y, (long)r0 + r, a.value(), alpha, beta);
-                    a.zero();
-                    ++r;
-                } else {
-                    s_prod[n].into(a);
-                    ++n;
-                }
-            }
-        }
-    }
-    if (tid == 0) carry[p] = total;
+    __shared__ MergeSmem<AT, COMP, IPT> sm;
+    const int p = blockIdx.x;
+    Acc<AT, COMP> total = merge_tile<VT, XT, AT, COMP, IPT>(sm, rp, ci, vals, x, y, pdiag[p], prow[p], pdiag[p + 1],
+                                                            prow[p + 1], alpha, beta);
+    if (threadIdx.x == 0) carry[p] = total;
 }
 
 // Adds carried partial sums to y. Carry t belongs to row key(t) =

@@ -1,0 +1,154 @@
+// One merge-path tile on a 256-lane workgroup: shared by the single-matrix
+// merge kernel and the batched persistent kernel.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace muxi {
+
+// Workgroup-wide inclusive segmented scan over (flag, value) in lane order:
+// (f1,v1) then (f2,v2) -> (f1|f2, f2 ? v2 : v1+v2). Returns the exclusive
+// value (sum carried into this lane) and the workgroup total in *tot.
+template <typename A, int NW>
+__device__ __forceinline__ A block_seg_scan_excl(A v, int f, A* s_agg, int* s_flag, A* tot) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        A vu = v.shfl_up(d);
+        int fu = __shfl_up(f, d);
+        if (lane >= d) {
+            if (!f) {
+                vu.add(v);
+                v = vu;
+            }
+            f |= fu;
+        }
+    }
+    if (lane == 63) {
+        s_agg[w] = v;
+        s_flag[w] = f;
+    }
+    __syncthreads();
+    A pre;
+    pre.zero();
+    for (int u = 0; u < w; ++u) {
+        if (s_flag[u])
+            pre = s_agg[u];
+        else
+            pre.add(s_agg[u]);
+    }
+    A inc = v;
+    if (!f) {
+        A t = pre;
+        t.add(v);
+        inc = t;
+    }
+    A ex = inc.shfl_up(1);
+    if (lane == 0) ex = pre;
+    A t;
+    t.zero();
+    for (int u = 0; u < NW; ++u) {
+        if (s_flag[u])
+            t = s_agg[u];
+        else
+            t.add(s_agg[u]);
+    }
+    *tot = t;
+    return ex;
+}
+
+template <typename AT, bool COMP, int IPT>
+struct MergeSmem {
+    int end[256 * IPT];
+    Prod<AT, COMP> prod[256 * IPT];
+    Acc<AT, COMP> agg[4];
+    int flag[4];
+};
+
+// Processes path items [d0, d1) with coordinates (r0, d0-r0) .. (r1, d1-r1):
+// writes y for rows r0..r1-1 and returns the partial sum of row r1 held in
+// this tile (the same value in every lane). Ends with LDS still in use: the
+// caller must __syncthreads() before reusing sm.
+template <typename VT, typename XT, typename AT, bool COMP, int IPT>
+__device__ __forceinline__ Acc<AT, COMP> merge_tile(MergeSmem<AT, COMP, IPT>& sm, const int* __restrict__ rp,
+                                                    const int* __restrict__ ci, const VT* __restrict__ vals,
+                                                    const XT* __restrict__ x, XT* __restrict__ y, int d0, int r0,
+                                                    int d1, int r1, AT alpha, AT beta) {
+    constexpr int B = 256;
+    using A = Acc<AT, COMP>;
+    using P = Prod<AT, COMP>;
+    const int tid = threadIdx.x;
+    const int n0 = d0 - r0, n1 = d1 - r1;
+    const int nr = r1 - r0, nn = n1 - n0, tot = nr + nn;
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        int j = tid + k * B;
+        if (j < nn) {
+            int c = ci[n0 + j];
+            sm.prod[j] = P::make(vals[n0 + j], x[c]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        int i = tid + k * B;
+        if (i < nr) sm.end[i] = rp[r0 + 1 + i] - n0;
+    }
+    __syncthreads();
+
+    const int di = min(tid * IPT, tot);
+    const int de = min(di + IPT, tot);
+    int lo = max(0, di - nn), hi = min(di, nr);
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (sm.end[mid] <= di - mid - 1)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    const int rs = lo, ns = di - lo;
+
+    // pass 1: does this lane close a row, and what is left open at its end
+    A acc;
+    acc.zero();
+    int fl = 0;
+    {
+        int r = rs, n = ns;
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            if (di + k < de) {
+                if (r < nr && sm.end[r] <= n) {
+                    acc.zero();
+                    fl = 1;
+                    ++r;
+                } else {
+                    sm.prod[n].into(acc);
+                    ++n;
+                }
+            }
+        }
+    }
+    A total;
+    A ex = block_seg_scan_excl<A, B / 64>(acc, fl, sm.agg, sm.flag, &total);
+
+    // pass 2: emit rows, the first one including the sum carried in
+    {
+        A a = ex;
+        int r = rs, n = ns;
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            if (di + k < de) {
+                if (r < nr && sm.end[r] <= n) {
+                    store_y(y, (long)r0 + r, a.value(), alpha, beta);
+                    a.zero();
+                    ++r;
+                } else {
+                    sm.prod[n].into(a);
+                    ++n;
+                }
+            }
+        }
+    }
+    return total;
+}
+
+}  // namespace muxi
