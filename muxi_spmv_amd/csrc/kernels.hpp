@@ -2,6 +2,8 @@
 // given stream and returns SPMV_SUCCESS or SPMV_ERROR_DEVICE; none allocates.
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace muxi {
@@ -68,6 +70,14 @@ int spmm_chunk_items(int k, int ipt);
 template <typename VT, typename XT>
 spmv_status_t launch_spmm(const DevCsr& A, int k, int ipt, const int* d_pos, const int* d_row, int P, void* carry,
                           const XT* X, XT* Y, XT alpha, XT beta, hipStream_t s, hipEvent_t ev = nullptr);
+
+// ---- adaptive row blocks (kernels_adaptive.hip)
+spmv_status_t adaptive_build(const int* d_rp, int R, int nb, hipStream_t s, std::vector<int>& blk_row,
+                             std::vector<int>& blk_chunk, std::vector<int>& long_row, std::vector<int>& long_first);
+template <typename VT, typename XT, typename AT, bool COMP>
+spmv_status_t launch_adaptive(const DevCsr& A, int nb, int nblocks, const int* blk_row, const int* blk_chunk,
+                              int nlong, const int* long_row, const int* long_first, void* partial, const XT* x, XT* y,
+                              AT alpha, AT beta, hipStream_t s, hipEvent_t ev = nullptr);
 
 // ---- misc (kernels_misc.hip)
 template <typename VT, typename XT>

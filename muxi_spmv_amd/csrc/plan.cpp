@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <new>
+#include <vector>
 
 #include "plan.hpp"
 
@@ -135,6 +136,35 @@ extern "C" spmv_status_t muxi_plan_create(const muxi_csr_t* Ain, spmv_type_t vt,
                 PLAN_HIP(hipMalloc(&p->d_carry, acc_bytes * (size_t)T));
                 p->scratch_bytes = 2 * sizeof(int) * (size_t)T + acc_bytes * (size_t)T;
                 PLAN_TRY(launch_csr5_tiles(p->A.rp, R, N, param0, T, p->d_tile_row, p->d_tile_off, s));
+            }
+            break;
+        }
+        case MUXI_KERNEL_ADAPTIVE: {
+            if (param0 <= 0) param0 = 1024;
+            if (!(param0 == 256 || param0 == 512 || param0 == 1024 || param0 == 2048)) {
+                set_error("adaptive: block nonzeros must be 256, 512, 1024 or 2048");
+                plan_free(p);
+                return SPMV_ERROR_INTERNAL;
+            }
+            p->param0 = param0;
+            if (R > 0) {
+                std::vector<int> br, bc, lr, lf;
+                PLAN_TRY(adaptive_build(p->A.rp, R, param0, s, br, bc, lr, lf));
+                p->nblocks = (int)bc.size();
+                p->nlong = (int)lr.size();
+                const size_t ints = br.size() + bc.size() + lr.size() + lf.size();
+                PLAN_HIP(hipMalloc(&p->d_blocks, sizeof(int) * ints));
+                p->d_blk_chunk = p->d_blocks + br.size();
+                p->d_long_row = p->d_blk_chunk + bc.size();
+                p->d_long_first = p->d_long_row + lr.size();
+                PLAN_HIP(hipMemcpy(p->d_blocks, br.data(), sizeof(int) * br.size(), hipMemcpyHostToDevice));
+                PLAN_HIP(hipMemcpy(p->d_blk_chunk, bc.data(), sizeof(int) * bc.size(), hipMemcpyHostToDevice));
+                if (!lr.empty()) {
+                    PLAN_HIP(hipMemcpy(p->d_long_row, lr.data(), sizeof(int) * lr.size(), hipMemcpyHostToDevice));
+                    PLAN_HIP(hipMemcpy(p->d_long_first, lf.data(), sizeof(int) * lf.size(), hipMemcpyHostToDevice));
+                }
+                PLAN_HIP(hipMalloc(&p->d_carry, acc_bytes * (size_t)(p->nblocks > 0 ? p->nblocks : 1)));
+                p->scratch_bytes = sizeof(int) * ints + acc_bytes * (size_t)p->nblocks;
             }
             break;
         }
@@ -313,12 +343,17 @@ static spmv_status_t plan_exec(const muxi_plan* p, const void* x, void* y, doubl
                                                   ev1);
             case MUXI_KERNEL_ELL:
                 return launch_ell<VT, XT, AT, C>(p->A.R, p->width, p->d_ecol, (const VT*)p->d_eval, xx, yy, a, b, s);
+            case MUXI_KERNEL_ADAPTIVE:
+                return launch_adaptive<VT, XT, AT, C>(p->A, p->param0, p->nblocks, p->d_blocks, p->d_blk_chunk,
+                                                      p->nlong, p->d_long_row, p->d_long_first, p->d_carry, xx, yy, a,
+                                                      b, s, ev1);
             default: set_error("plan kernel not executable"); return SPMV_ERROR_INTERNAL;
         }
     });
     if (st != SPMV_SUCCESS) return st;
     // single-kernel paths: the main kernel ends with the exec
-    if (ev1 && !((p->kernel == MUXI_KERNEL_MERGE && p->A.R > 0) || (p->kernel == MUXI_KERNEL_CSR5 && p->T > 0)))
+    if (ev1 && !((p->kernel == MUXI_KERNEL_MERGE && p->A.R > 0) || (p->kernel == MUXI_KERNEL_CSR5 && p->T > 0) ||
+                 (p->kernel == MUXI_KERNEL_ADAPTIVE && p->A.R > 0 && p->nblocks > 0)))
         MUXI_HIP(hipEventRecord(ev1, s));
     return SPMV_SUCCESS;
 }

@@ -25,9 +25,13 @@ struct muxi_plan {
     int width = 0;
     int* d_ecol = nullptr;
     void* d_eval = nullptr;
-    // adaptive row blocks
+    // adaptive row blocks: blk_row[nblocks+1], blk_chunk[nblocks], long_row[nlong], long_first[2*nlong]
     int nblocks = 0;
+    int nlong = 0;
     int* d_blocks = nullptr;
+    int* d_blk_chunk = nullptr;
+    int* d_long_row = nullptr;
+    int* d_long_first = nullptr;
     size_t scratch_bytes = 0;
     // SpMM: right-hand sides (0 for a SpMV plan)
     int nrhs = 0;

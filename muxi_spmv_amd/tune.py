@@ -30,6 +30,7 @@ CANDIDATES: list[tuple[str, int]] = [
     ("vwave", 2), ("vwave", 4), ("vwave", 8), ("vwave", 16), ("vwave", 32),
     ("merge", 4), ("merge", 8), ("merge", 16),
     ("csr5", 256), ("csr5", 512), ("csr5", 1024),
+    ("adaptive", 512), ("adaptive", 1024), ("adaptive", 2048),
     ("ell", 0),
 ]
 CAND_NAMES = [f"{k}:{p}" for k, p in CANDIDATES]

@@ -16,7 +16,8 @@ pytestmark = pytest.mark.gpu
 
 ZOO = matrix_zoo()
 KCONF = [("scalar", 0), ("vector", 0)] + [("vwave", k) for k in (1, 2, 4, 8, 16, 32, 64)] + \
-        [("merge", k) for k in (2, 4, 8, 16)] + [("csr5", s) for s in (64, 128, 256, 512, 1024)] + [("ell", 0)]
+        [("merge", k) for k in (2, 4, 8, 16)] + [("csr5", s) for s in (64, 128, 256, 512, 1024)] + [("ell", 0)] + \
+        [("adaptive", b) for b in (256, 512, 1024, 2048)]
 
 
 def xvec(n, seed=11, dtype=np.float32):
