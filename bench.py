@@ -108,6 +108,20 @@ def main(argv=None) -> int:
     ys = [torch.zeros(h.numRows, dtype=tdt, device="cuda") for _ in range(ncopy)]
     log(f"{ncopy} rotating copies of {bytes_step / 1e6:.1f} MB")
 
+    # achievable HBM bandwidth: a plain 1 GiB device-to-device copy (read + write)
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(10):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    copy_gbs = 2 * src.numel() * 4 * 10 / (time.perf_counter() - t) / 1e9
+    del src, dst
+    log(f"device copy: {copy_gbs:.0f} GB/s")
+
     sweep = {}
     if a.kernel == "sweep":
         cands = [c for c in CANDIDATES if not (c[0] in ("vector", "scalar") and h.nnz > 50_000_000)]
@@ -158,6 +172,7 @@ def main(argv=None) -> int:
         "gflops": round(calculate_gflops(h.nnz, ms), 3),
         "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
         "rel_l2_vs_fp64": rel_l2,
+        "hbm_copy_gbs": round(copy_gbs, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel_us_mean": round(kms * 1e3, 3), "bytes_per_launch": bytes_step},

@@ -28,9 +28,12 @@ def fp64_bound(rowPtr, absrow):
     return (n + 2) * U64 * absrow * 1.0001 + 1e-300
 
 
-def assert_close_fp32(y, h: G.HostCSR, x, values=None, rel_l2=1e-6):
+def assert_close_fp32(y, h: G.HostCSR, x, values=None, rel_l2=1e-5):
     """y (FP32 result) vs the FP64 oracle on the same stored values and x:
-    per-row worst-case bound and a relative L2 error of at most rel_l2."""
+    the per-row worst-case bound, and a relative L2 error of at most rel_l2.
+    1e-5 is the general cap: a single 60,000-term row can dominate the norm
+    with a legitimately larger relative error; the benchmark workloads are
+    held to 1e-6 where they are tested."""
     v = h.values if values is None else values
     y64, ab = oracle.spmv_f64(h.rowPtr, h.colIdx, np.asarray(v, np.float64), np.asarray(x, np.float64))
     yy = y.detach().cpu().double().numpy() if torch.is_tensor(y) else np.asarray(y, np.float64)

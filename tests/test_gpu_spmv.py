@@ -270,3 +270,67 @@ def test_cpp_reference_caller():
     r = subprocess.run([str(exe), "4099"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "PASSED" in r.stdout
+
+
+# ---------------------------------------------------------------- full-size properties
+@pytest.fixture(scope="module")
+def powerlaw_full():
+    h = G.cfg_powerlaw_2m()
+    return h, dev_csr(h)
+
+
+@pytest.mark.parametrize("kernel,param", [("merge", 4), ("merge", 8), ("csr5", 256), ("adaptive", 1024)])
+def test_full_size_powerlaw_properties(powerlaw_full, kernel, param):
+    """2M x 2M power-law (14.4M nnz): rel-L2 <= 1e-6 vs the device FP64 oracle,
+    bitwise reproducible, exactly homogeneous under x -> 2x, additive within tolerance."""
+    h, A = powerlaw_full
+    x = torch.from_numpy(G.uniform_vector(h.numCols, 21)).cuda()
+    x2 = torch.from_numpy(G.uniform_vector(h.numCols, 22)).cuda()
+    p = api.Plan(A, kernel, param)
+    y = torch.empty(h.numRows, device="cuda")
+    p.exec(x, y)
+    y64 = api.oracle_f64(A, x)
+    torch.cuda.synchronize()
+    rl2 = float(torch.linalg.vector_norm(y.double() - y64) / torch.linalg.vector_norm(y64))
+    assert rl2 <= 1e-6, rl2
+    yb = torch.empty_like(y)
+    p.exec(x, yb)
+    y2 = torch.empty_like(y)
+    p.exec(2 * x, y2)
+    ys = torch.empty_like(y)
+    p.exec(x + x2, ys)
+    ya = torch.empty_like(y)
+    p.exec(x2, ya)
+    torch.cuda.synchronize()
+    assert torch.equal(y, yb)
+    assert torch.equal(y2, 2 * y)
+    ab = api.oracle_f64(api.CSRMatrix(A.d_rowPtr, A.d_colIdx, A.d_values.abs(), h.numCols), x.abs() + x2.abs())
+    torch.cuda.synchronize()
+    n = torch.from_numpy(np.diff(h.rowPtr.astype(np.int64))).cuda().double()
+    assert torch.all((ys.double() - (y.double() + ya.double())).abs() <= 3 * (n + 2) * 2.0 ** -24 * ab)
+
+
+def test_full_size_laplacian_exact():
+    """4096^2 5-point Laplacian, FP64, x = 1: y is exactly 0 / 1 / 2 (interior /
+    edge / corner) for CSR (merge, csr5, vwave) and ELL, at full size."""
+    n = 4096
+    h = G.cfg_laplacian_4096()
+    A = dev_csr(h, torch.float64)
+    x = torch.ones(n * n, dtype=torch.float64, device="cuda")
+    exp = torch.zeros((n, n), dtype=torch.float64)
+    exp[0, :] += 1
+    exp[-1, :] += 1
+    exp[:, 0] += 1
+    exp[:, -1] += 1
+    exp = exp.ravel().cuda()
+    for kernel, param in [("merge", 8), ("csr5", 256), ("vwave", 4), ("ell", 0), ("adaptive", 1024)]:
+        y = torch.full((n * n,), float("nan"), dtype=torch.float64, device="cuda")
+        api.Plan(A, kernel, param, xtype=torch.float64).exec(x, y)
+        torch.cuda.synchronize()
+        assert torch.equal(y, exp), kernel
+    E = api.csr_to_ellpack(A)
+    assert E.numElements == 5
+    y = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+    api.spmv_ellpack(E, x, y)
+    torch.cuda.synchronize()
+    assert torch.equal(y, exp)
