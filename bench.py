@@ -26,6 +26,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 CANDIDATES = [("merge", 4), ("merge", 8), ("merge", 16), ("csr5", 256), ("csr5", 512), ("csr5", 1024),
+              ("adaptive", 512), ("adaptive", 1024), ("adaptive", 2048), ("ell", 0),
               ("vwave", 4), ("vwave", 8), ("vector", 0), ("scalar", 0)]
 
 
@@ -124,7 +125,9 @@ def main(argv=None) -> int:
 
     sweep = {}
     if a.kernel == "sweep":
-        cands = [c for c in CANDIDATES if not (c[0] in ("vector", "scalar") and h.nnz > 50_000_000)]
+        maxlen = int(np.diff(h.rowPtr).max())
+        cands = [c for c in CANDIDATES if not (c[0] in ("vector", "scalar") and h.nnz > 50_000_000)
+                 and not (c[0] == "ell" and h.numRows * maxlen > 4 * h.nnz)]
         for k, p in cands:
             plans = [api.Plan(m, k, p, xtype=tdt) for m in mats]
             m = measure_plans(plans, xs, ys, a.sweep_steps, 3, record_kernel=False)
