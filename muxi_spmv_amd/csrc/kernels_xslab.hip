@@ -1,0 +1,334 @@
+// Column-slab merge-path SpMV ("xslab"): keeps each XCD's x gathers in its
+// own L2.
+//
+// MI355X has 8 XCDs, each with a private 4 MiB L2, and hands workgroup i to
+// XCD i mod 8. When x is larger than one L2 (the 2M-column power-law matrix
+// has an 8 MB x) and the columns are spread, every XCD streams all of x
+// through its L2 and the gathers, not the nonzero stream, set the time. The
+// plan splits the columns into S slabs (column c is in slab floor(c*S/C))
+// and stores, per slab, a compressed CSR of the nonzeros that fall in it:
+// the ids of the rows that have any, slab-local row pointers, columns and
+// values. Workgroup b works on slab b mod S, so with S = 8 an XCD gathers
+// from one 1/8 slice of x (1 MB here) that stays resident in its L2; the
+// matrix stream uses non-temporal loads so it does not evict that slice.
+// Each slab is a merge path over its compressed rows (the shared tile body's
+// algorithm), and row sums, including a tile's open row, are added to y with
+// hardware float atomics after y is scaled by beta. The atomics make the last
+// bits of y depend on the order in which slabs finish.
+#include <hipcub/hipcub.hpp>
+
+#include <climits>
+#include <vector>
+
+#include "merge_tile.hpp"
+#include "plan.hpp"
+
+namespace muxi {
+
+template <typename T>
+__device__ __forceinline__ T ntl(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+__device__ __forceinline__ bf16_t ntl(const bf16_t* p) {
+    bf16_t b;
+    b.bits = __builtin_nontemporal_load(&p->bits);
+    return b;
+}
+
+template <typename VT, typename XT, typename AT, int IPT>
+__global__ __launch_bounds__(256) void xslab_kernel(SlabArgs sa, const int* __restrict__ srp,
+                                                    const int* __restrict__ rid, const int* __restrict__ sci,
+                                                    const VT* __restrict__ svals, const XT* __restrict__ x,
+                                                    XT* __restrict__ y, const int* __restrict__ gdiag,
+                                                    const int* __restrict__ grow, AT alpha) {
+    constexpr int B = 256, CH = B * IPT;
+    using A = Acc<AT, false>;
+    __shared__ int s_end[CH];
+    __shared__ AT s_prod[CH];
+    __shared__ A s_agg[4];
+    __shared__ int s_flag[4];
+    const int s = blockIdx.x % sa.S, t = blockIdx.x / sa.S;
+    if (t >= sa.tiles[s]) return;  // uniform over the workgroup
+    const int tid = threadIdx.x;
+    const int* rp = srp + sa.row_base[s] + s;
+    const int* ids = rid + sa.row_base[s];
+    const int* ci = sci + sa.nnz_base[s];
+    const VT* vals = svals + sa.nnz_base[s];
+    const int c = sa.coord_base[s] + t;
+    const int d0 = gdiag[c], r0 = grow[c], d1 = gdiag[c + 1], r1 = grow[c + 1];
+    const int n0 = d0 - r0, n1 = d1 - r1;
+    const int nr = r1 - r0, nn = n1 - n0, tot = nr + nn;
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const int j = tid + k * B;
+        if (j < nn) {
+            const int col = ntl(ci + n0 + j);
+            s_prod[j] = Conv<AT>::f(ntl(vals + n0 + j)) * Conv<AT>::f(x[col]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const int i = tid + k * B;
+        if (i < nr) s_end[i] = ntl(rp + r0 + 1 + i) - n0;
+    }
+    __syncthreads();
+    const int di = min(tid * IPT, tot), de = min(di + IPT, tot);
+    int lo = max(0, di - nn), hi = min(di, nr);
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s_end[mid] <= di - mid - 1)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    const int rs = lo, ns = di - lo;
+    A acc;
+    acc.zero();
+    int fl = 0;
+    {
+        int r = rs, n = ns;
+#pragma unroll
+        for (int k = 0; k < IPT; ++k)
+            if (di + k < de) {
+                if (r < nr && s_end[r] <= n) {
+                    acc.zero();
+                    fl = 1;
+                    ++r;
+                } else {
+                    acc.s += s_prod[n++];
+                }
+            }
+    }
+    A total;
+    A ex = block_seg_scan_excl<A, B / 64>(acc, fl, s_agg, s_flag, &total);
+    {
+        A a = ex;
+        int r = rs, n = ns;
+#pragma unroll
+        for (int k = 0; k < IPT; ++k)
+            if (di + k < de) {
+                if (r < nr && s_end[r] <= n) {
+                    atomicAdd(y + ntl(ids + r0 + r), (XT)(alpha * a.s));
+                    a.zero();
+                    ++r;
+                } else {
+                    a.s += s_prod[n++];
+                }
+            }
+    }
+    // the row left open at the tile end, if this tile holds any of it
+    if (tid == 0 && r1 < sa.rows[s] && n1 > ntl(rp + r1)) atomicAdd(y + ntl(ids + r1), (XT)(alpha * total.s));
+}
+
+// ---- plan construction
+__device__ __forceinline__ int slab_of(int col, int S, int C) { return (int)(((long)col * S) / C); }
+
+__global__ void xslab_count_kernel(const int* __restrict__ rp, const int* __restrict__ ci, int R, int C, int S,
+                                   int* __restrict__ cnt, int* __restrict__ flag) {
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    int c[kMaxSlabs];
+#pragma unroll
+    for (int s = 0; s < kMaxSlabs; ++s) c[s] = 0;
+    for (int i = rp[r]; i < rp[r + 1]; ++i) {
+        const int s = slab_of(ci[i], S, C);
+#pragma unroll
+        for (int q = 0; q < kMaxSlabs; ++q) c[q] += (q == s);
+    }
+#pragma unroll
+    for (int s = 0; s < kMaxSlabs; ++s)
+        if (s < S) {
+            cnt[(long)s * R + r] = c[s];
+            flag[(long)s * R + r] = c[s] > 0;
+        }
+}
+
+template <typename VT>
+__global__ void xslab_fill_kernel(const int* __restrict__ rp, const int* __restrict__ ci, const VT* __restrict__ v,
+                                  int R, int C, int S, const int* __restrict__ cnt, const int* __restrict__ pos,
+                                  const int* __restrict__ lrow,
+                                  const int* __restrict__ nnz_base, const int* __restrict__ row_base,
+                                  int* __restrict__ srp, int* __restrict__ rid, int* __restrict__ sci,
+                                  VT* __restrict__ sv) {
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    int off[kMaxSlabs];
+#pragma unroll
+    for (int s = 0; s < kMaxSlabs; ++s) off[s] = 0;
+    for (int s = 0; s < S; ++s) {
+        const long k = (long)s * R + r;
+        const int p = pos[k];
+        if (cnt[k] > 0) {  // row r has nonzeros in slab s
+            const int lr = lrow[k];
+            rid[lr] = (int)r;
+            srp[lr + s] = p - nnz_base[s];
+        }
+#pragma unroll
+        for (int q = 0; q < kMaxSlabs; ++q)
+            if (q == s) off[q] = p;
+    }
+    for (int i = rp[r]; i < rp[r + 1]; ++i) {
+        const int col = ci[i];
+        const int s = slab_of(col, S, C);
+        int o = 0;
+#pragma unroll
+        for (int q = 0; q < kMaxSlabs; ++q)
+            if (q == s) o = off[q]++;
+        sci[o] = col;
+        sv[o] = v[i];
+    }
+}
+
+static spmv_status_t exscan(const int* in, int* out, long n, hipStream_t s) {
+    size_t tmp = 0;
+    MUXI_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, s));
+    void* d_tmp = nullptr;
+    MUXI_HIP(hipMallocAsync(&d_tmp, tmp ? tmp : 16, s));
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp, in, out, (int)n, s);
+    (void)hipFreeAsync(d_tmp, s);
+    if (e != hipSuccess) {
+        set_error(std::string("scan: ") + hipGetErrorString(e));
+        return SPMV_ERROR_DEVICE;
+    }
+    return SPMV_SUCCESS;
+}
+
+spmv_status_t xslab_build(muxi_plan* p, int S, int ipt, spmv_type_t vt, hipStream_t s) {
+    const int R = p->A.R, C = p->A.C, N = p->A.N;
+    if (S < 1 || S > kMaxSlabs) {
+        set_error("xslab: slabs must be in [1, 16]");
+        return SPMV_ERROR_INTERNAL;
+    }
+    if ((long)S * R + 1 > INT_MAX) {
+        set_error("xslab: rows * slabs exceeds INT32_MAX");
+        return SPMV_ERROR_INVALID_MATRIX;
+    }
+    const long SR = (long)S * R;
+    int *cnt, *flag, *pos, *lrow;
+    MUXI_HIP(hipMallocAsync(&cnt, sizeof(int) * (SR + 1), s));
+    MUXI_HIP(hipMallocAsync(&flag, sizeof(int) * (SR + 1), s));
+    MUXI_HIP(hipMallocAsync(&pos, sizeof(int) * (SR + 1), s));
+    MUXI_HIP(hipMallocAsync(&lrow, sizeof(int) * (SR + 1), s));
+    MUXI_HIP(hipMemsetAsync(cnt + SR, 0, sizeof(int), s));
+    MUXI_HIP(hipMemsetAsync(flag + SR, 0, sizeof(int), s));
+    hipLaunchKernelGGL(xslab_count_kernel, dim3(grid_for(R, 256)), dim3(256), 0, s, p->A.rp, p->A.ci, R, C, S, cnt,
+                       flag);
+    MUXI_LAUNCH_CHECK();
+    spmv_status_t st = exscan(cnt, pos, SR + 1, s);
+    if (st == SPMV_SUCCESS) st = exscan(flag, lrow, SR + 1, s);
+    if (st != SPMV_SUCCESS) return st;
+    // per-slab bases: pos/lrow at s*R, s = 0..S
+    std::vector<int> hpos(S + 1), hrow(S + 1);
+    for (int q = 0; q <= S; ++q) {
+        MUXI_HIP(hipMemcpyAsync(&hpos[q], pos + (long)q * R, sizeof(int), hipMemcpyDeviceToHost, s));
+        MUXI_HIP(hipMemcpyAsync(&hrow[q], lrow + (long)q * R, sizeof(int), hipMemcpyDeviceToHost, s));
+    }
+    MUXI_HIP(hipStreamSynchronize(s));
+    const int total_rows = hrow[S];
+    SlabArgs& sa = p->slab;
+    sa.S = S;
+    for (int q = 0; q < S; ++q) {
+        sa.rows[q] = hrow[q + 1] - hrow[q];
+        sa.row_base[q] = hrow[q];
+        sa.nnz_base[q] = hpos[q];
+    }
+    int* d_bases = nullptr;
+    MUXI_HIP(hipMallocAsync(&d_bases, sizeof(int) * 2 * S, s));
+    MUXI_HIP(hipMemcpyAsync(d_bases, hpos.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
+    MUXI_HIP(hipMemcpyAsync(d_bases + S, hrow.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
+    const size_t vs = vsize(vt);
+    // one allocation: rowPtr (total_rows + S), row ids (total_rows), columns (N), then values
+    const size_t ints = (size_t)total_rows * 2 + S + (size_t)N;
+    MUXI_HIP(hipMalloc(&p->d_slab, sizeof(int) * ints + vs * (size_t)(N > 0 ? N : 1) + 256));
+    p->d_slab_rp = p->d_slab;
+    p->d_slab_rid = p->d_slab_rp + total_rows + S;
+    p->d_slab_ci = p->d_slab_rid + total_rows;
+    uintptr_t va = (uintptr_t)(p->d_slab_ci + N);
+    va = (va + 255) & ~(uintptr_t)255;
+    p->d_slab_v = (void*)va;
+    // slab rowPtr terminators
+    for (int q = 0; q < S; ++q) {
+        const int nq = hpos[q + 1] - hpos[q];
+        MUXI_HIP(hipMemcpyAsync(p->d_slab_rp + sa.row_base[q] + q + sa.rows[q], &nq, sizeof(int),
+                                hipMemcpyHostToDevice, s));
+        MUXI_HIP(hipStreamSynchronize(s));  // nq is a stack variable
+    }
+    st = with_types(vt, p->xtype, p->comp, [&](auto tag) -> spmv_status_t {
+        using T = decltype(tag);
+        using VT = typename T::VT;
+        hipLaunchKernelGGL(xslab_fill_kernel<VT>, dim3(grid_for(R, 256)), dim3(256), 0, s, p->A.rp, p->A.ci,
+                           (const VT*)p->A.v, R, C, S, cnt, pos, lrow, d_bases, d_bases + S, p->d_slab_rp,
+                           p->d_slab_rid, p->d_slab_ci, (VT*)p->d_slab_v);
+        MUXI_LAUNCH_CHECK();
+        return SPMV_SUCCESS;
+    });
+    if (st != SPMV_SUCCESS) return st;
+    // merge partitions per slab
+    const long CH = 256L * ipt;
+    long ncoord = 0;
+    for (int q = 0; q < S; ++q) {
+        const int nq = hpos[q + 1] - hpos[q];
+        sa.tiles[q] = sa.rows[q] > 0 ? (int)(((long)sa.rows[q] + nq + CH - 1) / CH) : 0;
+        sa.coord_base[q] = (int)ncoord;
+        ncoord += sa.tiles[q] + 1;
+    }
+    MUXI_HIP(hipMalloc(&p->d_pos, 2 * sizeof(int) * (size_t)ncoord));
+    p->d_row = p->d_pos + ncoord;
+    for (int q = 0; q < S; ++q) {
+        if (sa.tiles[q] == 0) continue;
+        const int nq = hpos[q + 1] - hpos[q];
+        st = launch_merge_partitions(sa.rows[q], nq, sa.tiles[q], p->d_pos + sa.coord_base[q], s);
+        if (st == SPMV_SUCCESS)
+            st = launch_merge_coords(p->d_slab_rp + sa.row_base[q] + q, sa.rows[q], nq, p->d_pos + sa.coord_base[q],
+                                     sa.tiles[q] + 1, p->d_row + sa.coord_base[q], nullptr, s);
+        if (st != SPMV_SUCCESS) return st;
+    }
+    int maxt = 0;
+    for (int q = 0; q < S; ++q) maxt = sa.tiles[q] > maxt ? sa.tiles[q] : maxt;
+    p->P = maxt;
+    (void)hipFreeAsync(cnt, s);
+    (void)hipFreeAsync(flag, s);
+    (void)hipFreeAsync(pos, s);
+    (void)hipFreeAsync(lrow, s);
+    (void)hipFreeAsync(d_bases, s);
+    MUXI_HIP(hipStreamSynchronize(s));
+    p->scratch_bytes = sizeof(int) * ints + vs * (size_t)N + 2 * sizeof(int) * (size_t)ncoord;
+    return SPMV_SUCCESS;
+}
+
+template <typename VT, typename XT, typename AT>
+spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT beta, hipStream_t s, hipEvent_t ev0) {
+    if (p->A.R <= 0) return SPMV_SUCCESS;
+    spmv_status_t st = launch_scale_y<XT>(y, p->A.R, (XT)beta, s);
+    if (st != SPMV_SUCCESS) return st;
+    if (ev0) MUXI_HIP(hipEventRecord(ev0, s));
+    const SlabArgs& sa = p->slab;
+    if (p->P == 0) return SPMV_SUCCESS;
+    const dim3 grid((unsigned)((long)p->P * sa.S));
+    switch (p->param0 & 0xff) {
+        case 4:
+            hipLaunchKernelGGL((xslab_kernel<VT, XT, AT, 4>), grid, dim3(256), 0, s, sa, p->d_slab_rp, p->d_slab_rid,
+                               p->d_slab_ci, (const VT*)p->d_slab_v, x, y, p->d_pos, p->d_row, alpha);
+            break;
+        case 16:
+            hipLaunchKernelGGL((xslab_kernel<VT, XT, AT, 16>), grid, dim3(256), 0, s, sa, p->d_slab_rp, p->d_slab_rid,
+                               p->d_slab_ci, (const VT*)p->d_slab_v, x, y, p->d_pos, p->d_row, alpha);
+            break;
+        default:
+            hipLaunchKernelGGL((xslab_kernel<VT, XT, AT, 8>), grid, dim3(256), 0, s, sa, p->d_slab_rp, p->d_slab_rid,
+                               p->d_slab_ci, (const VT*)p->d_slab_v, x, y, p->d_pos, p->d_row, alpha);
+    }
+    MUXI_LAUNCH_CHECK();
+    return SPMV_SUCCESS;
+}
+
+template spmv_status_t launch_xslab<float, float, float>(const muxi_plan*, const float*, float*, float, float,
+                                                         hipStream_t, hipEvent_t);
+template spmv_status_t launch_xslab<bf16_t, float, float>(const muxi_plan*, const float*, float*, float, float,
+                                                          hipStream_t, hipEvent_t);
+template spmv_status_t launch_xslab<double, float, float>(const muxi_plan*, const float*, float*, float, float,
+                                                          hipStream_t, hipEvent_t);
+template spmv_status_t launch_xslab<double, double, double>(const muxi_plan*, const double*, double*, double,
+                                                            double, hipStream_t, hipEvent_t);
+
+}  // namespace muxi

