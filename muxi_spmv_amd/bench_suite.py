@@ -1,0 +1,220 @@
+"""Secondary benchmark workloads (one JSON line each on stdout).
+
+python -m muxi_spmv_amd.bench_suite precision|batched|spmm|laplacian [--out FILE]
+
+precision  values stored FP64 / FP32 / BF16, x FP32, FP32 accumulation plain
+           and compensated (Dot2), on the 2M power-law matrix and on a
+           4M-row banded matrix (half-bandwidth 64, 16 nnz/row, N(0,1)) after
+           RCM; rel-L2 and max-abs error of y against the FP64 oracle (the
+           FP64 values and x), plus time and GB/s for each.
+batched    256 CSR matrices, rows log-uniform in [1e3, 2e5], mean nnz/row
+           log-uniform in [4, 64], Poisson row lengths: one persistent
+           launch (eager and hipGraph) against one CSR-vector launch per
+           matrix (eager and hipGraph).
+spmm       Y = A X on the power-law matrix for k = 8 and 32 (row-major X).
+laplacian  4096^2 5-point Laplacian in FP64, CSR kernels and ELLPACK.
+All timings are cold: each timed launch is preceded by a 512 MiB cache flush,
+and HIP events bracket the launch.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import statistics
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import api, prep, tune
+from . import generators as G
+from .benchmark import HBM_PEAK_GBS, algorithmic_bytes
+from ._lib import check, load
+
+
+def log(msg):
+    print(f"[suite] {msg}", file=sys.stderr, flush=True)
+
+
+class Timer:
+    def __init__(self):
+        self.ev = [C.c_void_p(), C.c_void_p()]
+        for e in self.ev:
+            check(load().muxi_event_create(C.byref(e)), "event_create")
+        self.fl = tune.Flusher()
+
+    def time(self, fn, reps=10, flush=True) -> float:
+        """Median ms of fn() over reps, each preceded by a cache flush."""
+        s = torch.cuda.current_stream().cuda_stream or None
+        lib = load()
+        fn()
+        ts = []
+        for _ in range(reps):
+            if flush:
+                self.fl()
+            lib.muxi_event_record(self.ev[0], s)
+            fn()
+            lib.muxi_event_record(self.ev[1], s)
+            ms = C.c_float()
+            check(lib.muxi_event_elapsed_ms(self.ev[0], self.ev[1], C.byref(ms)), "elapsed")
+            ts.append(ms.value)
+        return statistics.median(ts)
+
+
+def errors(y: torch.Tensor, y64: torch.Tensor) -> dict:
+    d = y.double() - y64
+    return {"rel_l2": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(y64)),
+            "max_abs": float(d.abs().max()), "max_abs_y": float(y64.abs().max())}
+
+
+def precision(args) -> list[dict]:
+    tm = Timer()
+    out = []
+    mats = {}
+    h = G.cfg_powerlaw_2m()
+    mats["powerlaw_2m"] = (h.rowPtr, h.colIdx, h.values.astype(np.float64), h.numCols)
+    log("banded 4M: generating")
+    B = tune.gen_matrix("banded", 4_000_000, 4_000_000, 64, 16, 0, normal=True, dtype=torch.float64, seed=4)
+    rp, ci, va = B.host()
+    B.free()
+    t = time.perf_counter()
+    perm = prep.rcm(rp, ci)
+    nrp, nci, nva = prep.permute_symmetric(rp, ci, va, perm)
+    log(f"banded 4M: RCM + permutation {time.perf_counter() - t:.1f}s, bandwidth {prep.bandwidth(rp, ci)} -> "
+        f"{prep.bandwidth(nrp, nci)}")
+    mats["banded_4m_rcm"] = (nrp, nci, nva, 4_000_000)
+    for name, (rp, ci, va64, ncol) in mats.items():
+        x = torch.from_numpy(G.uniform_vector(ncol, 99)).cuda()
+        A64 = api.CSRMatrix(rp, ci, torch.from_numpy(va64), ncol)
+        y64 = api.oracle_f64(A64, x)  # FP64 values, FP32 x, exact-ish FP64 sum
+        for vt in (torch.float64, torch.float32, torch.bfloat16):
+            Av = A64.with_values(api.convert_values(A64.d_values, vt))
+            for kernel, param in (("merge", 4), ("xslab", 0)):
+                for comp in (False, True):
+                    if kernel == "xslab" and comp:
+                        continue
+                    p = api.Plan(Av, kernel, param, compensated=comp)
+                    y = torch.empty(Av.numRows, device="cuda")
+                    ms = tm.time(lambda: p.exec(x, y))
+                    bv = {torch.float64: 8, torch.float32: 4, torch.bfloat16: 2}[vt]
+                    b = algorithmic_bytes(Av.numRows, ncol, Av.nnz, bv, 4)
+                    rec = {"workload": name, "values": str(vt).split(".")[-1], "kernel": f"{kernel}:{param}",
+                           "accumulation": "fp32_dot2" if comp else "fp32", "ms": round(ms, 5),
+                           "GBps": round(b / ms / 1e6, 1), **errors(y, y64)}
+                    log(json.dumps(rec))
+                    out.append(rec)
+                    p.destroy()
+    return out
+
+
+def batched(args) -> list[dict]:
+    from .batch import BatchPlan
+
+    rng = np.random.default_rng(5)
+    mats, xs, ys = [], [], []
+    tot_b = 0
+    for i in range(256):
+        R = int(math.exp(rng.uniform(math.log(1e3), math.log(2e5))))
+        mean = math.exp(rng.uniform(math.log(4), math.log(64)))
+        A = tune.gen_matrix("poisson", R, R, mean, seed=5000 + i)
+        mats.append(A)
+        xs.append(tune.gen_vector(R, seed=7000 + i))
+        ys.append(torch.empty(R, device="cuda"))
+        tot_b += algorithmic_bytes(R, R, A.nnz, 4, 4)
+    nnz = sum(m.nnz for m in mats)
+    log(f"256 matrices, {sum(m.numRows for m in mats)} rows, {nnz} nnz, {tot_b / 1e6:.0f} MB")
+    bp = BatchPlan(mats, xs, ys, 8)
+    tm = Timer()
+    s = torch.cuda.Stream()
+    out = []
+    with torch.cuda.stream(s):
+        g_batch = bp.graph(stream=s)
+        g_vec = bp.graph("vector", 0, stream=s)
+        t_batch = tm.time(lambda: bp.exec())
+        t_batch_g = tm.time(lambda: g_batch.launch(s))
+        t_vec = tm.time(lambda: bp.exec_per_matrix("vector", 0))
+        t_vec_g = tm.time(lambda: g_vec.launch(s))
+        # correctness of the batched result
+        bp.exec()
+        worst = 0.0
+        for A, x, y in zip(mats, xs, ys):
+            y64 = api.oracle_f64(A, x)
+            worst = max(worst, errors(y, y64)["rel_l2"])
+    torch.cuda.synchronize()
+    for name, ms in [("batched_persistent", t_batch), ("batched_persistent_graph", t_batch_g),
+                     ("per_matrix_vector", t_vec), ("per_matrix_vector_graph", t_vec_g)]:
+        out.append({"workload": "batched_256", "path": name, "ms": round(ms, 4), "GBps": round(tot_b / ms / 1e6, 1),
+                    "nnz": nnz})
+    out.append({"workload": "batched_256", "speedup_graph_vs_per_matrix_graph": round(t_vec_g / t_batch_g, 2),
+                "speedup_eager": round(t_vec / t_batch, 2), "worst_rel_l2": worst, "info": bp.info()})
+    for r in out:
+        log(json.dumps(r))
+    return out
+
+
+def spmm(args) -> list[dict]:
+    h = G.cfg_powerlaw_2m()
+    A = api.CSRMatrix(h.rowPtr, h.colIdx, torch.from_numpy(h.values), h.numCols)
+    tm = Timer()
+    out = []
+    for k in (8, 32):
+        X = torch.rand((h.numCols, k), device="cuda") * 2 - 1
+        Y = torch.empty((h.numRows, k), device="cuda")
+        best = None
+        for ipt in (4, 8, 16):
+            p = api.Plan.spmm(A, k, ipt)
+            ms = tm.time(lambda: p.exec(X, Y))
+            b = algorithmic_bytes(h.numRows, h.numCols, h.nnz, 4, 4, k=k)
+            rec = {"workload": "spmm_powerlaw_2m", "k": k, "ipt": ipt, "ms": round(ms, 4),
+                   "GBps": round(b / ms / 1e6, 1), "GFLOPs": round(2 * h.nnz * k / ms / 1e6, 1)}
+            log(json.dumps(rec))
+            if best is None or ms < best["ms"]:
+                best = rec
+        p = api.Plan.spmm(A, k, best["ipt"])
+        p.exec(X, Y)
+        y64 = api.oracle_f64(A, X[:, 3].contiguous())
+        best["rel_l2_col3"] = errors(Y[:, 3], y64)["rel_l2"]
+        out.append(best)
+    return out
+
+
+def laplacian(args) -> list[dict]:
+    h = G.cfg_laplacian_4096()
+    A = api.CSRMatrix(h.rowPtr, h.colIdx, torch.from_numpy(h.values), h.numCols)
+    x = torch.from_numpy(G.uniform_vector(h.numCols, 3, np.float64)).cuda()
+    y = torch.empty(h.numRows, dtype=torch.float64, device="cuda")
+    tm = Timer()
+    b = algorithmic_bytes(h.numRows, h.numCols, h.nnz, 8, 8)
+    out = []
+    for k, p in [("ell", 0), ("adaptive", 512), ("adaptive", 1024), ("vwave", 2), ("vwave", 4), ("csr5", 512),
+                 ("merge", 4), ("xslab", 0)]:
+        pl = api.Plan(A, k, p, xtype=torch.float64)
+        ms = tm.time(lambda: pl.exec(x, y))
+        rec = {"workload": "laplacian_4096_fp64", "kernel": f"{k}:{p}", "ms": round(ms, 4),
+               "GBps": round(b / ms / 1e6, 1), "frac_hbm": round(b / ms / 1e6 / HBM_PEAK_GBS, 4)}
+        log(json.dumps(rec))
+        out.append(rec)
+        pl.destroy()
+    return out
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["precision", "batched", "spmm", "laplacian"])
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    torch.cuda.set_device(0)
+    res = {"precision": precision, "batched": batched, "spmm": spmm, "laplacian": laplacian}[a.what](a)
+    txt = "\n".join(json.dumps(r) for r in res)
+    print(txt, flush=True)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(txt + "\n")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
