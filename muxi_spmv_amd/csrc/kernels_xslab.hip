@@ -3,18 +3,23 @@
 //
 // MI355X has 8 XCDs, each with a private 4 MiB L2, and hands workgroup i to
 // XCD i mod 8. When x is larger than one L2 (the 2M-column power-law matrix
-// has an 8 MB x) and the columns are spread, every XCD streams all of x
-// through its L2 and the gathers, not the nonzero stream, set the time. The
-// plan splits the columns into S slabs (column c is in slab floor(c*S/C))
-// and stores, per slab, a compressed CSR of the nonzeros that fall in it:
-// the ids of the rows that have any, slab-local row pointers, columns and
-// values. Workgroup b works on slab b mod S, so with S = 8 an XCD gathers
-// from one 1/8 slice of x (1 MB here) that stays resident in its L2; the
-// matrix stream uses non-temporal loads so it does not evict that slice.
-// Each slab is a merge path over its compressed rows (the shared tile body's
-// algorithm), and row sums, including a tile's open row, are added to y with
-// hardware float atomics after y is scaled by beta. The atomics make the last
-// bits of y depend on the order in which slabs finish.
+// has an 8 MB x) and the columns are spread, every XCD pulls all of x through
+// its L2: on that matrix the plain merge kernel moves ~1.1 GB through L2 for
+// 139 MB of matrix and vectors (TCC_MISS x 128 B). The plan splits the
+// columns into S slabs (column c is in slab floor(c*S/C)) and stores, per
+// slab, a compressed CSR of the nonzeros that fall in it: slab-local row
+// pointers over the rows that have any, columns and values. Workgroup b works
+// on slab b mod S, so with S = 8 an XCD gathers from one 1/8 slice of x that
+// stays in its L2; the matrix stream uses non-temporal loads so it does not
+// evict that slice.
+//
+// A (row, slab) pair with nonzeros is a "pair"; pairs are numbered row-major,
+// so a row's pairs are contiguous. Each slab runs the merge-path algorithm
+// over its compressed rows and stores a row's partial sum to P[pair] (plain
+// stores, no atomics); the partial of the row open at a tile end goes to a
+// carry keyed by its pair, and a keyed fix-up adds carries in tile order. A
+// combine kernel then forms y[r] = alpha * (sum of the row's pairs, in slab
+// order) + beta * y[r]. Every sum has a fixed order: results are reproducible.
 #include <hipcub/hipcub.hpp>
 
 #include <climits>
@@ -37,10 +42,11 @@ __device__ __forceinline__ bf16_t ntl(const bf16_t* p) {
 
 template <typename VT, typename XT, typename AT, int IPT>
 __global__ __launch_bounds__(256) void xslab_kernel(SlabArgs sa, const int* __restrict__ srp,
-                                                    const int* __restrict__ rid, const int* __restrict__ sci,
+                                                    const int* __restrict__ pid, const int* __restrict__ sci,
                                                     const VT* __restrict__ svals, const XT* __restrict__ x,
-                                                    XT* __restrict__ y, const int* __restrict__ gdiag,
-                                                    const int* __restrict__ grow, AT alpha) {
+                                                    const int* __restrict__ gdiag, const int* __restrict__ grow,
+                                                    AT* __restrict__ P, AT* __restrict__ carry,
+                                                    int* __restrict__ carry_key) {
     constexpr int B = 256, CH = B * IPT;
     using A = Acc<AT, false>;
     __shared__ int s_end[CH];
@@ -51,7 +57,7 @@ __global__ __launch_bounds__(256) void xslab_kernel(SlabArgs sa, const int* __re
     if (t >= sa.tiles[s]) return;  // uniform over the workgroup
     const int tid = threadIdx.x;
     const int* rp = srp + sa.row_base[s] + s;
-    const int* ids = rid + sa.row_base[s];
+    const int* ids = pid + sa.row_base[s];
     const int* ci = sci + sa.nnz_base[s];
     const VT* vals = svals + sa.nnz_base[s];
     const int c = sa.coord_base[s] + t;
@@ -108,7 +114,7 @@ __global__ __launch_bounds__(256) void xslab_kernel(SlabArgs sa, const int* __re
         for (int k = 0; k < IPT; ++k)
             if (di + k < de) {
                 if (r < nr && s_end[r] <= n) {
-                    atomicAdd(y + ntl(ids + r0 + r), (XT)(alpha * a.s));
+                    P[ntl(ids + r0 + r)] = a.s;
                     a.zero();
                     ++r;
                 } else {
@@ -116,15 +122,72 @@ __global__ __launch_bounds__(256) void xslab_kernel(SlabArgs sa, const int* __re
                 }
             }
     }
-    // the row left open at the tile end, if this tile holds any of it
-    if (tid == 0 && r1 < sa.rows[s] && n1 > ntl(rp + r1)) atomicAdd(y + ntl(ids + r1), (XT)(alpha * total.s));
+    if (tid == 0) {
+        const int g = sa.tile_base[s] + t;
+        const bool open = r1 < sa.rows[s] && n1 > rp[r1];
+        carry[g] = open ? total.s : AT(0);
+        carry_key[g] = open ? ids[r1] : -1;
+    }
+}
+
+// Same run logic as carry_fixup_kernel with explicit keys (-1 = nothing):
+// P[key] += sum of the run's carries, in tile order.
+template <typename AT>
+__global__ __launch_bounds__(256) void keyed_fixup_kernel(const int* __restrict__ keys, int count,
+                                                          const AT* __restrict__ carry, AT* __restrict__ P) {
+    const int lane = threadIdx.x & 63;
+    const long base = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64 * 64;
+    if (base >= count) return;
+    auto key_of = [&](long t) -> int { return t < count ? keys[t] : -2; };
+    const long t = base + lane;
+    const int key = key_of(t);
+    AT v = t < count ? carry[t] : AT(0);
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        AT vu = __shfl_up(v, d);
+        int ku = __shfl_up(key, d);
+        if (lane >= d && ku == key) v = vu + v;
+    }
+    int knext = __shfl_down(key, 1);
+    if (lane == 63) knext = key_of(base + 64);
+    const int key0 = __shfl(key, 0);
+    const bool owned = !((base > 0) && (key_of(base - 1) == key0) && key == key0);
+    if (knext != key && owned && key >= 0) P[key] += v;
+    const int k63 = __shfl(key, 63), kn63 = __shfl(knext, 63);
+    const bool own63 = __shfl((int)owned, 63);
+    if (kn63 == k63 && own63 && k63 >= 0) {
+        AT sum = __shfl(v, 63);
+        for (long nb = base + 64; nb < count; nb += 64) {
+            const long tt = nb + lane;
+            const int kk = key_of(tt);
+            AT c = (kk == k63) ? carry[tt] : AT(0);
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m);
+            sum += c;
+            if (__shfl(kk, 63) != k63) break;
+        }
+        if (lane == 0) P[k63] += sum;
+    }
+}
+
+// y[r] = alpha * (sum of the row's pair partials, in slab order) + beta * y[r]
+template <typename XT, typename AT>
+__global__ __launch_bounds__(256) void xslab_combine_kernel(int R, const int* __restrict__ rpp,
+                                                            const AT* __restrict__ P, XT* __restrict__ y, AT alpha,
+                                                            AT beta) {
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    AT s = AT(0);
+    const int b = rpp[r], e = rpp[r + 1];
+    for (int i = b; i < e; ++i) s += ntl(P + i);
+    store_y(y, r, s, alpha, beta);
 }
 
 // ---- plan construction
 __device__ __forceinline__ int slab_of(int col, int S, int C) { return (int)(((long)col * S) / C); }
 
 __global__ void xslab_count_kernel(const int* __restrict__ rp, const int* __restrict__ ci, int R, int C, int S,
-                                   int* __restrict__ cnt, int* __restrict__ flag) {
+                                   int* __restrict__ cnt, int* __restrict__ flag, int* __restrict__ npair) {
     const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     int c[kMaxSlabs];
@@ -135,32 +198,35 @@ __global__ void xslab_count_kernel(const int* __restrict__ rp, const int* __rest
 #pragma unroll
         for (int q = 0; q < kMaxSlabs; ++q) c[q] += (q == s);
     }
+    int np = 0;
 #pragma unroll
     for (int s = 0; s < kMaxSlabs; ++s)
         if (s < S) {
             cnt[(long)s * R + r] = c[s];
             flag[(long)s * R + r] = c[s] > 0;
+            np += c[s] > 0;
         }
+    npair[r] = np;
 }
 
 template <typename VT>
 __global__ void xslab_fill_kernel(const int* __restrict__ rp, const int* __restrict__ ci, const VT* __restrict__ v,
                                   int R, int C, int S, const int* __restrict__ cnt, const int* __restrict__ pos,
-                                  const int* __restrict__ lrow,
-                                  const int* __restrict__ nnz_base, const int* __restrict__ row_base,
-                                  int* __restrict__ srp, int* __restrict__ rid, int* __restrict__ sci,
-                                  VT* __restrict__ sv) {
+                                  const int* __restrict__ lrow, const int* __restrict__ nnz_base,
+                                  const int* __restrict__ rpp, int* __restrict__ srp, int* __restrict__ pid,
+                                  int* __restrict__ sci, VT* __restrict__ sv) {
     const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     int off[kMaxSlabs];
 #pragma unroll
     for (int s = 0; s < kMaxSlabs; ++s) off[s] = 0;
+    int rank = 0;
     for (int s = 0; s < S; ++s) {
         const long k = (long)s * R + r;
         const int p = pos[k];
         if (cnt[k] > 0) {  // row r has nonzeros in slab s
             const int lr = lrow[k];
-            rid[lr] = (int)r;
+            pid[lr] = rpp[r] + rank++;
             srp[lr + s] = p - nnz_base[s];
         }
 #pragma unroll
@@ -204,27 +270,31 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int ipt, spmv_type_t vt, hipStrea
         return SPMV_ERROR_INVALID_MATRIX;
     }
     const long SR = (long)S * R;
-    int *cnt, *flag, *pos, *lrow;
+    int *cnt, *flag, *pos, *lrow, *npair;
     MUXI_HIP(hipMallocAsync(&cnt, sizeof(int) * (SR + 1), s));
     MUXI_HIP(hipMallocAsync(&flag, sizeof(int) * (SR + 1), s));
     MUXI_HIP(hipMallocAsync(&pos, sizeof(int) * (SR + 1), s));
     MUXI_HIP(hipMallocAsync(&lrow, sizeof(int) * (SR + 1), s));
+    MUXI_HIP(hipMallocAsync(&npair, sizeof(int) * ((size_t)R + 1), s));
     MUXI_HIP(hipMemsetAsync(cnt + SR, 0, sizeof(int), s));
     MUXI_HIP(hipMemsetAsync(flag + SR, 0, sizeof(int), s));
+    MUXI_HIP(hipMemsetAsync(npair + R, 0, sizeof(int), s));
     hipLaunchKernelGGL(xslab_count_kernel, dim3(grid_for(R, 256)), dim3(256), 0, s, p->A.rp, p->A.ci, R, C, S, cnt,
-                       flag);
+                       flag, npair);
     MUXI_LAUNCH_CHECK();
+    // row-major pair offsets rpp[R+1] live in the plan
+    MUXI_HIP(hipMalloc(&p->d_slab_rpp, sizeof(int) * ((size_t)R + 1)));
     spmv_status_t st = exscan(cnt, pos, SR + 1, s);
     if (st == SPMV_SUCCESS) st = exscan(flag, lrow, SR + 1, s);
+    if (st == SPMV_SUCCESS) st = exscan(npair, p->d_slab_rpp, (long)R + 1, s);
     if (st != SPMV_SUCCESS) return st;
-    // per-slab bases: pos/lrow at s*R, s = 0..S
     std::vector<int> hpos(S + 1), hrow(S + 1);
     for (int q = 0; q <= S; ++q) {
         MUXI_HIP(hipMemcpyAsync(&hpos[q], pos + (long)q * R, sizeof(int), hipMemcpyDeviceToHost, s));
         MUXI_HIP(hipMemcpyAsync(&hrow[q], lrow + (long)q * R, sizeof(int), hipMemcpyDeviceToHost, s));
     }
     MUXI_HIP(hipStreamSynchronize(s));
-    const int total_rows = hrow[S];
+    const int total_rows = hrow[S];  // number of pairs
     SlabArgs& sa = p->slab;
     sa.S = S;
     for (int q = 0; q < S; ++q) {
@@ -232,12 +302,10 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int ipt, spmv_type_t vt, hipStrea
         sa.row_base[q] = hrow[q];
         sa.nnz_base[q] = hpos[q];
     }
-    int* d_bases = nullptr;
-    MUXI_HIP(hipMallocAsync(&d_bases, sizeof(int) * 2 * S, s));
-    MUXI_HIP(hipMemcpyAsync(d_bases, hpos.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
-    MUXI_HIP(hipMemcpyAsync(d_bases + S, hrow.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
+    int* d_nb = nullptr;
+    MUXI_HIP(hipMallocAsync(&d_nb, sizeof(int) * S, s));
+    MUXI_HIP(hipMemcpyAsync(d_nb, hpos.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
     const size_t vs = vsize(vt);
-    // one allocation: rowPtr (total_rows + S), row ids (total_rows), columns (N), then values
     const size_t ints = (size_t)total_rows * 2 + S + (size_t)N;
     MUXI_HIP(hipMalloc(&p->d_slab, sizeof(int) * ints + vs * (size_t)(N > 0 ? N : 1) + 256));
     p->d_slab_rp = p->d_slab;
@@ -246,34 +314,37 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int ipt, spmv_type_t vt, hipStrea
     uintptr_t va = (uintptr_t)(p->d_slab_ci + N);
     va = (va + 255) & ~(uintptr_t)255;
     p->d_slab_v = (void*)va;
-    // slab rowPtr terminators
-    for (int q = 0; q < S; ++q) {
-        const int nq = hpos[q + 1] - hpos[q];
-        MUXI_HIP(hipMemcpyAsync(p->d_slab_rp + sa.row_base[q] + q + sa.rows[q], &nq, sizeof(int),
+    std::vector<int> term(S);
+    for (int q = 0; q < S; ++q) term[q] = hpos[q + 1] - hpos[q];
+    for (int q = 0; q < S; ++q)
+        MUXI_HIP(hipMemcpyAsync(p->d_slab_rp + sa.row_base[q] + q + sa.rows[q], &term[q], sizeof(int),
                                 hipMemcpyHostToDevice, s));
-        MUXI_HIP(hipStreamSynchronize(s));  // nq is a stack variable
-    }
     st = with_types(vt, p->xtype, p->comp, [&](auto tag) -> spmv_status_t {
         using T = decltype(tag);
         using VT = typename T::VT;
         hipLaunchKernelGGL(xslab_fill_kernel<VT>, dim3(grid_for(R, 256)), dim3(256), 0, s, p->A.rp, p->A.ci,
-                           (const VT*)p->A.v, R, C, S, cnt, pos, lrow, d_bases, d_bases + S, p->d_slab_rp,
+                           (const VT*)p->A.v, R, C, S, cnt, pos, lrow, d_nb, p->d_slab_rpp, p->d_slab_rp,
                            p->d_slab_rid, p->d_slab_ci, (VT*)p->d_slab_v);
         MUXI_LAUNCH_CHECK();
         return SPMV_SUCCESS;
     });
     if (st != SPMV_SUCCESS) return st;
-    // merge partitions per slab
     const long CH = 256L * ipt;
-    long ncoord = 0;
+    long ncoord = 0, ntiles = 0;
     for (int q = 0; q < S; ++q) {
         const int nq = hpos[q + 1] - hpos[q];
         sa.tiles[q] = sa.rows[q] > 0 ? (int)(((long)sa.rows[q] + nq + CH - 1) / CH) : 0;
         sa.coord_base[q] = (int)ncoord;
+        sa.tile_base[q] = (int)ntiles;
         ncoord += sa.tiles[q] + 1;
+        ntiles += sa.tiles[q];
     }
+    p->T = (int)ntiles;
+    const size_t acc = p->xtype == SPMV_TYPE_DOUBLE ? 8 : 4;
     MUXI_HIP(hipMalloc(&p->d_pos, 2 * sizeof(int) * (size_t)ncoord));
     p->d_row = p->d_pos + ncoord;
+    MUXI_HIP(hipMalloc(&p->d_carry, acc * (size_t)(ntiles + total_rows + 1)));  // carries, then P
+    MUXI_HIP(hipMalloc(&p->d_slab_ckey, sizeof(int) * (size_t)(ntiles + 1)));
     for (int q = 0; q < S; ++q) {
         if (sa.tiles[q] == 0) continue;
         const int nq = hpos[q + 1] - hpos[q];
@@ -290,34 +361,49 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int ipt, spmv_type_t vt, hipStrea
     (void)hipFreeAsync(flag, s);
     (void)hipFreeAsync(pos, s);
     (void)hipFreeAsync(lrow, s);
-    (void)hipFreeAsync(d_bases, s);
+    (void)hipFreeAsync(npair, s);
+    (void)hipFreeAsync(d_nb, s);
     MUXI_HIP(hipStreamSynchronize(s));
-    p->scratch_bytes = sizeof(int) * ints + vs * (size_t)N + 2 * sizeof(int) * (size_t)ncoord;
+    p->scratch_bytes = sizeof(int) * ints + vs * (size_t)N + 2 * sizeof(int) * (size_t)ncoord +
+                       acc * (size_t)(ntiles + total_rows) + sizeof(int) * ((size_t)ntiles + R + 1);
     return SPMV_SUCCESS;
 }
 
 template <typename VT, typename XT, typename AT>
-spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT beta, hipStream_t s, hipEvent_t ev0) {
+spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT beta, hipStream_t s,
+                           hipEvent_t ev_end) {
     if (p->A.R <= 0) return SPMV_SUCCESS;
-    spmv_status_t st = launch_scale_y<XT>(y, p->A.R, (XT)beta, s);
-    if (st != SPMV_SUCCESS) return st;
-    if (ev0) MUXI_HIP(hipEventRecord(ev0, s));
     const SlabArgs& sa = p->slab;
-    if (p->P == 0) return SPMV_SUCCESS;
-    const dim3 grid((unsigned)((long)p->P * sa.S));
-    switch (p->param0 & 0xff) {
-        case 4:
-            hipLaunchKernelGGL((xslab_kernel<VT, XT, AT, 4>), grid, dim3(256), 0, s, sa, p->d_slab_rp, p->d_slab_rid,
-                               p->d_slab_ci, (const VT*)p->d_slab_v, x, y, p->d_pos, p->d_row, alpha);
-            break;
-        case 16:
-            hipLaunchKernelGGL((xslab_kernel<VT, XT, AT, 16>), grid, dim3(256), 0, s, sa, p->d_slab_rp, p->d_slab_rid,
-                               p->d_slab_ci, (const VT*)p->d_slab_v, x, y, p->d_pos, p->d_row, alpha);
-            break;
-        default:
-            hipLaunchKernelGGL((xslab_kernel<VT, XT, AT, 8>), grid, dim3(256), 0, s, sa, p->d_slab_rp, p->d_slab_rid,
-                               p->d_slab_ci, (const VT*)p->d_slab_v, x, y, p->d_pos, p->d_row, alpha);
+    AT* carry = (AT*)p->d_carry;
+    AT* P = carry + p->T + 1;
+    if (p->P > 0) {
+        const dim3 grid((unsigned)((long)p->P * sa.S));
+        switch (p->param0 & 0xff) {
+            case 4:
+                hipLaunchKernelGGL((xslab_kernel<VT, XT, AT, 4>), grid, dim3(256), 0, s, sa, p->d_slab_rp,
+                                   p->d_slab_rid, p->d_slab_ci, (const VT*)p->d_slab_v, x, p->d_pos, p->d_row, P,
+                                   carry, p->d_slab_ckey);
+                break;
+            case 16:
+                hipLaunchKernelGGL((xslab_kernel<VT, XT, AT, 16>), grid, dim3(256), 0, s, sa, p->d_slab_rp,
+                                   p->d_slab_rid, p->d_slab_ci, (const VT*)p->d_slab_v, x, p->d_pos, p->d_row, P,
+                                   carry, p->d_slab_ckey);
+                break;
+            default:
+                hipLaunchKernelGGL((xslab_kernel<VT, XT, AT, 8>), grid, dim3(256), 0, s, sa, p->d_slab_rp,
+                                   p->d_slab_rid, p->d_slab_ci, (const VT*)p->d_slab_v, x, p->d_pos, p->d_row, P,
+                                   carry, p->d_slab_ckey);
+        }
+        MUXI_LAUNCH_CHECK();
+        if (ev_end) MUXI_HIP(hipEventRecord(ev_end, s));
+        hipLaunchKernelGGL(keyed_fixup_kernel<AT>, dim3(grid_for(p->T, 256)), dim3(256), 0, s, p->d_slab_ckey, p->T,
+                           carry, P);
+        MUXI_LAUNCH_CHECK();
+    } else if (ev_end) {
+        MUXI_HIP(hipEventRecord(ev_end, s));
     }
+    hipLaunchKernelGGL((xslab_combine_kernel<XT, AT>), dim3(grid_for(p->A.R, 256)), dim3(256), 0, s, p->A.R,
+                       p->d_slab_rpp, P, y, alpha, beta);
     MUXI_LAUNCH_CHECK();
     return SPMV_SUCCESS;
 }

@@ -20,6 +20,8 @@ static void plan_free(muxi_plan* p) {
     (void)hipFree(p->d_eval);
     (void)hipFree(p->d_blocks);
     (void)hipFree(p->d_slab);
+    (void)hipFree(p->d_slab_rpp);
+    (void)hipFree(p->d_slab_ckey);
     delete p;
 }
 
@@ -179,7 +181,7 @@ extern "C" spmv_status_t muxi_plan_create(const muxi_csr_t* Ain, spmv_type_t vt,
                 return SPMV_ERROR_INTERNAL;
             }
             if (p->comp) {
-                set_error("xslab: compensated accumulation is not available (sums meet in atomics)");
+                set_error("xslab: compensated accumulation is not available");
                 plan_free(p);
                 return SPMV_ERROR_UNSUPPORTED_TYPE;
             }
@@ -367,7 +369,7 @@ static spmv_status_t plan_exec(const muxi_plan* p, const void* x, void* y, doubl
                     set_error("xslab: no compensated variant");
                     return SPMV_ERROR_UNSUPPORTED_TYPE;
                 } else {
-                    return launch_xslab<VT, XT, AT>(p, xx, yy, a, b, s, ev0);
+                    return launch_xslab<VT, XT, AT>(p, xx, yy, a, b, s, ev1);
                 }
             case MUXI_KERNEL_ADAPTIVE:
                 return launch_adaptive<VT, XT, AT, C>(p->A, p->param0, p->nblocks, p->d_blocks, p->d_blk_chunk,
@@ -379,7 +381,8 @@ static spmv_status_t plan_exec(const muxi_plan* p, const void* x, void* y, doubl
     if (st != SPMV_SUCCESS) return st;
     // single-kernel paths: the main kernel ends with the exec
     if (ev1 && !((p->kernel == MUXI_KERNEL_MERGE && p->A.R > 0) || (p->kernel == MUXI_KERNEL_CSR5 && p->T > 0) ||
-                 (p->kernel == MUXI_KERNEL_ADAPTIVE && p->A.R > 0 && p->nblocks > 0)))
+                 (p->kernel == MUXI_KERNEL_ADAPTIVE && p->A.R > 0 && p->nblocks > 0) ||
+                 (p->kernel == MUXI_KERNEL_XSLAB && p->A.R > 0)))
         MUXI_HIP(hipEventRecord(ev1, s));
     return SPMV_SUCCESS;
 }

@@ -13,6 +13,7 @@ struct SlabArgs {
     int row_base[kMaxSlabs];    // first row id of the slab; its rowPtr starts at row_base + s
     int nnz_base[kMaxSlabs];    // first nonzero of the slab
     int coord_base[kMaxSlabs];  // first of the slab's P+1 merge coordinates
+    int tile_base[kMaxSlabs];   // first global tile (carry slot) of the slab
 };
 }  // namespace muxi
 
@@ -54,6 +55,8 @@ struct muxi_plan {
     int* d_slab_rid = nullptr;
     int* d_slab_ci = nullptr;
     void* d_slab_v = nullptr;
+    int* d_slab_rpp = nullptr;   // row-major pair offsets [R+1]
+    int* d_slab_ckey = nullptr;  // carry keys (pair ids) per tile
 };
 
 namespace muxi {
@@ -95,6 +98,7 @@ inline size_t vsize(spmv_type_t t) { return t == SPMV_TYPE_DOUBLE ? 8 : (t == SP
 // column-slab plan and launcher (kernels_xslab.hip); param0 = (slabs << 8) | items per lane
 spmv_status_t xslab_build(muxi_plan* p, int S, int ipt, spmv_type_t vt, hipStream_t s);
 template <typename VT, typename XT, typename AT>
-spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT beta, hipStream_t s, hipEvent_t ev0);
+spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT beta, hipStream_t s,
+                           hipEvent_t ev_end);
 
 }  // namespace muxi
