@@ -7,6 +7,7 @@
 #include <new>
 #include <vector>
 
+#include "muxi_spmv/muxi_spmv_tune.h"
 #include "plan.hpp"
 
 using namespace muxi;
@@ -85,8 +86,14 @@ extern "C" spmv_status_t muxi_plan_create(const muxi_csr_t* Ain, spmv_type_t vt,
     const int R = Ain->numRows, N = Ain->nnz;
 
     if (kernel == MUXI_KERNEL_AUTO) {
-        kernel = MUXI_KERNEL_MERGE;
-        param0 = 8;
+        int k = MUXI_KERNEL_MERGE, q = 4;
+        PLAN_TRY(muxi_selector_pick(Ain, xt, stream, &k, &q));
+        if (p->comp && k == MUXI_KERNEL_XSLAB) {  // no compensated column-slab variant
+            k = MUXI_KERNEL_MERGE;
+            q = 4;
+        }
+        kernel = (muxi_kernel_t)k;
+        param0 = q;
     }
     p->kernel = kernel;
     p->param0 = param0;
