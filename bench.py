@@ -27,7 +27,7 @@ sys.path.insert(0, str(ROOT))
 
 CANDIDATES = [("merge", 4), ("merge", 8), ("merge", 16), ("csr5", 256), ("csr5", 512), ("csr5", 1024),
               ("adaptive", 512), ("adaptive", 1024), ("adaptive", 2048), ("ell", 0),
-              ("xslab", (8 << 8) | 4), ("xslab", (8 << 8) | 8), ("xslab", (16 << 8) | 4), ("xslab", (4 << 8) | 4),
+              ("xslab", (2 << 8) | 4), ("xslab", (4 << 8) | 4), ("xslab", (8 << 8) | 4), ("xslab", (4 << 8) | 8),
               ("vwave", 4), ("vwave", 8), ("vector", 0), ("scalar", 0)]
 
 

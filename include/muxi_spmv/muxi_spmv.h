@@ -225,7 +225,8 @@ typedef enum {
     MUXI_KERNEL_CSR5 = 6,      /* nnz tiles of param0 = sigma, one wavefront per tile */
     MUXI_KERNEL_ELL = 7,       /* ELLPACK built at plan time */
     MUXI_KERNEL_XSLAB = 8,     /* column slabs, slab = workgroup mod slabs (one x slice per XCD L2);
-                                  param0 = (slabs << 8) | items per lane, 0 = 8 slabs x 4 */
+                                  param0 = (slabs << 8) | items per lane; slabs 0 = the fewest of
+                                  1, 2, 4, 8 that keep a slab of x within 2 MiB; items 0 = 4 */
     MUXI_KERNEL_COUNT = 9
 } muxi_kernel_t;
 

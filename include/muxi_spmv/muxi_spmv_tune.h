@@ -82,7 +82,9 @@ spmv_status_t muxi_mlp_adam(float* d_params, const float* d_grad, float* d_m, fl
                             float beta1, float beta2, float eps, int t, void* stream);
 
 /* ---- learned selection for MUXI_KERNEL_AUTO --------------------------------
- * Installs a trained selector (process-wide, replaces any previous one):
+ * A library built with trained weights (the package's selector_weights.json)
+ * starts with that selector installed ("built-in"). muxi_selector_set
+ * installs a trained selector (process-wide, replaces any previous one):
  * params as above with in = 10, mu/sd the input normalisation, and the
  * candidate each output scores (kernels[n_out], params0[n_out]). The model
  * input is log1p(rows), log1p(nnz), log1p(mean), log1p(sqrt(var)),
@@ -90,11 +92,17 @@ spmv_status_t muxi_mlp_adam(float* d_params, const float* d_grad, float* d_m, fl
  * log1p(max/mean), log1p(nnz) - log1p(rows). */
 spmv_status_t muxi_selector_set(const float* params, int n_in, int hidden, int n_out, const float* mu,
                                 const float* sd, const int* kernels, const int* params0);
+/* Removes any selector: AUTO falls back to the fixed rule below. */
 spmv_status_t muxi_selector_clear(void);
+/* Returns to the built-in selector (or to the fixed rule if there is none). */
+spmv_status_t muxi_selector_reset(void);
+/* 0 = fixed rule, 1 = built-in selector, 2 = a selector set by the caller. */
+int muxi_selector_state(void);
 /* The (kernel, param0) an AUTO plan would use for A: the selector's argmin
  * over the candidates the sweep admits for A, or, with no selector
- * installed, merge-path (4 items per lane) or column slabs (8 x 4) when the
- * x vector exceeds 4 MiB and rows average under 64 nonzeros. */
+ * installed, merge-path (4 items per lane) or column slabs (param0 = 4: the
+ * default slab count, 4 items per lane) when the x vector exceeds 4 MiB and
+ * rows average under 64 nonzeros. */
 spmv_status_t muxi_selector_pick(const muxi_csr_t* A, spmv_type_t xtype, void* stream, int* kernel, int* param0);
 
 #ifdef __cplusplus

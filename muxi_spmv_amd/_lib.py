@@ -151,6 +151,8 @@ SIGNATURES: dict[str, tuple] = {
     "muxi_mlp_adam": (_I, [_VP, _VP, _VP, _VP, C.c_long, _F, _F, _F, _F, _I, _VP]),
     "muxi_selector_set": (_I, [_VP, _I, _I, _I, _VP, _VP, _VP, _VP]),
     "muxi_selector_clear": (_I, []),
+    "muxi_selector_reset": (_I, []),
+    "muxi_selector_state": (_I, []),
     "muxi_selector_pick": (_I, [_P_CSR, _I, _VP, C.POINTER(_I), C.POINTER(_I)]),
 }
 

@@ -3,12 +3,16 @@
 Usage: python -m muxi_spmv_amd.build [--jobs N]
 Every source under csrc/ is compiled as HIP for gfx950 and linked into one
 shared library next to this file. Packed-FP32 instructions are disabled
-(``-Xclang -target-feature -Xclang -packed-fp32-ops``).
+(``-Xclang -target-feature -Xclang -packed-fp32-ops``). When the package holds
+trained selector weights (selector_weights.json), they are written to a
+generated header that selector.cpp compiles in, so MUXI_KERNEL_AUTO plans use
+the learned selector without any set-up call.
 """
 from __future__ import annotations
 
 import argparse
 import concurrent.futures as cf
+import json
 import os
 import shutil
 import subprocess
@@ -36,7 +40,41 @@ FLAGS = [
     "-Wno-unused-variable",
     f"-I{ROOT / 'include'}",
     f"-I{CSRC}",
+    f"-I{BUILD}",
 ]
+
+WEIGHTS = PKG / "selector_weights.json"
+_KERNEL_IDS = {"scalar": 1, "vector": 2, "vwave": 3, "adaptive": 4, "merge": 5, "csr5": 6, "ell": 7, "xslab": 8}
+
+
+def write_selector_header() -> Path | None:
+    """Writes _build/selector_weights.inc from selector_weights.json (or removes a stale one)."""
+    inc = BUILD / "selector_weights.inc"
+    if not WEIGHTS.exists():
+        inc.unlink(missing_ok=True)
+        return None
+    st = json.loads(WEIGHTS.read_text())
+    if st["n_in"] != 10 or len(st["candidates"]) != st["n_out"]:
+        raise RuntimeError(f"{WEIGHTS}: expected 10 inputs and one output per candidate")
+    ks, ps = [], []
+    for name in st["candidates"]:
+        k, p = name.split(":")
+        ks.append(_KERNEL_IDS[k])
+        ps.append(int(p))
+
+    def arr(ctype, name, vals, fmt):
+        body = ",".join(fmt(v) for v in vals)
+        return f"static const {ctype} {name}[] = {{{body}}};\n"
+
+    f32 = lambda v: f"{float(v)!r}f"  # noqa: E731  (repr round-trips the float32 value)
+    txt = ("// Generated by muxi_spmv_amd/build.py from selector_weights.json; do not edit.\n"
+           f"static const int kSelIn = {st['n_in']}, kSelHidden = {st['hidden']}, kSelOut = {st['n_out']};\n"
+           + arr("float", "kSelParams", st["params"], f32) + arr("float", "kSelMu", st["mu"], f32)
+           + arr("float", "kSelSd", st["sd"], f32) + arr("int", "kSelKernel", ks, str)
+           + arr("int", "kSelParam0", ps, str))
+    if not inc.exists() or inc.read_text() != txt:
+        inc.write_text(txt)
+    return inc
 
 
 def sources() -> list[Path]:
@@ -54,6 +92,7 @@ def _compile(src: Path) -> Path:
 
 def build(jobs: int | None = None, verbose: bool = True) -> Path:
     BUILD.mkdir(exist_ok=True)
+    write_selector_header()
     srcs = sources()
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -229,8 +229,8 @@ static spmv_status_t spmv_ell_impl(const muxi_ell_t* E, const T* x, T* y, void* 
         set_error("x or y is NULL");
         return SPMV_ERROR_INVALID_VECTOR;
     }
-    return launch_ell<T, T, T, false>(E->numRows, E->numElements, E->d_colIdx, (const T*)E->d_values, x, y, T(1),
-                                      T(0), (hipStream_t)stream);
+    return launch_ell<T, T, T, false>(E->numRows, E->numCols, E->numElements, E->d_colIdx, (const T*)E->d_values, x,
+                                      y, T(1), T(0), (hipStream_t)stream);
 }
 
 template <typename T>

@@ -140,6 +140,18 @@ __device__ __forceinline__ A group_reduce(A a) {
     return a;
 }
 
+// Non-temporal load for matrix arrays that one SpMV reads once: they pass
+// through L2 without displacing the x lines that the gathers reuse.
+template <typename T>
+__device__ __forceinline__ T ntl(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+__device__ __forceinline__ bf16_t ntl(const bf16_t* p) {
+    bf16_t b;
+    b.bits = __builtin_nontemporal_load(&p->bits);
+    return b;
+}
+
 // y[i] = alpha*v + beta*y[i]; beta == 0 never reads y (so y may hold NaN).
 template <typename YT, typename AT>
 __device__ __forceinline__ void store_y(YT* y, long i, AT v, AT alpha, AT beta) {

@@ -61,7 +61,7 @@ spmv_status_t launch_max_row_len(const int* rp, int R, int* d_out, hipStream_t s
 template <typename VT>
 spmv_status_t launch_csr_to_ell(const DevCsr& A, int width, int* d_col, VT* d_val, hipStream_t s);
 template <typename VT, typename XT, typename AT, bool COMP>
-spmv_status_t launch_ell(int R, int width, const int* d_col, const VT* d_val, const XT* x, XT* y, AT alpha,
+spmv_status_t launch_ell(int R, int C, int width, const int* d_col, const VT* d_val, const XT* x, XT* y, AT alpha,
                          AT beta, hipStream_t s);
 
 // ---- SpMM over merge-path partitions (kernels_spmm.hip); carry holds P*k elements.

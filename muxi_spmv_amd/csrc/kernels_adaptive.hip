@@ -52,10 +52,22 @@ __global__ __launch_bounds__(256) void adaptive_kernel(const int* __restrict__ r
     if (chunk < 0) {
         const int nrows = r1 - r0;
         const int n0 = rp[r0], nn = rp[r1] - n0;
+        constexpr int K = NB / 256;
+        // clamped (unconditional) loads: all K columns and values, then all K
+        // gathers of a lane are in flight together; nn > 0 is block-uniform
+        if (nn > 0) {
+            int c[K];
+            VT v[K];
+            XT xv[K];
 #pragma unroll
-        for (int k = 0; k < NB / 256; ++k) {
-            const int j = tid + k * 256;
-            if (j < nn) s_prod[j] = P::make(vals[n0 + j], x[ci[n0 + j]]);
+            for (int k = 0; k < K; ++k) c[k] = ntl(ci + n0 + min(tid + k * 256, nn - 1));
+#pragma unroll
+            for (int k = 0; k < K; ++k) v[k] = ntl(vals + n0 + min(tid + k * 256, nn - 1));
+#pragma unroll
+            for (int k = 0; k < K; ++k) xv[k] = x[c[k]];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (tid + k * 256 < nn) s_prod[tid + k * 256] = P::make(v[k], xv[k]);
         }
         for (int i = tid; i <= nrows; i += 256) s_rp[i] = rp[r0 + i];
         __syncthreads();
@@ -80,11 +92,21 @@ __global__ __launch_bounds__(256) void adaptive_kernel(const int* __restrict__ r
         const long e1 = e0 + NB < rp[r0 + 1] ? e0 + NB : rp[r0 + 1];
         A a;
         a.zero();
+        constexpr int K = NB / 256;
+        const long last = e1 - 1;  // a piece holds at least one nonzero
+        auto at = [&](int k) { const long i = e0 + tid + k * 256; return i < last ? i : last; };
+        int c[K];
+        VT v[K];
+        XT xv[K];
 #pragma unroll
-        for (int k = 0; k < NB / 256; ++k) {
-            const long e = e0 + tid + k * 256;
-            if (e < e1) a.fma(vals[e], x[ci[e]]);
-        }
+        for (int k = 0; k < K; ++k) c[k] = ntl(ci + at(k));
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = ntl(vals + at(k));
+#pragma unroll
+        for (int k = 0; k < K; ++k) xv[k] = x[c[k]];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (e0 + tid + k * 256 < e1) a.fma(v[k], xv[k]);
         a = group_reduce<64>(a);
         if ((tid & 63) == 0) s_w[tid >> 6] = a;
         __syncthreads();

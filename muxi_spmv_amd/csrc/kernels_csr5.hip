@@ -87,25 +87,46 @@ __global__ __launch_bounds__(256) void csr5_spmv_kernel(const int* __restrict__ 
     const int cnt = (rb < R ? rb : R) - ra + 1;
     const bool staged = cnt <= CAP;
     int* srp = s_rp[w];
+    auto RP = [&](int r) -> int { return staged ? srp[r - ra] : rp[r]; };
+
+    // Issue the lane's column and value loads first, then stage the row
+    // pointers, then the x gathers, so the three latencies overlap. Lanes past
+    // the matrix end load a clamped (valid) element and drop its product;
+    // no load sits behind a per-element branch.
+    const long e0 = ts + (long)lane * EPL;
+    const int nv = e0 < te ? (int)((te - e0) < EPL ? (te - e0) : EPL) : 0;
+    const long last = te - 1;
+    int cv[EPL];
+    VT vv[EPL];
+    if (vec && nv == EPL) {
+        const VecN<int, EPL> c4 = *(const VecN<int, EPL>*)(ci + e0);
+        const VecN<VT, EPL> v4 = *(const VecN<VT, EPL>*)(vals + e0);
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) {
+            cv[j] = c4.v[j];
+            vv[j] = v4.v[j];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) {
+            const long i = e0 + j < last ? e0 + j : last;
+            cv[j] = ci[i];
+            vv[j] = vals[i];
+        }
+    }
     if (staged) {
         for (int i = lane; i < cnt; i += 64) srp[i] = rp[ra + i];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    auto RP = [&](int r) -> int { return staged ? srp[r - ra] : rp[r]; };
-
-    const long e0 = ts + (long)lane * EPL;
-    const int nv = e0 < te ? (int)((te - e0) < EPL ? (te - e0) : EPL) : 0;
     P pr[EPL];
-    if (vec && nv == EPL) {
-        VecN<int, EPL> cv = *(const VecN<int, EPL>*)(ci + e0);
-        VecN<VT, EPL> vv = *(const VecN<VT, EPL>*)(vals + e0);
+    {
+        XT xv[EPL];
 #pragma unroll
-        for (int j = 0; j < EPL; ++j) pr[j] = P::make(vv.v[j], x[cv.v[j]]);
-    } else {
+        for (int j = 0; j < EPL; ++j) xv[j] = x[cv[j]];
 #pragma unroll
-        for (int j = 0; j < EPL; ++j) pr[j] = (j < nv) ? P::make(vals[e0 + j], x[ci[e0 + j]]) : P::zero();
+        for (int j = 0; j < EPL; ++j) pr[j] = (j < nv) ? P::make(vv[j], xv[j]) : P::zero();
     }
     // lane's first row: last r in [ra, min(rb, R-1)] with rowPtr[r] <= e0
     int rf = ra;

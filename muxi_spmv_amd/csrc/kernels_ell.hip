@@ -66,40 +66,58 @@ spmv_status_t launch_csr_to_ell(const DevCsr& A, int width, int* d_col, VT* d_va
 // One lane per row; each slot index i is one coalesced 64-lane load of
 // columns and values. Padding (col < 0) contributes nothing.
 template <typename VT, typename XT, typename AT, bool COMP>
-__global__ __launch_bounds__(256) void ell_spmv_kernel(int R, int width, const int* __restrict__ ecol,
+__global__ __launch_bounds__(256) void ell_spmv_kernel(int R, int C, int width, const int* __restrict__ ecol,
                                                        const VT* __restrict__ eval, const XT* __restrict__ x,
                                                        XT* __restrict__ y, AT alpha, AT beta) {
     long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= R) return;
     Acc<AT, COMP> a;
     a.zero();
+    // Slots in groups of U: all U column/value loads, then all U gathers, are
+    // issued before any is used. A padding slot gathers x[0] (its column is
+    // -1) and its product is skipped, so no load sits behind a branch. With no
+    // columns every slot is padding and x is never read.
+    constexpr int U = 8;
+    if (C <= 0) width = 0;
     int i = 0;
-    for (; i + 4 <= width; i += 4) {
-        int c[4];
-        VT v[4];
+    for (; i + U <= width; i += U) {
+        int c[U];
+        VT v[U];
+        XT xv[U];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            c[k] = ecol[row + (long)(i + k) * R];
-            v[k] = eval[row + (long)(i + k) * R];
-        }
+        for (int k = 0; k < U; ++k) c[k] = ntl(ecol + row + (long)(i + k) * R);
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (c[k] >= 0) a.fma(v[k], x[c[k]]);
+        for (int k = 0; k < U; ++k) v[k] = ntl(eval + row + (long)(i + k) * R);
+#pragma unroll
+        for (int k = 0; k < U; ++k) xv[k] = x[c[k] > 0 ? c[k] : 0];
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+            if (c[k] >= 0) a.fma(v[k], xv[k]);
     }
-    for (; i < width; ++i) {
-        long idx = row + (long)i * R;
-        int c = ecol[idx];
-        if (c >= 0) a.fma(eval[idx], x[c]);
+    if (i < width) {
+        int c[U];
+        VT v[U];
+        XT xv[U];
+        const int n = width - i;
+#pragma unroll
+        for (int k = 0; k < U; ++k) c[k] = ntl(ecol + row + (long)(i + (k < n ? k : n - 1)) * R);
+#pragma unroll
+        for (int k = 0; k < U; ++k) v[k] = ntl(eval + row + (long)(i + (k < n ? k : n - 1)) * R);
+#pragma unroll
+        for (int k = 0; k < U; ++k) xv[k] = x[c[k] > 0 ? c[k] : 0];
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+            if (k < n && c[k] >= 0) a.fma(v[k], xv[k]);
     }
     store_y(y, row, a.value(), alpha, beta);
 }
 
 template <typename VT, typename XT, typename AT, bool COMP>
-spmv_status_t launch_ell(int R, int width, const int* d_col, const VT* d_val, const XT* x, XT* y, AT alpha, AT beta,
-                         hipStream_t s) {
+spmv_status_t launch_ell(int R, int C, int width, const int* d_col, const VT* d_val, const XT* x, XT* y, AT alpha,
+                         AT beta, hipStream_t s) {
     if (R <= 0) return SPMV_SUCCESS;
-    hipLaunchKernelGGL((ell_spmv_kernel<VT, XT, AT, COMP>), dim3(grid_for(R, 256)), dim3(256), 0, s, R, width, d_col,
-                       d_val, x, y, alpha, beta);
+    hipLaunchKernelGGL((ell_spmv_kernel<VT, XT, AT, COMP>), dim3(grid_for(R, 256)), dim3(256), 0, s, R, C, width,
+                       d_col, d_val, x, y, alpha, beta);
     MUXI_LAUNCH_CHECK();
     return SPMV_SUCCESS;
 }
@@ -109,7 +127,7 @@ template spmv_status_t launch_csr_to_ell<double>(const DevCsr&, int, int*, doubl
 template spmv_status_t launch_csr_to_ell<bf16_t>(const DevCsr&, int, int*, bf16_t*, hipStream_t);
 
 #define INST(VT, XT, AT, COMP)                                                                                       \
-    template spmv_status_t launch_ell<VT, XT, AT, COMP>(int, int, const int*, const VT*, const XT*, XT*, AT, AT, \
+    template spmv_status_t launch_ell<VT, XT, AT, COMP>(int, int, int, const int*, const VT*, const XT*, XT*, AT, AT, \
                                                         hipStream_t);
 MUXI_FOR_EACH_COMBO(INST)
 #undef INST

@@ -80,18 +80,35 @@ __device__ __forceinline__ Acc<AT, COMP> merge_tile(MergeSmem<AT, COMP, IPT>& sm
     const int tid = threadIdx.x;
     const int n0 = d0 - r0, n1 = d1 - r1;
     const int nr = r1 - r0, nn = n1 - n0, tot = nr + nn;
+    // Load phase. Indices past the tile end are clamped to its last item, so
+    // no load sits behind a per-item branch: all IPT column, value and row-end
+    // loads, then all IPT x gathers, are in flight together. (A guarded load
+    // per item makes the compiler wait for each gather before the next.) The
+    // nn > 0 / nr > 0 tests are uniform over the workgroup.
+    int c[IPT], e[IPT];
+    VT v[IPT];
+    if (nn > 0) {
 #pragma unroll
-    for (int k = 0; k < IPT; ++k) {
-        int j = tid + k * B;
-        if (j < nn) {
-            int c = ci[n0 + j];
-            sm.prod[j] = P::make(vals[n0 + j], x[c]);
-        }
+        for (int k = 0; k < IPT; ++k) c[k] = ntl(ci + n0 + min(tid + k * B, nn - 1));
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) v[k] = ntl(vals + n0 + min(tid + k * B, nn - 1));
     }
+    if (nr > 0) {
 #pragma unroll
-    for (int k = 0; k < IPT; ++k) {
-        int i = tid + k * B;
-        if (i < nr) sm.end[i] = rp[r0 + 1 + i] - n0;
+        for (int k = 0; k < IPT; ++k) e[k] = ntl(rp + r0 + 1 + min(tid + k * B, nr - 1));
+    }
+    if (nn > 0) {
+        XT xv[IPT];
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) xv[k] = x[c[k]];
+#pragma unroll
+        for (int k = 0; k < IPT; ++k)
+            if (tid + k * B < nn) sm.prod[tid + k * B] = P::make(v[k], xv[k]);
+    }
+    if (nr > 0) {
+#pragma unroll
+        for (int k = 0; k < IPT; ++k)
+            if (tid + k * B < nr) sm.end[tid + k * B] = e[k] - n0;
     }
     __syncthreads();
 

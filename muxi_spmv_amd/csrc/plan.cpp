@@ -180,7 +180,7 @@ extern "C" spmv_status_t muxi_plan_create(const muxi_csr_t* Ain, spmv_type_t vt,
         }
         case MUXI_KERNEL_XSLAB: {
             int S = param0 >> 8, ipt = param0 & 0xff;
-            if (S <= 0) S = 8;
+            if (S <= 0) S = xslab_default_slabs(Ain->numCols, xt == SPMV_TYPE_DOUBLE ? 8 : 4);
             if (ipt <= 0) ipt = 4;
             if (!(ipt == 4 || ipt == 8 || ipt == 16) || S > kMaxSlabs) {
                 set_error("xslab: param0 = (slabs << 8) | ipt with slabs in [1, 16] and ipt in {4, 8, 16}");
@@ -370,7 +370,8 @@ static spmv_status_t plan_exec(const muxi_plan* p, const void* x, void* y, doubl
                 return launch_csr5<VT, XT, AT, C>(p->A, p->param0, p->d_tile_row, p->T, p->d_carry, xx, yy, a, b, s,
                                                   ev1);
             case MUXI_KERNEL_ELL:
-                return launch_ell<VT, XT, AT, C>(p->A.R, p->width, p->d_ecol, (const VT*)p->d_eval, xx, yy, a, b, s);
+                return launch_ell<VT, XT, AT, C>(p->A.R, p->A.C, p->width, p->d_ecol, (const VT*)p->d_eval, xx, yy, a, b,
+                                                 s);
             case MUXI_KERNEL_XSLAB:
                 if constexpr (C) {
                     set_error("xslab: no compensated variant");

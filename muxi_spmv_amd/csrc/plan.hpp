@@ -48,11 +48,12 @@ struct muxi_plan {
     size_t scratch_bytes = 0;
     // SpMM: right-hand sides (0 for a SpMV plan)
     int nrhs = 0;
-    // column slabs (xslab): one allocation holding rowPtr, row ids, columns, values
+    // column slabs (xslab): one allocation holding rowPtr, pair slots, columns, values
     muxi::SlabArgs slab{};
+    int slab_W = 1;  // persistent workgroups per slab
     int* d_slab = nullptr;
     int* d_slab_rp = nullptr;
-    int* d_slab_rid = nullptr;
+    int* d_slab_ploc = nullptr;  // P slot of each pair, pairs in row-major order
     int* d_slab_ci = nullptr;
     void* d_slab_v = nullptr;
     int* d_slab_rpp = nullptr;   // row-major pair offsets [R+1]
@@ -94,6 +95,15 @@ inline int merge_default_partitions(int L, int ipt) {
 }
 
 inline size_t vsize(spmv_type_t t) { return t == SPMV_TYPE_DOUBLE ? 8 : (t == SPMV_TYPE_BF16 ? 2 : 4); }
+
+// Slabs for a column-slab plan when the caller gives none: the fewest (1, 2, 4
+// or 8) that keep one slab of x within 2 MiB, half of an XCD's L2.
+inline int xslab_default_slabs(long cols, int xbytes) {
+    const long b = cols * xbytes;
+    int S = 1;
+    while (S < 8 && b > (long)S * (2L << 20)) S <<= 1;
+    return S;
+}
 
 // column-slab plan and launcher (kernels_xslab.hip); param0 = (slabs << 8) | items per lane
 spmv_status_t xslab_build(muxi_plan* p, int S, int ipt, spmv_type_t vt, hipStream_t s);

@@ -1,11 +1,13 @@
 // Learned kernel selection for MUXI_KERNEL_AUTO.
 //
-// The caller installs a trained selector MLP (weights, input normalisation and
-// the candidate list it scores). For an AUTO plan the library computes the
-// matrix features on the device, forms the model input, runs the MLP with the
-// HIP forward kernels, masks candidates that the autotune sweep would not run
-// on this matrix, and takes the candidate with the smallest predicted
-// log-time. Without a selector installed it falls back to a fixed rule.
+// A selector is an MLP (weights, input normalisation and the candidate list
+// it scores). For an AUTO plan the library computes the matrix features on
+// the device, forms the model input, runs the MLP with the HIP forward
+// kernels, masks candidates that the autotune sweep would not run on this
+// matrix, and takes the candidate with the smallest predicted log-time.
+// When the build found trained weights (selector_weights.inc, generated from
+// the package's selector_weights.json) they are the selector until the caller
+// installs another one or clears it; with none, a fixed rule decides.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -15,6 +17,13 @@
 #include "common.hpp"
 #include "muxi_spmv/muxi_spmv_tune.h"
 #include "plan.hpp"
+
+#if __has_include("selector_weights.inc")
+#include "selector_weights.inc"
+#define MUXI_BUILTIN_SELECTOR 1
+#else
+#define MUXI_BUILTIN_SELECTOR 0
+#endif
 
 using namespace muxi;
 
@@ -32,6 +41,9 @@ struct Selector {
 
 std::mutex g_mu;
 Selector* g_sel = nullptr;
+// which selector decides: the built-in one (installed on first use), one the
+// caller set, or none (fixed rule)
+int g_state = MUXI_BUILTIN_SELECTOR ? 1 : 0;
 
 void free_sel(Selector* s) {
     if (!s) return;
@@ -66,8 +78,8 @@ static bool candidate_allowed(int kernel, const double* f) {
     return true;
 }
 
-extern "C" spmv_status_t muxi_selector_set(const float* params, int n_in, int hidden, int n_out, const float* mu,
-                                           const float* sd, const int* kernels, const int* params0) {
+static spmv_status_t make_selector(const float* params, int n_in, int hidden, int n_out, const float* mu,
+                                   const float* sd, const int* kernels, const int* params0, Selector** out) {
     if (!params || !mu || !sd || !kernels || !params0 || n_in != 10 || hidden <= 0 || n_out <= 0) {
         set_error("selector: invalid model (expects 10 inputs)");
         return SPMV_ERROR_INVALID_VECTOR;
@@ -90,9 +102,19 @@ extern "C" spmv_status_t muxi_selector_set(const float* params, int n_in, int hi
         set_error("selector: device allocation failed");
         return SPMV_ERROR_MEMORY_ALLOC;
     }
+    *out = s;
+    return SPMV_SUCCESS;
+}
+
+extern "C" spmv_status_t muxi_selector_set(const float* params, int n_in, int hidden, int n_out, const float* mu,
+                                           const float* sd, const int* kernels, const int* params0) {
+    Selector* s = nullptr;
+    spmv_status_t st = make_selector(params, n_in, hidden, n_out, mu, sd, kernels, params0, &s);
+    if (st != SPMV_SUCCESS) return st;
     std::lock_guard<std::mutex> lk(g_mu);
     free_sel(g_sel);
     g_sel = s;
+    g_state = 2;
     return SPMV_SUCCESS;
 }
 
@@ -100,15 +122,30 @@ extern "C" spmv_status_t muxi_selector_clear(void) {
     std::lock_guard<std::mutex> lk(g_mu);
     free_sel(g_sel);
     g_sel = nullptr;
+    g_state = 0;
     return SPMV_SUCCESS;
 }
 
-// Fixed rule without a model: merge-path, or column slabs when x does not fit
-// one XCD's L2 (4 MiB) and the rows are short on average.
+extern "C" spmv_status_t muxi_selector_reset(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    free_sel(g_sel);
+    g_sel = nullptr;
+    g_state = MUXI_BUILTIN_SELECTOR ? 1 : 0;
+    return SPMV_SUCCESS;
+}
+
+extern "C" int muxi_selector_state(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_state;
+}
+
+// Fixed rule without a model: merge-path, or column slabs (default slab count,
+// 4 items per lane) when x does not fit one XCD's L2 (4 MiB) and the rows are
+// short on average.
 static void fallback_rule(const double* f, double x_bytes, int* kernel, int* param0) {
     if (x_bytes > 4.0 * 1024 * 1024 && f[2] < 64) {
         *kernel = MUXI_KERNEL_XSLAB;
-        *param0 = (8 << 8) | 4;
+        *param0 = 4;
     } else {
         *kernel = MUXI_KERNEL_MERGE;
         *param0 = 4;
@@ -122,6 +159,12 @@ extern "C" spmv_status_t muxi_selector_pick(const muxi_csr_t* A, spmv_type_t xty
     spmv_status_t st = muxi_matrix_features(A, stream, f);
     if (st != SPMV_SUCCESS) return st;
     std::lock_guard<std::mutex> lk(g_mu);
+#if MUXI_BUILTIN_SELECTOR
+    if (g_state == 1 && !g_sel) {  // first use of the built-in selector on this process
+        st = make_selector(kSelParams, kSelIn, kSelHidden, kSelOut, kSelMu, kSelSd, kSelKernel, kSelParam0, &g_sel);
+        if (st != SPMV_SUCCESS) return st;
+    }
+#endif
     if (!g_sel) {
         fallback_rule(f, (double)A->numCols * (xtype == SPMV_TYPE_DOUBLE ? 8 : 4), kernel, param0);
         return SPMV_SUCCESS;

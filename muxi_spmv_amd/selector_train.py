@@ -7,8 +7,9 @@ python -m muxi_spmv_amd.selector_train [--train 2000] [--test 400] [--db PATH] [
    row and column permutations; family = i mod 5, seed 6 + i), rows
    log-uniform in [1e4, 4e6], FP32, generated on the device.
 2. The autotune sweep times every candidate kernel/configuration cold and
-   appends one JSON record per matrix to the DB (resumable: indices already
-   in the DB are skipped).
+   appends one JSON record per matrix to the DB (resumable: a record is
+   only topped up with the candidates it lacks; --skip leaves kernels out of
+   a call, --retime times kernels again after they changed).
 3. The MLP (HIP forward/backward/Adam kernels) is fit on the training
    records to predict log(t_candidate / t_best); its pick is the argmin.
 4. On the held-out matrices it reports the median and p95 regret
@@ -67,24 +68,40 @@ def load_db(path: Path) -> dict[int, dict]:
     return recs
 
 
-def run_sweeps(indices, db_path: Path, rows_min, rows_max, reps, log_every=25, time_budget_s=None):
+def _todo(rec, cands, retime):
+    """Candidates still to time on a record: feasible ones it lacks, plus those named for re-timing."""
+    if rec is None:
+        return list(cands)
+    f = np.array(rec["features"])
+    return [(k, p) for k, p in cands
+            if tune.feasible(k, f) and (f"{k}:{p}" not in rec["times"] or k in retime)]
+
+
+def run_sweeps(indices, db_path: Path, rows_min, rows_max, reps, log_every=25, time_budget_s=None,
+               skip=(), retime=()):
+    """Sweeps matrices into the DB (one JSON line per matrix; a later line for the same index replaces an
+    earlier one). Kernels in `skip` are not timed; kernels in `retime` are timed again on records that
+    already hold them; candidates a record lacks are added to it."""
     done = load_db(db_path)
+    cands = [(k, p) for k, p in tune.CANDIDATES if k not in skip]
     fl = tune.Flusher()
     t0 = time.time()
     n_new = 0
     with open(db_path, "a") as f:
         for i in indices:
-            if i in done:
+            todo = _todo(done.get(i), cands, retime)
+            if not todo:
                 continue
             if time_budget_s is not None and time.time() - t0 > time_budget_s:
-                print(f"[sweep] time budget reached after {n_new} new matrices", flush=True)
+                print(f"[sweep] time budget reached after {n_new} matrices", flush=True)
                 break
             spec = matrix_spec(i, rows_min, rows_max)
             A = build(spec)
             x = tune.gen_vector(A.numCols, seed=spec["seed"])
             y = torch.empty(A.numRows, device="cuda")
-            feats = tune.features(A)
-            times = tune.sweep(A, x, y, fl, reps=reps, f=feats)
+            feats = tune.features(A) if i not in done else np.array(done[i]["features"])
+            times = dict(done[i]["times"]) if i in done else {}
+            times.update(tune.sweep(A, x, y, fl, reps=reps, candidates=todo, f=feats))
             rec = dict(spec, nnz=int(A.nnz), features=feats.tolist(), times=times)
             f.write(json.dumps(rec) + "\n")
             f.flush()
@@ -155,6 +172,8 @@ def main(argv=None) -> int:
     ap.add_argument("--train-only", action="store_true")
     ap.add_argument("--epochs", type=int, default=4000)
     ap.add_argument("--time-budget", type=float, default=None, help="seconds of sweeping in this call")
+    ap.add_argument("--skip", default="", help="comma-separated kernels not to time in this call")
+    ap.add_argument("--retime", default="", help="comma-separated kernels to time again on existing records")
     a = ap.parse_args(argv)
     torch.cuda.set_device(0)
     db = Path(a.db)
@@ -163,7 +182,8 @@ def main(argv=None) -> int:
     if not a.train_only:
         # interleave so that a partial run covers both splits
         order = list(range(a.train, n)) + list(range(a.train))
-        run_sweeps(order, db, a.rows_min, a.rows_max, a.reps, time_budget_s=a.time_budget)
+        run_sweeps(order, db, a.rows_min, a.rows_max, a.reps, time_budget_s=a.time_budget,
+                   skip=tuple(k for k in a.skip.split(",") if k), retime=tuple(k for k in a.retime.split(",") if k))
         if a.sweep_only:
             return 0
     recs = load_db(db)

@@ -31,7 +31,7 @@ CANDIDATES: list[tuple[str, int]] = [
     ("merge", 4), ("merge", 8), ("merge", 16),
     ("csr5", 256), ("csr5", 512), ("csr5", 1024),
     ("adaptive", 512), ("adaptive", 1024), ("adaptive", 2048),
-    ("xslab", (8 << 8) | 4), ("xslab", (8 << 8) | 8),
+    ("xslab", (2 << 8) | 4), ("xslab", (4 << 8) | 4), ("xslab", (8 << 8) | 4),
     ("ell", 0),
 ]
 CAND_NAMES = [f"{k}:{p}" for k, p in CANDIDATES]
@@ -151,16 +151,23 @@ def time_plan_cold(plan: Plan, x, y, reps: int, flusher: Flusher) -> float:
     return statistics.median(ts)
 
 
+def feasible(kernel: str, f: np.ndarray) -> bool:
+    """Whether the sweep times this kernel on a matrix with features f."""
+    if kernel == "ell":
+        return ell_feasible(f)
+    if kernel in ("scalar", "vector"):
+        return f[4] <= 20000  # one lane / one wavefront per row on 10^4+ rows: hopeless and slow to time
+    return True
+
+
 def sweep(A, x: torch.Tensor, y: torch.Tensor, flusher: Flusher, reps: int = 5,
           candidates=CANDIDATES, f: np.ndarray | None = None) -> dict[str, float]:
     """Cold time (ms) of every feasible candidate; infeasible ones are absent."""
     f = features(A) if f is None else f
     out = {}
     for k, p in candidates:
-        if k == "ell" and not ell_feasible(f):
+        if not feasible(k, f):
             continue
-        if k in ("scalar", "vector") and f[4] > 20000:
-            continue  # one lane / one wavefront per row on 10^4+ rows: hopeless and slow to time
         plan = Plan(A, k, p, xtype=x.dtype)
         time_plan_cold(plan, x, y, 1, flusher)  # warm the code path
         out[f"{k}:{p}"] = time_plan_cold(plan, x, y, reps, flusher)
@@ -260,9 +267,31 @@ class Selector:
     def save(self, path: str | Path) -> None:
         Path(path).write_text(json.dumps(self.state()))
 
+    def install(self) -> None:
+        """Makes this network the library's selector for MUXI_KERNEL_AUTO plans (muxi_selector_set)."""
+        ks = (C.c_int * self.n_out)(*[_lib.KERNELS[k] for k, _ in CANDIDATES])
+        ps = (C.c_int * self.n_out)(*[p for _, p in CANDIDATES])
+        params = self.params.detach().float().cpu().contiguous()
+        mu = (C.c_float * self.n_in)(*self.mu.tolist())
+        sd = (C.c_float * self.n_in)(*self.sd.tolist())
+        check(load().muxi_selector_set(params.data_ptr(), self.n_in, self.h, self.n_out, mu, sd, ks, ps),
+              "selector_set")
+
     @classmethod
     def load(cls, path: str | Path) -> "Selector":
         return cls.from_state(json.loads(Path(path).read_text()))
+
+
+def library_pick(A, xtype=torch.float32) -> str:
+    """The (kernel, param) an AUTO plan takes for A, as a candidate name (muxi_selector_pick)."""
+    k, p = C.c_int(), C.c_int()
+    check(load().muxi_selector_pick(C.byref(A.c), _TORCH2SPMV[xtype], torch.cuda.current_stream().cuda_stream or None,
+                                    C.byref(k), C.byref(p)), "selector_pick")
+    return f"{_lib.KERNEL_NAMES[k.value]}:{p.value}"
+
+
+def feasible_mask(f: np.ndarray) -> np.ndarray:
+    return np.array([1.0 if feasible(k, f) else 0.0 for k, _ in CANDIDATES], np.float32)
 
 
 def targets_from_sweep(times: dict[str, float]) -> tuple[np.ndarray, np.ndarray]:

@@ -18,7 +18,8 @@ ZOO = matrix_zoo()
 KCONF = [("scalar", 0), ("vector", 0)] + [("vwave", k) for k in (1, 2, 4, 8, 16, 32, 64)] + \
         [("merge", k) for k in (2, 4, 8, 16)] + [("csr5", s) for s in (64, 128, 256, 512, 1024)] + [("ell", 0)] + \
         [("adaptive", b) for b in (256, 512, 1024, 2048)] + \
-        [("xslab", (S << 8) | ipt) for S, ipt in ((8, 4), (8, 8), (1, 16), (3, 8), (16, 4))]
+        [("xslab", (S << 8) | ipt) for S, ipt in ((8, 4), (8, 8), (1, 16), (3, 8), (16, 4), (2, 4), (4, 4))] + \
+        [("xslab", 0)]
 
 
 def xvec(n, seed=11, dtype=np.float32):
@@ -49,7 +50,7 @@ def test_plan_fp32(mname, kernel, param):
 
 @pytest.mark.parametrize("mname", ["uniform", "powerlaw", "empty_rows", "long_row", "tiny"])
 @pytest.mark.parametrize("kernel,param", [("scalar", 0), ("vwave", 8), ("merge", 8), ("csr5", 256), ("ell", 0),
-                                          ("adaptive", 1024), ("xslab", 0)])
+                                          ("adaptive", 1024), ("xslab", 1028)])
 def test_plan_fp64(mname, kernel, param):
     h = ZOO[mname]
     if kernel == "ell" and h.row_lengths().max(initial=0) * h.numRows > 50_000_000:
@@ -61,7 +62,7 @@ def test_plan_fp64(mname, kernel, param):
 
 @pytest.mark.parametrize("mname", ["uniform", "powerlaw", "long_row", "empty_rows"])
 @pytest.mark.parametrize("kernel,param", [("vector", 0), ("vwave", 4), ("merge", 8), ("csr5", 256), ("ell", 0),
-                                          ("adaptive", 1024), ("xslab", 0)])
+                                          ("adaptive", 1024), ("xslab", 1028)])
 @pytest.mark.parametrize("vdtype", [torch.bfloat16, torch.float64])
 @pytest.mark.parametrize("comp", [False, True])
 def test_plan_mixed_precision(mname, kernel, param, vdtype, comp):
@@ -100,7 +101,7 @@ def test_compensated_beats_plain_on_long_rows(kernel, param):
 
 
 @pytest.mark.parametrize("kernel,param", [("scalar", 0), ("vwave", 16), ("merge", 4), ("csr5", 512), ("ell", 0),
-                                          ("adaptive", 512), ("xslab", 0)])
+                                          ("adaptive", 512), ("xslab", 1028)])
 def test_alpha_beta(kernel, param):
     h = ZOO["uniform"]
     y0 = torch.from_numpy(xvec(h.numRows, seed=5)).cuda()
@@ -110,7 +111,7 @@ def test_alpha_beta(kernel, param):
     assert np.max(np.abs(y.double().cpu().numpy() - exp)) < 1e-5
 
 
-@pytest.mark.parametrize("kernel,param", [("merge", 8), ("csr5", 256), ("vwave", 8), ("adaptive", 1024), ("xslab", 0)])
+@pytest.mark.parametrize("kernel,param", [("merge", 8), ("csr5", 256), ("vwave", 8), ("adaptive", 1024), ("xslab", 1028)])
 def test_bitwise_deterministic(kernel, param):
     h = ZOO["powerlaw"]
     y1, x, A, p = run_plan(h, kernel, param)
@@ -289,7 +290,7 @@ def powerlaw_full():
     return h, dev_csr(h)
 
 
-@pytest.mark.parametrize("kernel,param", [("merge", 4), ("merge", 8), ("csr5", 256), ("adaptive", 1024), ("xslab", 0)])
+@pytest.mark.parametrize("kernel,param", [("merge", 4), ("merge", 8), ("csr5", 256), ("adaptive", 1024), ("xslab", 1028)])
 def test_full_size_powerlaw_properties(powerlaw_full, kernel, param):
     """2M x 2M power-law (14.4M nnz): rel-L2 <= 1e-6 vs the device FP64 oracle,
     bitwise reproducible, exactly homogeneous under x -> 2x, additive within tolerance."""
@@ -333,7 +334,7 @@ def test_full_size_laplacian_exact():
     exp[:, 0] += 1
     exp[:, -1] += 1
     exp = exp.ravel().cuda()
-    for kernel, param in [("merge", 8), ("csr5", 256), ("vwave", 4), ("ell", 0), ("adaptive", 1024), ("xslab", 0)]:
+    for kernel, param in [("merge", 8), ("csr5", 256), ("vwave", 4), ("ell", 0), ("adaptive", 1024), ("xslab", 1028)]:
         y = torch.full((n * n,), float("nan"), dtype=torch.float64, device="cuda")
         api.Plan(A, kernel, param, xtype=torch.float64).exec(x, y)
         torch.cuda.synchronize()
