@@ -5,8 +5,12 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload powerlaw|uniform
 A step is one y = A*x over the whole matrix with A, x, y resident in HBM.
 Every step reads a different copy of (A, x, y) (enough copies that the
 others evict it from the 256 MiB Infinity Cache), so each step is a cold
-read. The kernel and its tile size are picked by the library's autotune
-sweep over the candidate kernels (the sweep runs before the timed region).
+read. By default the timed kernel is the library's own choice for an AUTO
+plan (the learned selector built into the library); the autotune sweep over
+the candidate kernels runs before the timed region, and the line reports the
+pick's regret against the sweep's best (t_pick / t_best - 1, both timed the
+same way). --kernel sweep times the sweep's best instead, --kernel k:p one
+configuration.
 
 Prints one JSON line: value = algorithmic GB/s over the K timed steps
 (bytes = N*(4+4) + 4*(R+1) + 4*C + 4*R per step); roofline = the main
@@ -76,7 +80,8 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="powerlaw")
-    ap.add_argument("--kernel", default="sweep", help="sweep, or kernel:param (e.g. merge:8)")
+    ap.add_argument("--kernel", default="auto",
+                    help="auto (the library's pick, sweep timed beside it), sweep (the sweep's best), or kernel:param")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep-steps", type=int, default=30)
     a = ap.parse_args(argv)
@@ -125,7 +130,7 @@ def main(argv=None) -> int:
     log(f"device copy: {copy_gbs:.0f} GB/s")
 
     sweep = {}
-    if a.kernel == "sweep":
+    if a.kernel in ("auto", "sweep"):
         maxlen = int(np.diff(h.rowPtr).max())
         cands = [c for c in CANDIDATES if not (c[0] in ("vector", "scalar") and h.nnz > 50_000_000)
                  and not (c[0] == "ell" and h.numRows * maxlen > 4 * h.nnz)]
@@ -136,11 +141,25 @@ def main(argv=None) -> int:
             log(f"sweep {k}:{p} {m.ms_per_step * 1e3:.1f} us/step")
             del plans
         best = min(sweep, key=sweep.get)
+    selector = None
+    if a.kernel == "auto":
+        from muxi_spmv_amd._lib import load
+        plans = [api.Plan(m, "auto", 0, xtype=tdt) for m in mats]
+        info = plans[0].info()
+        pick = f"{info.kernel}:{info.param0}"
+        if pick not in sweep:  # time the pick the way the sweep timed its candidates
+            m = measure_plans(plans, xs, ys, a.sweep_steps, 3, record_kernel=False)
+            sweep[pick] = round(m.ms_per_step, 5)
+        state = {0: "fixed rule", 1: "learned (built in)", 2: "learned (set by caller)"}[load().muxi_selector_state()]
+        selector = {"state": state, "pick": pick, "sweep_best": best,
+                    "regret": round(sweep[pick] / sweep[best] - 1.0, 4)}
+        log(f"selector ({state}) picks {pick}; sweep best {best}; regret {selector['regret']:.3f}")
     else:
-        best = a.kernel
-    kname, kparam = best.split(":")
-    plans = [api.Plan(m, kname, int(kparam), xtype=tdt) for m in mats]
-    info = plans[0].info()
+        if a.kernel != "sweep":
+            best = a.kernel
+        kname, kparam = best.split(":")
+        plans = [api.Plan(m, kname, int(kparam), xtype=tdt) for m in mats]
+        info = plans[0].info()
 
     torch.cuda.synchronize()
     meas = measure_plans(plans, xs, ys, a.steps, a.warmup)
@@ -148,7 +167,8 @@ def main(argv=None) -> int:
     value = bytes_step / ms / 1e6
     kms = meas.kernel_ms_mean
     achieved = bytes_step / kms / 1e6
-    log(f"{best}: {ms * 1e3:.2f} us/step, main kernel {kms * 1e3:.2f} us (median {meas.kernel_ms_median * 1e3:.2f})")
+    log(f"{info.kernel}:{info.param0}: {ms * 1e3:.2f} us/step, main kernel {kms * 1e3:.2f} us "
+        f"(median {meas.kernel_ms_median * 1e3:.2f})")
 
     # correctness of the measured configuration against the device FP64 oracle
     y = torch.zeros(h.numRows, dtype=tdt, device="cuda")
@@ -181,6 +201,7 @@ def main(argv=None) -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel_us_mean": round(kms * 1e3, 3), "bytes_per_launch": bytes_step},
         "sweep_ms": sweep,
+        "selector": selector,
     }
     if not a.no_cpu:
         log("cpu baseline ...")

@@ -19,6 +19,7 @@ python -m muxi_spmv_amd.selector_train [--train 2000] [--test 400] [--db PATH] [
 from __future__ import annotations
 
 import argparse
+import gzip
 import json
 import math
 import sys
@@ -59,9 +60,11 @@ def build(spec: dict):
 
 
 def load_db(path: Path) -> dict[int, dict]:
+    """Records by matrix index (a later line for an index replaces an earlier one); .gz files are read as gzip."""
     recs = {}
     if path.exists():
-        for line in path.read_text().splitlines():
+        text = gzip.decompress(path.read_bytes()).decode() if path.suffix == ".gz" else path.read_text()
+        for line in text.splitlines():
             if line.strip():
                 r = json.loads(line)
                 recs[r["i"]] = r
