@@ -35,6 +35,16 @@ CANDIDATES = [("merge", 4), ("merge", 8), ("merge", 16), ("csr5", 256), ("csr5",
               ("vwave", 4), ("vwave", 8), ("vector", 0), ("scalar", 0)]
 
 
+# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
+# `bench.py --kernel K:P` (KiB per dispatch, FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors a 1 GiB copy
+# kernel reads); the runs and commands are in profiles/xslab_spmv_powerlaw_2m_fp32.md and
+# profiles/ell_spmv_laplacian_4096_fp64.md. Configurations not measured report null.
+MEASURED_TRAFFIC = {
+    ("powerlaw_2m_fp32", "xslab", 1028): int((78316.6 * 2.0 + 16587.4 * 1.0) * 1024),
+    ("laplacian_4096_fp64", "ell", 5): int((572854.8 * 2.0 + 131072.0 * 1.0) * 1024),
+}
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -198,7 +208,8 @@ def main(argv=None) -> int:
         "rel_l2_vs_fp64": rel_l2,
         "hbm_copy_gbs": round(copy_gbs, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((wname, info.kernel, info.param0)),
                      "kernel_us_mean": round(kms * 1e3, 3), "bytes_per_launch": bytes_step},
         "sweep_ms": sweep,
         "selector": selector,
