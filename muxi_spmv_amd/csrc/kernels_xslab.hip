@@ -1,5 +1,4 @@
-// Column-slab merge-path SpMV ("xslab"): keeps each XCD's x gathers in its
-// own L2.
+// Column-slab SpMV ("xslab"): keeps each XCD's x gathers in its own L2.
 //
 // MI355X has 8 XCDs, each with a private 4 MiB L2, and hands workgroup i to
 // XCD i mod 8. When x is larger than one L2 (the 2M-column power-law matrix
@@ -8,195 +7,180 @@
 // 139 MB of matrix and vectors (TCC_MISS x 128 B). The plan splits the
 // columns into S slabs (column c is in slab floor(c*S/C)) and stores, per
 // slab, a compressed CSR of the nonzeros that fall in it: slab-local row
-// pointers over the rows that have any, columns and values. Workgroup b works
-// on slab b mod S, so with S = 8 an XCD gathers from one 1/8 slice of x that
-// stays in its L2; the matrix stream uses non-temporal loads so it does not
-// evict that slice.
+// pointers over the rows that have any (so no slab row is empty), columns
+// and values, padded to whole tiles, plus one row-end bit per nonzero and a
+// descriptor per tile (the slab row of its first nonzero). Workgroup b works
+// on slab b mod S, so an XCD gathers from one slice of x that stays in its
+// L2; the matrix stream uses non-temporal loads so it does not evict that
+// slice.
 //
 // A (row, slab) pair with nonzeros is a "pair"; pairs are numbered row-major,
-// so a row's pairs are contiguous. Each slab runs the merge-path algorithm
-// over its compressed rows and stores a row's partial sum to P[pair] (plain
-// stores, no atomics); the partial of the row open at a tile end goes to a
-// carry keyed by its pair, and a keyed fix-up adds carries in tile order. A
-// combine kernel then forms y[r] = alpha * (sum of the row's pairs, in slab
-// order) + beta * y[r]. Every sum has a fixed order: results are reproducible.
+// so a row's pairs are contiguous. The slab kernel stores a row's partial
+// sum to the row's slot in its slab (slab-local order; plain stores, no
+// atomics); the partial of the row open at a tile end goes to a carry keyed
+// by its slot, and a keyed fix-up adds carries in tile order. A combine
+// kernel then forms y[r] = alpha * (sum of the row's pairs, in slab order) +
+// beta * y[r]. Every sum has a fixed order: results are reproducible.
 #include <hipcub/hipcub.hpp>
 
 #include <climits>
 #include <vector>
 
-#include "merge_tile.hpp"
+#include "kernels.hpp"
 #include "plan.hpp"
 
 namespace muxi {
 
-// Registers of one tile in flight: columns, values, row ends and gathered x.
-template <typename VT, typename XT, int IPT>
-struct TileRegs {
-    int c[IPT], e[IPT];
-    VT v[IPT];
-    XT x[IPT];
-};
-
-struct TileMeta {
-    int n0, r0, nn, nr;
-};
-
-// Persistent, software-pipelined column-slab kernel. Workgroup b works on
-// slab b mod S (so on one XCD's x slice) and walks a contiguous run of that
-// slab's merge tiles. While it reduces tile t out of LDS, the x gathers of
-// tile t+1 and the column/value/row-end loads of tile t+2 are in flight, so
-// the memory system never waits for the reduction. Clamped indices keep all
-// loads unconditional. A row's partial goes to P[row_base[s] + slab row]
-// (slab-local order: consecutive rows store to consecutive slots); the row
-// left open at a tile end goes to the tile's carry, keyed by its slot.
-template <typename VT, typename XT, typename AT, int IPT>
-__global__ __launch_bounds__(256) void xslab_kernel(SlabArgs sa, int W, const int* __restrict__ srp,
-                                                    const int* __restrict__ sci, const VT* __restrict__ svals,
-                                                    const XT* __restrict__ x, const int* __restrict__ gdiag,
-                                                    const int* __restrict__ grow, AT* __restrict__ P,
-                                                    AT* __restrict__ carry, int* __restrict__ carry_key) {
-    constexpr int B = 256, CH = B * IPT;
-    using A = Acc<AT, false>;
-    __shared__ int s_end[CH];
-    __shared__ AT s_prod[CH];
-    __shared__ A s_agg[4];
-    __shared__ int s_flag[4];
-    const int s = blockIdx.x % sa.S, j = blockIdx.x / sa.S;
-    const int T = sa.tiles[s];
-    const int tb = (int)((long)j * T / W), te = (int)((long)(j + 1) * T / W);
-    if (tb >= te) return;  // uniform over the workgroup
-    const int tid = threadIdx.x;
-    const int* rp = srp + sa.row_base[s] + s;
-    const int* ci = sci + sa.nnz_base[s];
-    const VT* vals = svals + sa.nnz_base[s];
-    const int* gd = gdiag + sa.coord_base[s];
-    const int* gr = grow + sa.coord_base[s];
-    const int rows = sa.rows[s], pbase = sa.row_base[s], cbase = sa.tile_base[s];
-
-    auto meta = [&](int t) {
-        TileMeta m;
-        const int d0 = gd[t], r0 = gr[t], d1 = gd[t + 1], r1 = gr[t + 1];
-        m.n0 = d0 - r0;
-        m.r0 = r0;
-        m.nn = (d1 - r1) - m.n0;
-        m.nr = r1 - r0;
-        return m;
-    };
-    auto loads = [&](const TileMeta& m, TileRegs<VT, XT, IPT>& g) {
-        if (m.nn > 0) {
+// N consecutive elements with one non-temporal vector load (the address is
+// aligned to N elements: slab regions start on tile boundaries).
+template <typename T, int N>
+struct VecLoad {
+    __device__ __forceinline__ static void run(const T* p, T* out) {
+        typedef T V __attribute__((ext_vector_type(N)));
+        const V v = __builtin_nontemporal_load((const V*)p);
 #pragma unroll
-            for (int k = 0; k < IPT; ++k) g.c[k] = ntl(ci + m.n0 + min(tid + k * B, m.nn - 1));
-#pragma unroll
-            for (int k = 0; k < IPT; ++k) g.v[k] = ntl(vals + m.n0 + min(tid + k * B, m.nn - 1));
-        }
-        if (m.nr > 0) {
-#pragma unroll
-            for (int k = 0; k < IPT; ++k) g.e[k] = ntl(rp + m.r0 + 1 + min(tid + k * B, m.nr - 1));
-        }
-    };
-    auto gathers = [&](const TileMeta& m, TileRegs<VT, XT, IPT>& g) {
-        if (m.nn > 0) {
-#pragma unroll
-            for (int k = 0; k < IPT; ++k) g.x[k] = x[g.c[k]];
-        }
-    };
-    auto stage = [&](const TileMeta& m, const TileRegs<VT, XT, IPT>& g) {
-        if (m.nn > 0) {
-#pragma unroll
-            for (int k = 0; k < IPT; ++k)
-                if (tid + k * B < m.nn) s_prod[tid + k * B] = Conv<AT>::f(g.v[k]) * Conv<AT>::f(g.x[k]);
-        }
-        if (m.nr > 0) {
-#pragma unroll
-            for (int k = 0; k < IPT; ++k)
-                if (tid + k * B < m.nr) s_end[tid + k * B] = g.e[k] - m.n0;
-        }
-    };
-
-    // Reduces tile t (staged in LDS) while tile t+1's gathers (into G1) and
-    // tile t+2's loads (into G2) are in flight, then stages tile t+1. The
-    // loop below alternates the two register sets so no set is copied (a copy
-    // would wait for its loads).
-    TileMeta cur = meta(tb), nb{0, 0, 0, 0}, nc{0, 0, 0, 0};
-    auto step = [&](int t, TileRegs<VT, XT, IPT>& G1, TileRegs<VT, XT, IPT>& G2) {
-        if (t + 1 < te) gathers(nb, G1);
-        if (t + 2 < te) {
-            nc = meta(t + 2);
-            loads(nc, G2);
-        }
-        const int nr = cur.nr, nn = cur.nn, tot = nr + nn, r0 = cur.r0;
-        const int di = min(tid * IPT, tot), de = min(di + IPT, tot);
-        int lo = max(0, di - nn), hi = min(di, nr);
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (s_end[mid] <= di - mid - 1)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        const int rs = lo, ns = di - lo;
-        A acc;
-        acc.zero();
-        int fl = 0;
-        {
-            int r = rs, n = ns;
-#pragma unroll
-            for (int k = 0; k < IPT; ++k)
-                if (di + k < de) {
-                    if (r < nr && s_end[r] <= n) {
-                        acc.zero();
-                        fl = 1;
-                        ++r;
-                    } else {
-                        acc.s += s_prod[n++];
-                    }
-                }
-        }
-        A total;
-        A ex = block_seg_scan_excl<A, B / 64>(acc, fl, s_agg, s_flag, &total);
-        {
-            A a = ex;
-            int r = rs, n = ns;
-#pragma unroll
-            for (int k = 0; k < IPT; ++k)
-                if (di + k < de) {
-                    if (r < nr && s_end[r] <= n) {
-                        P[pbase + r0 + r] = a.s;
-                        a.zero();
-                        ++r;
-                    } else {
-                        a.s += s_prod[n++];
-                    }
-                }
-        }
-        if (tid == 0) {
-            const int r1 = r0 + nr, n1 = cur.n0 + nn;
-            const bool open = r1 < rows && n1 > rp[r1];
-            carry[cbase + t] = open ? total.s : AT(0);
-            carry_key[cbase + t] = open ? pbase + r1 : -1;
-        }
-        __syncthreads();  // every lane is done with LDS (and with s_agg)
-        if (t + 1 < te) {
-            stage(nb, G1);
-            cur = nb;
-            nb = nc;
-        }
-        __syncthreads();
-    };
-
-    TileRegs<VT, XT, IPT> ga, gb;
-    loads(cur, ga);
-    gathers(cur, ga);
-    stage(cur, ga);
-    if (tb + 1 < te) {
-        nb = meta(tb + 1);
-        loads(nb, gb);
+        for (int j = 0; j < N; ++j) out[j] = v[j];
     }
-    __syncthreads();
-    for (int t = tb; t < te; t += 2) {
-        step(t, gb, ga);
-        if (t + 1 < te) step(t + 1, ga, gb);
+};
+template <int N>
+struct VecLoad<bf16_t, N> {
+    __device__ __forceinline__ static void run(const bf16_t* p, bf16_t* out) {
+        typedef uint16_t V __attribute__((ext_vector_type(N)));
+        const V v = __builtin_nontemporal_load((const V*)p);
+#pragma unroll
+        for (int j = 0; j < N; ++j) out[j].bits = v[j];
+    }
+};
+
+// Cross-lane moves with DPP (GFX9 encodings: row_shr:n = 0x110 | n within a
+// row of 16 lanes, row_bcast:15 = 0x142, row_bcast:31 = 0x143, wave_shr:1 =
+// 0x138); lanes without a source read 0. These run in the ALUs, no LDS trip.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_v(float v) {
+    return __int_as_float(dpp_i<CTRL, ROWS>(__float_as_int(v)));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_v(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i<CTRL, ROWS>((int)(b & 0xffffffffll)), hi = dpp_i<CTRL, ROWS>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// One step of the segmented scan: (f1, v1) then (f2, v2) -> (f1 | f2, f2 ? v2 : v1 + v2).
+template <int CTRL, int ROWS, typename AT>
+__device__ __forceinline__ void seg_step(AT& v, int& f) {
+    const AT vu = dpp_v<CTRL, ROWS>(v);
+    const int fu = dpp_i<CTRL, ROWS>(f);
+    if (!f) v = vu + v;
+    f |= fu;
+}
+
+// Inclusive segmented scan over the 64 lanes: shifts of 1, 2, 4, 8 inside
+// each row of 16, then row 0's total into rows 1 and 3 (row_bcast:15) and
+// rows 0-1's total into rows 2 and 3 (row_bcast:31).
+template <typename AT>
+__device__ __forceinline__ void wave_seg_scan(AT& v, int& f) {
+    seg_step<0x111, 0xf>(v, f);
+    seg_step<0x112, 0xf>(v, f);
+    seg_step<0x114, 0xf>(v, f);
+    seg_step<0x118, 0xf>(v, f);
+    seg_step<0x142, 0xa>(v, f);
+    seg_step<0x143, 0xc>(v, f);
+}
+
+// Column-slab SpMV with row-end flags. A slab's compressed rows are never
+// empty, so its nonzeros form one segmented sequence: bit e of the slab's
+// flag words is set when nonzero e ends its row, and a tile descriptor holds
+// the slab row of the tile's first nonzero. One wavefront per tile of 64*EPL
+// nonzeros, lane l owning EPL consecutive ones (one vector load of columns
+// and one of values, then all EPL gathers in flight together). A lane sums
+// its products, cutting at its flags; one segmented wave scan (DPP) carries
+// the open sums between lanes; the lane's k-th row end is slab row
+// first + (flags before the lane) + k, whose partial goes to
+// P[row_base[s] + that row]. No LDS, no barrier: wavefronts run
+// independently, and a kernel this light keeps eight of them per SIMD (on
+// MI355X this beat a persistent, software-pipelined walk over the tiles,
+// whose second register set halves that occupancy). Workgroup b works on
+// slab b mod S (one XCD's x slice in its L2); its four wavefronts take four
+// consecutive tiles. The row open at a tile end goes to the tile's carry,
+// keyed by its P slot.
+template <typename VT, typename XT, typename AT, int EPL>
+__global__ __launch_bounds__(256) void xslab_kernel(SlabArgs sa, const int* __restrict__ sci,
+                                                    const VT* __restrict__ svals,
+                                                    const uint32_t* __restrict__ sflags,
+                                                    const int* __restrict__ tile_row, const XT* __restrict__ x,
+                                                    AT* __restrict__ P, AT* __restrict__ carry,
+                                                    int* __restrict__ carry_key) {
+    constexpr int SIG = 64 * EPL, NW = SIG / 32;  // nonzeros and flag words per tile
+    const int s = blockIdx.x % sa.S;
+    const int t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x / sa.S) * 4 + (int)(threadIdx.x >> 6));
+    if (t >= sa.tiles[s]) return;  // whole wavefront
+    const int lane = threadIdx.x & 63;
+    const long e0 = sa.nnz_base[s] + (long)t * SIG + lane * EPL;  // padded layout: the whole tile is allocated
+    int c[EPL];
+    VT v[EPL];
+    VecLoad<int, EPL>::run(sci + e0, c);
+    VecLoad<VT, EPL>::run(svals + e0, v);
+    const uint32_t* fw = sflags + (sa.nnz_base[s] + (long)t * SIG) / 32;
+    uint32_t w[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) w[i] = fw[i];  // the tile's flags: uniform, scalar loads
+    const int row0 = tile_row[sa.tile_base[s] + t];
+    XT xv[EPL];
+#pragma unroll
+    for (int j = 0; j < EPL; ++j) xv[j] = x[c[j]];
+
+    // this lane's flag bits and the number of row ends before it in the tile
+    constexpr int LPW = 32 / EPL;  // lanes per flag word
+    const int wi = lane / LPW, sh = (lane % LPW) * EPL;
+    uint32_t wsel = 0;
+    int rk = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        const int pc = __popc(w[i]);
+        rk += i < wi ? pc : 0;
+        tot += pc;
+        wsel = i == wi ? w[i] : wsel;
+    }
+    rk += __popc(wsel & ((1u << sh) - 1u));
+    const uint32_t f = (wsel >> sh) & ((1u << EPL) - 1u);
+    const int nreal = sa.nnz[s] - t * SIG - lane * EPL;  // real nonzeros of this lane (padding past the end)
+    AT p[EPL];
+#pragma unroll
+    for (int j = 0; j < EPL; ++j) p[j] = j < nreal ? Conv<AT>::f(v[j]) * Conv<AT>::f(xv[j]) : AT(0);
+    // the sum left open after the lane's last row end (all of it if none)
+    AT seg = AT(0);
+#pragma unroll
+    for (int j = 0; j < EPL; ++j) {
+        seg += p[j];
+        if ((f >> j) & 1u) seg = AT(0);
+    }
+    // segmented inclusive scan over lanes: a lane with a row end restarts the sum
+    AT sc = seg;
+    int fl = f != 0u;
+    wave_seg_scan(sc, fl);
+    AT* Ps = P + sa.row_base[s] + row0 + rk;
+    seg = dpp_v<0x138, 0xf>(sc);  // open sum entering the lane: the lane before's inclusive sum (0 for lane 0)
+    int k = 0;
+#pragma unroll
+    for (int j = 0; j < EPL; ++j) {
+        seg += p[j];
+        if ((f >> j) & 1u) {
+            Ps[k++] = seg;
+            seg = AT(0);
+        }
+    }
+    if (lane == 63) {
+        // the open sum at the tile end belongs to slab row row0 + tot, unless the tile's last nonzero ended a row
+        // or every row is closed (the slab's last tile)
+        const int orow = row0 + tot;
+        const bool open = orow < sa.rows[s] && !((f >> (EPL - 1)) & 1u);
+        carry[sa.tile_base[s] + t] = open ? sc : AT(0);
+        carry_key[sa.tile_base[s] + t] = open ? sa.row_base[s] + orow : -1;
     }
 }
 
@@ -303,6 +287,7 @@ __global__ __launch_bounds__(256) void xslab_fill_kernel(const int* __restrict__
                                                          const int* __restrict__ cnt, const int* __restrict__ pos,
                                                          const int* __restrict__ lrow,
                                                          const int* __restrict__ nnz_base,
+                                                         const int* __restrict__ pad_base,
                                                          const int* __restrict__ rpp, int* __restrict__ srp,
                                                          int* __restrict__ ploc, int* __restrict__ sci,
                                                          VT* __restrict__ sv) {
@@ -334,10 +319,30 @@ __global__ __launch_bounds__(256) void xslab_fill_kernel(const int* __restrict__
             if (lane == q) off += __popcll(m);
         }
         if (in) {
-            sci[o] = col;
-            sv[o] = v[i + lane];
+            const int op = o - nnz_base[s] + pad_base[s];  // into the slab's tile-aligned region
+            sci[op] = col;
+            sv[op] = v[i + lane];
         }
     }
+}
+
+// Tile padding of slab q: column lo (a column of the slab, so the gather stays in its x slice), value 0.
+template <typename VT>
+__global__ void xslab_pad_kernel(int* __restrict__ sci, VT* __restrict__ sv, long from, long to, int lo) {
+    const long i = from + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= to) return;
+    sci[i] = lo;
+    VT z;
+    __builtin_memset(&z, 0, sizeof(VT));
+    sv[i] = z;
+}
+
+// Row-end flags of one slab: bit (pad_base + rowPtr[q+1] - 1) for each of its rows q.
+__global__ void xslab_flags_kernel(const int* __restrict__ srp, int rows, long pad_base, uint32_t* __restrict__ flags) {
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= rows) return;
+    const long e = pad_base + srp[q + 1] - 1;
+    atomicOr(flags + e / 32, 1u << (e % 32));
 }
 
 static spmv_status_t exscan(const int* in, int* out, long n, hipStream_t s) {
@@ -355,34 +360,15 @@ static spmv_status_t exscan(const int* in, int* out, long n, hipStream_t s) {
 }
 
 template <typename VT, typename XT, typename AT>
-using XslabFn = void (*)(SlabArgs, int, const int*, const int*, const VT*, const XT*, const int*, const int*, AT*, AT*,
-                         int*);
+using XslabFn = void (*)(SlabArgs, const int*, const VT*, const uint32_t*, const int*, const XT*, AT*, AT*, int*);
 
 template <typename VT, typename XT, typename AT>
-static XslabFn<VT, XT, AT> xslab_fn(int ipt) {
-    switch (ipt) {
-        case 4: return xslab_kernel<VT, XT, AT, 4>;
-        case 16: return xslab_kernel<VT, XT, AT, 16>;
-        default: return xslab_kernel<VT, XT, AT, 8>;
-    }
+static XslabFn<VT, XT, AT> xslab_fn(int epl) {
+    return epl == 16 ? xslab_kernel<VT, XT, AT, 16> : epl == 8 ? xslab_kernel<VT, XT, AT, 8> : xslab_kernel<VT, XT, AT, 4>;
 }
 
-// Workgroups per slab for the persistent kernel: the slab's share of what
-// the device holds resident at once, and no more than the slab's tiles.
-template <typename VT, typename XT, typename AT>
-static spmv_status_t xslab_workgroups(muxi_plan* p) {
-    int cus = 0, per_cu = 0;
-    MUXI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
-    MUXI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, xslab_fn<VT, XT, AT>(p->param0 & 0xff), 256, 0));
-    const int S = p->slab.S;
-    int W = cus * (per_cu > 0 ? per_cu : 1) / S;
-    W = W > p->P ? p->P : W;  // p->P: the most tiles any slab has
-    p->slab_W = W > 0 ? W : 1;
-    return SPMV_SUCCESS;
-}
-
-spmv_status_t xslab_build(muxi_plan* p, int S, int ipt, spmv_type_t vt, hipStream_t s) {
-    const int R = p->A.R, C = p->A.C, N = p->A.N;
+spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStream_t s) {
+    const int R = p->A.R, C = p->A.C;
     if (S < 1 || S > kMaxSlabs) {
         set_error("xslab: slabs must be in [1, 16]");
         return SPMV_ERROR_INTERNAL;
@@ -391,6 +377,7 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int ipt, spmv_type_t vt, hipStrea
         set_error("xslab: rows * slabs exceeds INT32_MAX");
         return SPMV_ERROR_INVALID_MATRIX;
     }
+    const int SIG = 64 * epl;
     const long SR = (long)S * R;
     int *cnt, *flag, *pos, *lrow, *npair;
     MUXI_HIP(hipMallocAsync(&cnt, sizeof(int) * (SR + 1), s));
@@ -417,25 +404,55 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int ipt, spmv_type_t vt, hipStrea
     }
     MUXI_HIP(hipStreamSynchronize(s));
     const int total_rows = hrow[S];  // number of pairs
+    // slab q's nonzeros start on a tile boundary of the padded arrays
     SlabArgs& sa = p->slab;
     sa.S = S;
+    std::vector<int> hpad(S + 1);
+    long tp = 0, ntiles = 0;
+    int maxt = 0;
     for (int q = 0; q < S; ++q) {
+        const int nq = hpos[q + 1] - hpos[q];
         sa.rows[q] = hrow[q + 1] - hrow[q];
         sa.row_base[q] = hrow[q];
-        sa.nnz_base[q] = hpos[q];
+        sa.nnz[q] = nq;
+        sa.tiles[q] = (int)(((long)nq + SIG - 1) / SIG);
+        sa.tile_base[q] = (int)ntiles;
+        hpad[q] = (int)tp;
+        sa.nnz_base[q] = (int)tp;
+        tp += (long)sa.tiles[q] * SIG;
+        ntiles += sa.tiles[q];
+        maxt = sa.tiles[q] > maxt ? sa.tiles[q] : maxt;
     }
+    hpad[S] = (int)tp;
+    if (tp >= INT_MAX) {
+        for (int* t : {cnt, flag, pos, lrow, npair}) (void)hipFreeAsync(t, s);
+        set_error("xslab: padded nonzeros exceed INT32_MAX");
+        return SPMV_ERROR_INVALID_MATRIX;
+    }
+    p->T = (int)ntiles;
+    p->P = maxt;
+    const long ncarry = ntiles;  // one carry per tile
     int* d_nb = nullptr;
-    MUXI_HIP(hipMallocAsync(&d_nb, sizeof(int) * S, s));
+    MUXI_HIP(hipMallocAsync(&d_nb, sizeof(int) * 2 * S, s));
     MUXI_HIP(hipMemcpyAsync(d_nb, hpos.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
+    MUXI_HIP(hipMemcpyAsync(d_nb + S, hpad.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
     const size_t vs = vsize(vt);
-    const size_t ints = (size_t)total_rows * 2 + S + (size_t)N;
-    MUXI_HIP(hipMalloc(&p->d_slab, sizeof(int) * ints + vs * (size_t)(N > 0 ? N : 1) + 256));
+    const size_t nflag = (size_t)tp / 32;
+    // rowPtr (pairs + S), pair slots (pairs), then columns, values and flags at 256-byte boundaries (the kernel
+    // reads a lane's columns and values with one vector load each)
+    const size_t head = ((size_t)total_rows * 2 + S + 63) / 64 * 64;
+    const size_t ints = head + (size_t)tp;
+    MUXI_HIP(hipMalloc(&p->d_slab, sizeof(int) * ints + vs * (size_t)(tp > 0 ? tp : 1) + 4 * (nflag + 1) + 512));
     p->d_slab_rp = p->d_slab;
     p->d_slab_ploc = p->d_slab_rp + total_rows + S;
-    p->d_slab_ci = p->d_slab_ploc + total_rows;
-    uintptr_t va = (uintptr_t)(p->d_slab_ci + N);
+    p->d_slab_ci = p->d_slab + head;
+    uintptr_t va = (uintptr_t)(p->d_slab_ci + tp);
     va = (va + 255) & ~(uintptr_t)255;
     p->d_slab_v = (void*)va;
+    uintptr_t fa = va + vs * (size_t)(tp > 0 ? tp : 1);
+    fa = (fa + 255) & ~(uintptr_t)255;
+    p->d_slab_flags = (uint32_t*)fa;
+    MUXI_HIP(hipMemsetAsync(p->d_slab_flags, 0, 4 * (nflag + 1), s));
     std::vector<int> term(S);
     for (int q = 0; q < S; ++q) term[q] = hpos[q + 1] - hpos[q];
     for (int q = 0; q < S; ++q)
@@ -445,54 +462,40 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int ipt, spmv_type_t vt, hipStrea
         using T = decltype(tag);
         using VT = typename T::VT;
         hipLaunchKernelGGL(xslab_fill_kernel<VT>, dim3(grid_for((long)R * 64, 256)), dim3(256), 0, s, p->A.rp, p->A.ci,
-                           (const VT*)p->A.v, R, C, S, cnt, pos, lrow, d_nb, p->d_slab_rpp, p->d_slab_rp,
+                           (const VT*)p->A.v, R, C, S, cnt, pos, lrow, d_nb, d_nb + S, p->d_slab_rpp, p->d_slab_rp,
                            p->d_slab_ploc, p->d_slab_ci, (VT*)p->d_slab_v);
         MUXI_LAUNCH_CHECK();
+        for (int q = 0; q < S; ++q) {
+            const long from = (long)hpad[q] + sa.nnz[q], to = (long)hpad[q] + (long)sa.tiles[q] * SIG;
+            const int lo = (int)(((long)q * C + S - 1) / S);  // first column of slab q
+            if (to > from) {
+                hipLaunchKernelGGL(xslab_pad_kernel<VT>, dim3(grid_for(to - from, 256)), dim3(256), 0, s,
+                                   p->d_slab_ci, (VT*)p->d_slab_v, from, to, lo);
+                MUXI_LAUNCH_CHECK();
+            }
+        }
         return SPMV_SUCCESS;
     });
     if (st != SPMV_SUCCESS) return st;
-    const long CH = 256L * ipt;
-    long ncoord = 0, ntiles = 0;
-    for (int q = 0; q < S; ++q) {
-        const int nq = hpos[q + 1] - hpos[q];
-        sa.tiles[q] = sa.rows[q] > 0 ? (int)(((long)sa.rows[q] + nq + CH - 1) / CH) : 0;
-        sa.coord_base[q] = (int)ncoord;
-        sa.tile_base[q] = (int)ntiles;
-        ncoord += sa.tiles[q] + 1;
-        ntiles += sa.tiles[q];
-    }
-    p->T = (int)ntiles;
     const size_t acc = p->xtype == SPMV_TYPE_DOUBLE ? 8 : 4;
-    MUXI_HIP(hipMalloc(&p->d_pos, 2 * sizeof(int) * (size_t)ncoord));
-    p->d_row = p->d_pos + ncoord;
-    MUXI_HIP(hipMalloc(&p->d_carry, acc * (size_t)(ntiles + total_rows + 1)));  // carries, then P
-    MUXI_HIP(hipMalloc(&p->d_slab_ckey, sizeof(int) * (size_t)(ntiles + 1)));
+    MUXI_HIP(hipMalloc(&p->d_tile_row, 2 * sizeof(int) * (size_t)(ntiles + 1)));  // descriptors, then offsets
+    p->d_tile_off = p->d_tile_row + ntiles + 1;
+    MUXI_HIP(hipMalloc(&p->d_carry, acc * (size_t)(ncarry + total_rows + 1)));  // carries, then P
+    MUXI_HIP(hipMalloc(&p->d_slab_ckey, sizeof(int) * (size_t)(ncarry + 1)));
     for (int q = 0; q < S; ++q) {
         if (sa.tiles[q] == 0) continue;
-        const int nq = hpos[q + 1] - hpos[q];
-        st = launch_merge_partitions(sa.rows[q], nq, sa.tiles[q], p->d_pos + sa.coord_base[q], s);
-        if (st == SPMV_SUCCESS)
-            st = launch_merge_coords(p->d_slab_rp + sa.row_base[q] + q, sa.rows[q], nq, p->d_pos + sa.coord_base[q],
-                                     sa.tiles[q] + 1, p->d_row + sa.coord_base[q], nullptr, s);
+        const int* srp_q = p->d_slab_rp + sa.row_base[q] + q;
+        hipLaunchKernelGGL(xslab_flags_kernel, dim3(grid_for(sa.rows[q], 256)), dim3(256), 0, s, srp_q, sa.rows[q],
+                           (long)hpad[q], p->d_slab_flags);
+        MUXI_LAUNCH_CHECK();
+        st = launch_csr5_tiles(srp_q, sa.rows[q], sa.nnz[q], SIG, sa.tiles[q], p->d_tile_row + sa.tile_base[q],
+                               p->d_tile_off + sa.tile_base[q], s);
         if (st != SPMV_SUCCESS) return st;
     }
-    int maxt = 0;
-    for (int q = 0; q < S; ++q) maxt = sa.tiles[q] > maxt ? sa.tiles[q] : maxt;
-    p->P = maxt;
-    st = with_types(vt, p->xtype, false, [&](auto tag) -> spmv_status_t {
-        using T = decltype(tag);
-        return xslab_workgroups<typename T::VT, typename T::XT, typename T::AT>(p);
-    });
-    if (st != SPMV_SUCCESS) return st;
-    (void)hipFreeAsync(cnt, s);
-    (void)hipFreeAsync(flag, s);
-    (void)hipFreeAsync(pos, s);
-    (void)hipFreeAsync(lrow, s);
-    (void)hipFreeAsync(npair, s);
-    (void)hipFreeAsync(d_nb, s);
+    for (int* t : {cnt, flag, pos, lrow, npair, d_nb}) (void)hipFreeAsync(t, s);
     MUXI_HIP(hipStreamSynchronize(s));
-    p->scratch_bytes = sizeof(int) * ints + vs * (size_t)N + 2 * sizeof(int) * (size_t)ncoord +
-                       acc * (size_t)(ntiles + total_rows) + sizeof(int) * ((size_t)ntiles + R + 1);
+    p->scratch_bytes = sizeof(int) * ints + vs * (size_t)tp + 4 * nflag + 2 * sizeof(int) * (size_t)(ntiles + 1) +
+                       acc * (size_t)(ncarry + total_rows) + sizeof(int) * ((size_t)ncarry + R + 1);
     return SPMV_SUCCESS;
 }
 
@@ -501,18 +504,18 @@ spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT 
                            hipEvent_t ev_end) {
     if (p->A.R <= 0) return SPMV_SUCCESS;
     const SlabArgs& sa = p->slab;
+    const int ncarry = p->T;
     AT* carry = (AT*)p->d_carry;
-    AT* P = carry + p->T + 1;
+    AT* P = carry + ncarry + 1;
     if (p->P > 0) {
-        const dim3 grid((unsigned)((long)p->slab_W * sa.S));
+        const dim3 grid((unsigned)(((long)p->P + 3) / 4 * sa.S));  // four tiles per workgroup
         const XslabFn<VT, XT, AT> fn = xslab_fn<VT, XT, AT>(p->param0 & 0xff);
-        hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, sa, p->slab_W,
-                           p->d_slab_rp, p->d_slab_ci, (const VT*)p->d_slab_v, x, p->d_pos, p->d_row, P, carry,
-                           p->d_slab_ckey);
+        hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, sa, p->d_slab_ci, (const VT*)p->d_slab_v, p->d_slab_flags,
+                           p->d_tile_row, x, P, carry, p->d_slab_ckey);
         MUXI_LAUNCH_CHECK();
         if (ev_end) MUXI_HIP(hipEventRecord(ev_end, s));
-        hipLaunchKernelGGL(keyed_fixup_kernel<AT>, dim3(grid_for(p->T, 256)), dim3(256), 0, s, p->d_slab_ckey, p->T,
-                           carry, P);
+        hipLaunchKernelGGL(keyed_fixup_kernel<AT>, dim3(grid_for(ncarry, 256)), dim3(256), 0, s, p->d_slab_ckey,
+                           ncarry, carry, P);
         MUXI_LAUNCH_CHECK();
     } else if (ev_end) {
         MUXI_HIP(hipEventRecord(ev_end, s));

@@ -179,11 +179,11 @@ extern "C" spmv_status_t muxi_plan_create(const muxi_csr_t* Ain, spmv_type_t vt,
             break;
         }
         case MUXI_KERNEL_XSLAB: {
-            int S = param0 >> 8, ipt = param0 & 0xff;
+            int S = param0 >> 8, ipt = param0 & 0xff;  // ipt: nonzeros per lane
             if (S <= 0) S = xslab_default_slabs(Ain->numCols, xt == SPMV_TYPE_DOUBLE ? 8 : 4);
             if (ipt <= 0) ipt = 4;
             if (!(ipt == 4 || ipt == 8 || ipt == 16) || S > kMaxSlabs) {
-                set_error("xslab: param0 = (slabs << 8) | ipt with slabs in [1, 16] and ipt in {4, 8, 16}");
+                set_error("xslab: param0 = (slabs << 8) | nonzeros per lane, slabs in [1, 16], per lane 4, 8 or 16");
                 plan_free(p);
                 return SPMV_ERROR_INTERNAL;
             }

@@ -8,12 +8,12 @@ namespace muxi {
 constexpr int kMaxSlabs = 16;
 struct SlabArgs {
     int S;
-    int rows[kMaxSlabs];        // compressed rows of each slab
-    int tiles[kMaxSlabs];       // merge tiles of each slab
-    int row_base[kMaxSlabs];    // first row id of the slab; its rowPtr starts at row_base + s
-    int nnz_base[kMaxSlabs];    // first nonzero of the slab
-    int coord_base[kMaxSlabs];  // first of the slab's P+1 merge coordinates
-    int tile_base[kMaxSlabs];   // first global tile (carry slot) of the slab
+    int rows[kMaxSlabs];       // compressed rows of each slab (none is empty)
+    int tiles[kMaxSlabs];      // tiles of 64*EPL nonzeros of each slab
+    int row_base[kMaxSlabs];   // first row id of the slab; its rowPtr starts at row_base + s
+    int nnz_base[kMaxSlabs];   // first element of the slab in the padded (tile-aligned) arrays
+    int nnz[kMaxSlabs];        // nonzeros of the slab
+    int tile_base[kMaxSlabs];  // first global tile (carry slot, tile descriptor) of the slab
 };
 }  // namespace muxi
 
@@ -48,14 +48,14 @@ struct muxi_plan {
     size_t scratch_bytes = 0;
     // SpMM: right-hand sides (0 for a SpMV plan)
     int nrhs = 0;
-    // column slabs (xslab): one allocation holding rowPtr, pair slots, columns, values
+    // column slabs (xslab): one allocation holding rowPtr, pair slots, columns, values, row-end flags
     muxi::SlabArgs slab{};
-    int slab_W = 1;  // persistent workgroups per slab
     int* d_slab = nullptr;
     int* d_slab_rp = nullptr;
     int* d_slab_ploc = nullptr;  // P slot of each pair, pairs in row-major order
     int* d_slab_ci = nullptr;
     void* d_slab_v = nullptr;
+    uint32_t* d_slab_flags = nullptr;
     int* d_slab_rpp = nullptr;   // row-major pair offsets [R+1]
     int* d_slab_ckey = nullptr;  // carry keys (pair ids) per tile
 };
@@ -105,7 +105,7 @@ inline int xslab_default_slabs(long cols, int xbytes) {
     return S;
 }
 
-// column-slab plan and launcher (kernels_xslab.hip); param0 = (slabs << 8) | items per lane
+// column-slab plan and launcher (kernels_xslab.hip); param0 = (slabs << 8) | nonzeros per lane (4 or 8)
 spmv_status_t xslab_build(muxi_plan* p, int S, int ipt, spmv_type_t vt, hipStream_t s);
 template <typename VT, typename XT, typename AT>
 spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT beta, hipStream_t s,
