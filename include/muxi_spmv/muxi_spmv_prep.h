@@ -72,6 +72,15 @@ spmv_status_t muxi_permute_symmetric(int n, const int* rowPtr, const int* colIdx
 /* Half-bandwidth max |i - j| over the nonzeros. */
 int muxi_bandwidth(int numRows, const int* rowPtr, const int* colIdx);
 
+/* Row shards for G GPUs. No counterpart in the reference (it has no multi-GPU
+ * path); for callers that split one SpMV over several devices with x
+ * replicated. The merge path (numRows row ends + nnz nonzeros) is cut at the G-1
+ * diagonals d_g = floor(g * (numRows + nnz) / G), and each cut moves to the
+ * start of the row that holds it (the merge-path coordinate row at d_g), so
+ * shard g is rows [bounds[g], bounds[g+1]) with about (numRows + nnz) / G path
+ * items, and no row is split. bounds[0] = 0, bounds[G] = numRows. Host arrays. */
+spmv_status_t muxi_shard_rows(const int* rowPtr, int numRows, int G, int* bounds);
+
 /* ---- device application ---- */
 /* out[i] = map[in[i]] for i < n (column remap by an old -> new map). */
 spmv_status_t muxi_remap_indices_dev(const int* d_in, const int* d_map, int* d_out, int n, void* stream);

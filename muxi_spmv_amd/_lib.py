@@ -122,6 +122,7 @@ SIGNATURES: dict[str, tuple] = {
     "muxi_rcm": (_I, [_I, _VP, _VP, _VP]),
     "muxi_permute_symmetric": (_I, [_I, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP]),
     "muxi_bandwidth": (_I, [_I, _VP, _VP]),
+    "muxi_shard_rows": (_I, [_VP, _I, _I, _VP]),
     "muxi_remap_indices_dev": (_I, [_VP, _VP, _VP, _I, _VP]),
     "muxi_gather_dev": (_I, [_VP, _VP, _VP, _I, _I, _VP]),
     "muxi_scatter_dev": (_I, [_VP, _VP, _VP, _I, _I, _VP]),

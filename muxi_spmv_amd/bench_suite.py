@@ -13,6 +13,10 @@ batched    256 CSR matrices, rows log-uniform in [1e3, 2e5], mean nnz/row
            matrix (eager and hipGraph).
 spmm       Y = A X on the power-law matrix for k = 8 and 32 (row-major X).
 laplacian  4096^2 5-point Laplacian in FP64, CSR kernels and ELLPACK.
+shard      one GPU's share of an 8-GPU row split (muxi_shard_rows: equal merge-path
+           diagonals moved to row starts, x replicated) of the power-law and
+           Laplacian matrices: the heaviest shard run with the AUTO plan, next
+           to the whole matrix on one GPU.
 All timings are cold: each timed launch is preceded by a 512 MiB cache flush,
 and HIP events bracket the launch.
 """
@@ -201,13 +205,47 @@ def laplacian(args) -> list[dict]:
     return out
 
 
+def shard(args) -> list[dict]:
+    G_ = 8
+    tm = Timer()
+    out = []
+    for name, h, dt in (("powerlaw_2m_fp32", G.cfg_powerlaw_2m(), torch.float32),
+                        ("laplacian_4096_fp64", G.cfg_laplacian_4096(), torch.float64)):
+        vdt = np.float64 if dt == torch.float64 else np.float32
+        esz = 8 if dt == torch.float64 else 4
+        b = prep.shard_rows(h.rowPtr, G_)
+        items = np.diff(b) + np.diff(h.rowPtr[b].astype(np.int64))
+        g = int(np.argmax(items))  # the shard that sets an 8-GPU step's time
+        r0, r1 = int(b[g]), int(b[g + 1])
+        n0, n1 = int(h.rowPtr[r0]), int(h.rowPtr[r1])
+        rp = (h.rowPtr[r0:r1 + 1] - n0).astype(np.int32)
+        x = torch.from_numpy(G.uniform_vector(h.numCols, 7, vdt)).cuda()
+        res = {"workload": name, "gpus": G_, "shard": g, "shard_rows": r1 - r0, "shard_nnz": n1 - n0,
+               "items_max_over_mean": round(float(items.max() / items.mean()), 4)}
+        for part, (A, R) in (("shard", (api.CSRMatrix(rp, h.colIdx[n0:n1], torch.from_numpy(h.values[n0:n1]).to(dt),
+                                                     h.numCols), r1 - r0)),
+                             ("whole", (api.CSRMatrix(h.rowPtr, h.colIdx, torch.from_numpy(h.values).to(dt),
+                                                      h.numCols), h.numRows))):
+            p = api.Plan(A, "auto", 0, xtype=dt)
+            y = torch.empty(R, dtype=dt, device="cuda")
+            ms = tm.time(lambda: p.exec(x, y))
+            nb = algorithmic_bytes(R, h.numCols, A.nnz, esz, esz)
+            info = p.info()
+            res[part] = {"kernel": f"{info.kernel}:{info.param0}", "ms": round(ms, 4), "GBps": round(nb / ms / 1e6, 1)}
+            p.destroy()
+        res["speedup_8_gpus_if_x_replicated"] = round(res["whole"]["ms"] / res["shard"]["ms"], 2)
+        log(json.dumps(res))
+        out.append(res)
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["precision", "batched", "spmm", "laplacian"])
+    ap.add_argument("what", choices=["precision", "batched", "spmm", "laplacian", "shard"])
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     torch.cuda.set_device(0)
-    res = {"precision": precision, "batched": batched, "spmm": spmm, "laplacian": laplacian}[a.what](a)
+    res = {"precision": precision, "batched": batched, "spmm": spmm, "laplacian": laplacian, "shard": shard}[a.what](a)
     txt = "\n".join(json.dumps(r) for r in res)
     print(txt, flush=True)
     if a.out:

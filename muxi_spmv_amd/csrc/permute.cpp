@@ -211,6 +211,28 @@ extern "C" spmv_status_t muxi_reorder_rows_by_length_f32(int R, const int* rp, c
     return SPMV_SUCCESS;
 }
 
+extern "C" spmv_status_t muxi_shard_rows(const int* rp, int R, int G, int* bounds) {
+    if (R < 0 || G < 1 || !bounds || (R > 0 && !rp)) return SPMV_ERROR_INVALID_MATRIX;
+    const long N = R > 0 ? rp[R] : 0, L = (long)R + N;
+    bounds[0] = 0;
+    for (int g = 1; g < G; ++g) {
+        const long d = (long)g * L / G;
+        // merge-path coordinate row at diagonal d (the search the merge plan kernels use): the number of row
+        // ends among the first d path items, the smallest r in [max(0, d-N), min(d, R)] with rp[r+1] >= d - r
+        long lo = d - N > 0 ? d - N : 0, hi = d < R ? d : R;
+        while (lo < hi) {
+            const long mid = (lo + hi) >> 1;
+            if ((long)rp[mid + 1] <= d - mid - 1)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        bounds[g] = (int)lo;
+    }
+    bounds[G] = R;
+    return SPMV_SUCCESS;
+}
+
 extern "C" int muxi_bandwidth(int R, const int* rp, const int* ci) {
     int bw = 0;
     for (int r = 0; r < R; ++r)

@@ -105,6 +105,14 @@ def permute_symmetric(rowPtr, colIdx, values, perm):
     return nrp, nci, nv
 
 
+def shard_rows(rowPtr, G: int) -> np.ndarray:
+    """Row bounds[G+1] of G shards with about equal (rows + nonzeros), no row split (muxi_shard_rows)."""
+    rp = _i32(rowPtr)
+    b = np.empty(G + 1, np.int32)
+    check(load().muxi_shard_rows(_p(rp), rp.size - 1, G, _p(b)), "shard_rows")
+    return b
+
+
 def bandwidth(rowPtr, colIdx) -> int:
     rp, ci = _i32(rowPtr), _i32(colIdx)
     return int(load().muxi_bandwidth(rp.size - 1, _p(rp), _p(ci)))

@@ -101,3 +101,33 @@ def test_rcm_on_path_graph_gives_band_one():
     perm = prep.rcm(rp, ci)
     nrp, nci, _ = prep.permute_symmetric(rp, ci, np.ones(ci.size, np.float32), perm)
     assert prep.bandwidth(nrp, nci) == 1
+
+
+@pytest.mark.parametrize("G", [1, 2, 3, 8, 64])
+def test_shard_rows_are_merge_path_cuts(G):
+    """Shard bounds are the merge-path coordinate rows of the G equal diagonals (the oracle's search, the
+    reference's merge_path_search shape), cover every row once and carry about (R + N) / G path items each."""
+    import oracle
+    from muxi_spmv_amd import generators as Gen
+    for h in (Gen.powerlaw_rows(5000, 5000, seed=2, kmax=800), Gen.uniform_rows(3000, 3000, 0, 9, seed=3)):
+        rp = h.rowPtr
+        R, N = rp.size - 1, int(rp[-1])
+        b = prep.shard_rows(rp, G)
+        L = R + N
+        d = np.array([g * L // G for g in range(G + 1)], np.int32)
+        rows, _ = oracle.merge_path_coords(rp, N, d)
+        exp = rows.copy()
+        exp[0], exp[G] = 0, R
+        assert np.array_equal(b, exp)
+        assert b[0] == 0 and b[-1] == R and np.all(np.diff(b) >= 0)
+        # moving a cut to its row start shifts a shard by at most one row (its nonzeros and its end), plus rounding
+        items = np.diff(b) + np.diff(rp[b])
+        assert np.all(np.abs(items - L / G) <= np.diff(rp).max() + 2)
+
+
+def test_shard_rows_edge_cases():
+    rp = np.array([0, 0, 0, 5, 5], np.int32)  # empty rows around one row
+    assert prep.shard_rows(rp, 1).tolist() == [0, 4]
+    b = prep.shard_rows(rp, 4)
+    assert b[0] == 0 and b[-1] == 4 and np.all(np.diff(b) >= 0)
+    assert prep.shard_rows(np.array([0], np.int32), 3).tolist() == [0, 0, 0, 0]
