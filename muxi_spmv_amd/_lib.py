@@ -122,6 +122,12 @@ SIGNATURES: dict[str, tuple] = {
     "muxi_rcm": (_I, [_I, _VP, _VP, _VP]),
     "muxi_permute_symmetric": (_I, [_I, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP]),
     "muxi_bandwidth": (_I, [_I, _VP, _VP]),
+    "muxi_mtx_read_header": (_I, [C.c_char_p, C.c_void_p]),
+    "muxi_mtx_read_csr_f32": (_I, [C.c_char_p, _I, _P_CSR]),
+    "muxi_mtx_read_csr_f64": (_I, [C.c_char_p, _I, _P_CSR]),
+    "muxi_mtx_write_csr_f32": (_I, [C.c_char_p, _P_CSR, _I]),
+    "muxi_mtx_write_csr_f64": (_I, [C.c_char_p, _P_CSR, _I]),
+    "muxi_csr_free_host": (_I, [_P_CSR]),
     "muxi_shard_rows": (_I, [_VP, _I, _I, _VP]),
     "muxi_remap_indices_dev": (_I, [_VP, _VP, _VP, _I, _VP]),
     "muxi_gather_dev": (_I, [_VP, _VP, _VP, _I, _I, _VP]),
