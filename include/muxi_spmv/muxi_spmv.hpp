@@ -9,6 +9,11 @@
 //   spmv_merge_based<T>, spmv_merge_based_v2<T>, spmv_sparse_optimized<T>,
 //   csr_to_ellpack<T>, spmv_ellpack<T>, spmv_exec<T>, spmv_exec_transpose<T>,
 //   handle functions (src/api/spmv_api.h:33-92)
+//   COOMatrix<T>, BSRMatrix<T, B>, coo_to_csr, csr_to_coo, bsr_to_csr,
+//   convertCSRToCSR5 (src/formats/sparse_formats.h:113-267, 389-405),
+//   spmv_csr_transpose (src/spmv/csr/spmv_csr.cuh:1107-1114)
+//   io::MTXHeader, io::readMTXHeader, io::readMTXFile, io::writeMTXFile,
+//   io::readMTXToCOO, io::writeCOOToMTX (src/generators/mtx_io.h)
 // Instantiated for float and double, as the reference is.
 #ifndef MUXI_SPMV_HPP_
 #define MUXI_SPMV_HPP_
@@ -19,6 +24,8 @@
 #include <type_traits>
 
 #include "muxi_spmv.h"
+#include "muxi_spmv_formats.h"
+#include "muxi_spmv_io.h"
 #include "muxi_spmv_prep.h"
 
 namespace muxi_spmv {
@@ -217,6 +224,254 @@ inline spmv_status_t spmv_exec_transpose(spmv_handle_t* h, spmv_format_t format,
     if constexpr (std::is_same<T, float>::value) return ::muxi_spmv_exec_transpose_f32(h, format, m, x, y, alpha, beta);
     else return ::muxi_spmv_exec_transpose_f64(h, format, m, x, y, alpha, beta);
 }
+
+// ---- COO / BSR and the conversions ------------------------------------------
+// The reference declares coo_to_csr / csr_to_coo / bsr_to_csr without a status
+// result; here they convert the DEVICE arrays on the GPU (stream 0) and return
+// a matrix owning new device arrays. On failure the result is empty
+// (numRows = -1) and muxi_spmv_last_error() says why.
+template <typename FloatType>
+struct COOMatrix {
+    int numRows = 0, numCols = 0, nnz = 0;
+    int* rowIdx = nullptr;
+    int* colIdx = nullptr;
+    FloatType* values = nullptr;
+    int* d_rowIdx = nullptr;
+    int* d_colIdx = nullptr;
+    FloatType* d_values = nullptr;
+    bool ownsHostMemory = false;
+    bool ownsDeviceMemory = false;
+
+    COOMatrix() = default;
+    COOMatrix(const COOMatrix&) = delete;
+    COOMatrix& operator=(const COOMatrix&) = delete;
+    COOMatrix(COOMatrix&& o) noexcept { *this = static_cast<COOMatrix&&>(o); }
+    COOMatrix& operator=(COOMatrix&& o) noexcept {
+        if (this != &o) {
+            release();
+            numRows = o.numRows; numCols = o.numCols; nnz = o.nnz;
+            rowIdx = o.rowIdx; colIdx = o.colIdx; values = o.values;
+            d_rowIdx = o.d_rowIdx; d_colIdx = o.d_colIdx; d_values = o.d_values;
+            ownsHostMemory = o.ownsHostMemory; ownsDeviceMemory = o.ownsDeviceMemory;
+            o.ownsHostMemory = o.ownsDeviceMemory = false;
+        }
+        return *this;
+    }
+    ~COOMatrix() { release(); }
+    void release() {
+        if (ownsHostMemory) { std::free(rowIdx); std::free(colIdx); std::free(values); }
+        if (ownsDeviceMemory) { (void)hipFree(d_rowIdx); (void)hipFree(d_colIdx); (void)hipFree(d_values); }
+        ownsHostMemory = ownsDeviceMemory = false;
+    }
+    void allocateHost(int rows, int cols, int nonzeros) {
+        numRows = rows; numCols = cols; nnz = nonzeros;
+        rowIdx = (int*)std::malloc((size_t)nnz * sizeof(int));
+        colIdx = (int*)std::malloc((size_t)nnz * sizeof(int));
+        values = (FloatType*)std::malloc((size_t)nnz * sizeof(FloatType));
+        ownsHostMemory = true;
+    }
+    void allocateDevice() {
+        (void)hipMalloc(&d_rowIdx, (size_t)nnz * sizeof(int));
+        (void)hipMalloc(&d_colIdx, (size_t)nnz * sizeof(int));
+        (void)hipMalloc(&d_values, (size_t)nnz * sizeof(FloatType));
+        ownsDeviceMemory = true;
+    }
+    ::muxi_coo_t abi() const {
+        ::muxi_coo_t c;
+        c.numRows = numRows; c.numCols = numCols; c.nnz = nnz;
+        c.rowIdx = rowIdx; c.colIdx = colIdx; c.values = values;
+        c.d_rowIdx = d_rowIdx; c.d_colIdx = d_colIdx; c.d_values = d_values;
+        c.ownsHostMemory = false; c.ownsDeviceMemory = false;
+        return c;
+    }
+};
+
+template <typename FloatType, int BlockSize>
+struct BSRMatrix {
+    int numRows = 0, numCols = 0, numBlockRows = 0, numBlockCols = 0, numBlocks = 0;
+    int blockSize = BlockSize;
+    int* rowPtr = nullptr;
+    int* colIdx = nullptr;
+    FloatType* values = nullptr;
+    int* d_rowPtr = nullptr;
+    int* d_colIdx = nullptr;
+    FloatType* d_values = nullptr;
+    bool ownsHostMemory = false;
+    bool ownsDeviceMemory = false;
+
+    BSRMatrix() = default;
+    BSRMatrix(const BSRMatrix&) = delete;
+    BSRMatrix& operator=(const BSRMatrix&) = delete;
+    BSRMatrix(BSRMatrix&& o) noexcept { *this = static_cast<BSRMatrix&&>(o); }
+    BSRMatrix& operator=(BSRMatrix&& o) noexcept {
+        if (this != &o) {
+            release();
+            numRows = o.numRows; numCols = o.numCols; numBlockRows = o.numBlockRows;
+            numBlockCols = o.numBlockCols; numBlocks = o.numBlocks; blockSize = o.blockSize;
+            rowPtr = o.rowPtr; colIdx = o.colIdx; values = o.values;
+            d_rowPtr = o.d_rowPtr; d_colIdx = o.d_colIdx; d_values = o.d_values;
+            ownsHostMemory = o.ownsHostMemory; ownsDeviceMemory = o.ownsDeviceMemory;
+            o.ownsHostMemory = o.ownsDeviceMemory = false;
+        }
+        return *this;
+    }
+    ~BSRMatrix() { release(); }
+    void release() {
+        if (ownsHostMemory) { std::free(rowPtr); std::free(colIdx); std::free(values); }
+        if (ownsDeviceMemory) { (void)hipFree(d_rowPtr); (void)hipFree(d_colIdx); (void)hipFree(d_values); }
+        ownsHostMemory = ownsDeviceMemory = false;
+    }
+    ::muxi_bsr_t abi() const {
+        ::muxi_bsr_t c;
+        c.numRows = numRows; c.numCols = numCols; c.numBlockRows = numBlockRows; c.numBlockCols = numBlockCols;
+        c.numBlocks = numBlocks; c.blockSize = blockSize;
+        c.rowPtr = rowPtr; c.colIdx = colIdx; c.values = values;
+        c.d_rowPtr = d_rowPtr; c.d_colIdx = d_colIdx; c.d_values = d_values;
+        c.ownsHostMemory = false; c.ownsDeviceMemory = false;
+        return c;
+    }
+};
+
+namespace detail {
+template <typename T> constexpr spmv_type_t vtype() { return std::is_same<T, float>::value ? SPMV_TYPE_FLOAT : SPMV_TYPE_DOUBLE; }
+template <typename T>
+inline CSRMatrix<T> csr_from(const ::muxi_csr_t& c, spmv_status_t st) {
+    CSRMatrix<T> m;
+    if (st != SPMV_SUCCESS) { m.numRows = -1; return m; }
+    m.numRows = c.numRows; m.numCols = c.numCols; m.nnz = c.nnz;
+    m.d_rowPtr = c.d_rowPtr; m.d_colIdx = c.d_colIdx; m.d_values = (T*)c.d_values;
+    m.ownsDeviceMemory = true;
+    return m;
+}
+}  // namespace detail
+
+template <typename T>
+inline COOMatrix<T> csr_to_coo(const CSRMatrix<T>& csr) {
+    ::muxi_csr_t c = csr.abi();
+    ::muxi_coo_t o{};
+    COOMatrix<T> m;
+    if (::muxi_csr_to_coo(&c, detail::vtype<T>(), &o, nullptr) != SPMV_SUCCESS) { m.numRows = -1; return m; }
+    m.numRows = o.numRows; m.numCols = o.numCols; m.nnz = o.nnz;
+    m.d_rowIdx = o.d_rowIdx; m.d_colIdx = o.d_colIdx; m.d_values = (T*)o.d_values;
+    m.ownsDeviceMemory = true;
+    return m;
+}
+
+template <typename T>
+inline CSRMatrix<T> coo_to_csr(const COOMatrix<T>& coo) {
+    ::muxi_coo_t c = coo.abi();
+    ::muxi_csr_t o{};
+    spmv_status_t st = ::muxi_coo_to_csr(&c, detail::vtype<T>(), &o, nullptr);
+    return detail::csr_from<T>(o, st);
+}
+
+template <typename T, int B>
+inline CSRMatrix<T> bsr_to_csr(const BSRMatrix<T, B>& bsr) {
+    ::muxi_bsr_t c = bsr.abi();
+    ::muxi_csr_t o{};
+    spmv_status_t st = ::muxi_bsr_to_csr(&c, detail::vtype<T>(), &o, nullptr);
+    return detail::csr_from<T>(o, st);
+}
+
+// The inverse of bsr_to_csr (no counterpart in the reference).
+template <typename T, int B>
+inline spmv_status_t csr_to_bsr(const CSRMatrix<T>& csr, BSRMatrix<T, B>& bsr, hipStream_t stream = 0) {
+    ::muxi_csr_t c = csr.abi();
+    ::muxi_bsr_t o{};
+    spmv_status_t st = ::muxi_csr_to_bsr(&c, detail::vtype<T>(), B, &o, stream);
+    if (st != SPMV_SUCCESS) return st;
+    bsr.release();
+    bsr.numRows = o.numRows; bsr.numCols = o.numCols; bsr.numBlockRows = o.numBlockRows;
+    bsr.numBlockCols = o.numBlockCols; bsr.numBlocks = o.numBlocks; bsr.blockSize = o.blockSize;
+    bsr.d_rowPtr = o.d_rowPtr; bsr.d_colIdx = o.d_colIdx; bsr.d_values = (T*)o.d_values;
+    bsr.ownsDeviceMemory = true;
+    return SPMV_SUCCESS;
+}
+
+// A^T as a CSR matrix on the device (deterministic: a stable sort by column).
+template <typename T>
+inline spmv_status_t csr_transpose(const CSRMatrix<T>& A, CSRMatrix<T>& AT, hipStream_t stream = 0) {
+    ::muxi_csr_t c = A.abi();
+    ::muxi_csr_t o{};
+    spmv_status_t st = ::muxi_csr_transpose(&c, detail::vtype<T>(), &o, stream);
+    if (st == SPMV_SUCCESS) AT = detail::csr_from<T>(o, st);
+    return st;
+}
+
+template <typename T>
+inline spmv_status_t convertCSRToCSR5(const CSRMatrix<T>& csr, CSR5Matrix<T>& csr5, int sigma = 256,
+                                      hipStream_t stream = 0) {
+    return csr5_preprocess<T>(csr, csr5, sigma, stream);
+}
+
+// y = alpha*A^T*x + beta*y without atomics: A^T is built, a plan chosen by the
+// library's selector runs it, both are released. A caller that repeats the
+// product keeps csr_transpose's result and a muxi_plan instead.
+template <typename T>
+inline spmv_status_t spmv_csr_transpose(const CSRMatrix<T>& A, const T* x, T* y, T alpha, T beta,
+                                        const spmv_opts_t& opts) {
+    hipStream_t s = (hipStream_t)opts.stream;
+    CSRMatrix<T> AT;
+    spmv_status_t st = csr_transpose<T>(A, AT, s);
+    if (st != SPMV_SUCCESS) return st;
+    ::muxi_csr_t c = AT.abi();
+    ::muxi_plan* p = nullptr;
+    st = ::muxi_plan_create(&c, detail::vtype<T>(), detail::vtype<T>(), MUXI_KERNEL_AUTO, 0, 0, s, &p);
+    if (st != SPMV_SUCCESS) return st;
+    st = ::muxi_plan_exec(p, x, y, (double)alpha, (double)beta, 1, s);
+    if (st == SPMV_SUCCESS) st = hipStreamSynchronize(s) == hipSuccess ? SPMV_SUCCESS : SPMV_ERROR_DEVICE;
+    ::muxi_plan_destroy(p);
+    return st;
+}
+
+// ---- Matrix Market files (host arrays) -----------------------------------------
+namespace io {
+using MTXHeader = ::muxi_mtx_header_t;
+
+inline spmv_status_t readMTXHeader(const char* filename, MTXHeader& header) {
+    return ::muxi_mtx_read_header(filename, &header);
+}
+
+template <typename T>
+inline spmv_status_t readMTXFile(const char* filename, CSRMatrix<T>& matrix, bool expandSymmetric = true) {
+    ::muxi_csr_t c{};
+    spmv_status_t st = std::is_same<T, float>::value ? ::muxi_mtx_read_csr_f32(filename, expandSymmetric, &c)
+                                                     : ::muxi_mtx_read_csr_f64(filename, expandSymmetric, &c);
+    if (st != SPMV_SUCCESS) return st;
+    matrix.release();
+    matrix.numRows = c.numRows; matrix.numCols = c.numCols; matrix.nnz = c.nnz;
+    matrix.rowPtr = c.rowPtr; matrix.colIdx = c.colIdx; matrix.values = (T*)c.values;
+    matrix.ownsHostMemory = true;  // malloc'd by the library, released with std::free
+    return SPMV_SUCCESS;
+}
+
+template <typename T>
+inline spmv_status_t writeMTXFile(const char* filename, const CSRMatrix<T>& matrix, bool writeSymmetric = false) {
+    ::muxi_csr_t c = matrix.abi();
+    return std::is_same<T, float>::value ? ::muxi_mtx_write_csr_f32(filename, &c, writeSymmetric)
+                                         : ::muxi_mtx_write_csr_f64(filename, &c, writeSymmetric);
+}
+
+template <typename T>
+inline spmv_status_t readMTXToCOO(const char* filename, COOMatrix<T>& matrix) {
+    ::muxi_coo_t c{};
+    spmv_status_t st = std::is_same<T, float>::value ? ::muxi_mtx_read_coo_f32(filename, &c)
+                                                     : ::muxi_mtx_read_coo_f64(filename, &c);
+    if (st != SPMV_SUCCESS) return st;
+    matrix.release();
+    matrix.numRows = c.numRows; matrix.numCols = c.numCols; matrix.nnz = c.nnz;
+    matrix.rowIdx = c.rowIdx; matrix.colIdx = c.colIdx; matrix.values = (T*)c.values;
+    matrix.ownsHostMemory = true;
+    return SPMV_SUCCESS;
+}
+
+template <typename T>
+inline spmv_status_t writeCOOToMTX(const char* filename, const COOMatrix<T>& matrix) {
+    ::muxi_coo_t c = matrix.abi();
+    return std::is_same<T, float>::value ? ::muxi_mtx_write_coo_f32(filename, &c)
+                                         : ::muxi_mtx_write_coo_f64(filename, &c);
+}
+}  // namespace io
 
 }  // namespace muxi_spmv
 

@@ -25,6 +25,7 @@
 #define MUXI_SPMV_IO_H_
 
 #include "muxi_spmv.h"
+#include "muxi_spmv_formats.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -55,8 +56,21 @@ spmv_status_t muxi_mtx_read_csr_f64(const char* filename, int expandSymmetric, m
 spmv_status_t muxi_mtx_write_csr_f32(const char* filename, const muxi_csr_t* A, int writeSymmetric);
 spmv_status_t muxi_mtx_write_csr_f64(const char* filename, const muxi_csr_t* A, int writeSymmetric);
 
+/* readMTXToCOO<float/double> (src/generators/mtx_io.cu:273-299): the entries
+ * as stored in the file (symmetric files are not mirrored), in (row, column)
+ * order, as host arrays (ownsHostMemory = true); free with muxi_coo_free_host. */
+spmv_status_t muxi_mtx_read_coo_f32(const char* filename, muxi_coo_t* out);
+spmv_status_t muxi_mtx_read_coo_f64(const char* filename, muxi_coo_t* out);
+
+/* writeCOOToMTX<float/double> (src/generators/mtx_io.cu:301-324): a "general"
+ * file with the host entries in their order; an entry outside the matrix is
+ * SPMV_ERROR_INVALID_MATRIX. */
+spmv_status_t muxi_mtx_write_coo_f32(const char* filename, const muxi_coo_t* A);
+spmv_status_t muxi_mtx_write_coo_f64(const char* filename, const muxi_coo_t* A);
+
 /* Frees host arrays that muxi_mtx_read_* allocated and clears the pointers. */
 spmv_status_t muxi_csr_free_host(muxi_csr_t* A);
+spmv_status_t muxi_coo_free_host(muxi_coo_t* A);
 
 #ifdef __cplusplus
 }

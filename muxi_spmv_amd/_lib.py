@@ -57,6 +57,28 @@ class muxi_ell_t(C.Structure):
                 ("d_colIdx", C.c_void_p), ("d_values", C.c_void_p)]
 
 
+class muxi_coo_t(C.Structure):
+    _fields_ = [("numRows", C.c_int), ("numCols", C.c_int), ("nnz", C.c_int),
+                ("rowIdx", C.c_void_p), ("colIdx", C.c_void_p), ("values", C.c_void_p),
+                ("d_rowIdx", C.c_void_p), ("d_colIdx", C.c_void_p), ("d_values", C.c_void_p),
+                ("ownsHostMemory", C.c_bool), ("ownsDeviceMemory", C.c_bool)]
+
+
+class muxi_csr2_t(C.Structure):
+    _fields_ = [("numRows", C.c_int), ("numCols", C.c_int), ("nnz", C.c_int),
+                ("rowPtr", C.c_void_p), ("values", C.c_void_p), ("metadata", C.c_void_p),
+                ("d_rowPtr", C.c_void_p), ("d_values", C.c_void_p), ("d_metadata", C.c_void_p),
+                ("ownsHostMemory", C.c_bool), ("ownsDeviceMemory", C.c_bool)]
+
+
+class muxi_bsr_t(C.Structure):
+    _fields_ = [("numRows", C.c_int), ("numCols", C.c_int), ("numBlockRows", C.c_int), ("numBlockCols", C.c_int),
+                ("numBlocks", C.c_int), ("blockSize", C.c_int),
+                ("rowPtr", C.c_void_p), ("colIdx", C.c_void_p), ("values", C.c_void_p),
+                ("d_rowPtr", C.c_void_p), ("d_colIdx", C.c_void_p), ("d_values", C.c_void_p),
+                ("ownsHostMemory", C.c_bool), ("ownsDeviceMemory", C.c_bool)]
+
+
 class muxi_spmv_handle_t(C.Structure):
     _fields_ = [("deviceId", C.c_int), ("stream", C.c_void_p), ("opts", spmv_opts_t), ("warpSize", C.c_int),
                 ("smCount", C.c_int), ("maxThreadsPerBlock", C.c_int), ("sharedMemPerSM", C.c_size_t)]
@@ -65,6 +87,7 @@ class muxi_spmv_handle_t(C.Structure):
 _VP, _I, _D, _F, _Z = C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_size_t
 _P_CSR, _P_CSR5, _P_ELL = C.POINTER(muxi_csr_t), C.POINTER(muxi_csr5_t), C.POINTER(muxi_ell_t)
 _P_OPTS = C.POINTER(spmv_opts_t)
+_P_COO, _P_BSR = C.POINTER(muxi_coo_t), C.POINTER(muxi_bsr_t)
 _P_H = C.POINTER(muxi_spmv_handle_t)
 
 # name -> (restype, argtypes). Every symbol declared in include/muxi_spmv/*.h appears here.
@@ -128,6 +151,19 @@ SIGNATURES: dict[str, tuple] = {
     "muxi_mtx_write_csr_f32": (_I, [C.c_char_p, _P_CSR, _I]),
     "muxi_mtx_write_csr_f64": (_I, [C.c_char_p, _P_CSR, _I]),
     "muxi_csr_free_host": (_I, [_P_CSR]),
+    "muxi_mtx_read_coo_f32": (_I, [C.c_char_p, _P_COO]),
+    "muxi_mtx_read_coo_f64": (_I, [C.c_char_p, _P_COO]),
+    "muxi_mtx_write_coo_f32": (_I, [C.c_char_p, _P_COO]),
+    "muxi_mtx_write_coo_f64": (_I, [C.c_char_p, _P_COO]),
+    "muxi_coo_free_host": (_I, [_P_COO]),
+    "muxi_csr_to_coo": (_I, [_P_CSR, _I, _P_COO, _VP]),
+    "muxi_coo_to_csr": (_I, [_P_COO, _I, _P_CSR, _VP]),
+    "muxi_csr_to_bsr": (_I, [_P_CSR, _I, _I, _P_BSR, _VP]),
+    "muxi_bsr_to_csr": (_I, [_P_BSR, _I, _P_CSR, _VP]),
+    "muxi_csr_transpose": (_I, [_P_CSR, _I, _P_CSR, _VP]),
+    "muxi_csr_free_device": (_I, [_P_CSR]),
+    "muxi_coo_free_device": (_I, [_P_COO]),
+    "muxi_bsr_free_device": (_I, [_P_BSR]),
     "muxi_shard_rows": (_I, [_VP, _I, _I, _VP]),
     "muxi_remap_indices_dev": (_I, [_VP, _VP, _VP, _I, _VP]),
     "muxi_gather_dev": (_I, [_VP, _VP, _VP, _I, _I, _VP]),

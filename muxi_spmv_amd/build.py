@@ -108,7 +108,7 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
     return LIB
 
 
-CPP_TESTS = [ROOT / "tests" / "cpp" / "test_spmv_known_answer.cpp"]
+CPP_TESTS = [ROOT / "tests" / "cpp" / "test_spmv_known_answer.cpp", ROOT / "tests" / "cpp" / "test_formats_io.cpp"]
 PROBES = [PKG / "probe" / "gather_probe.hip"]
 
 

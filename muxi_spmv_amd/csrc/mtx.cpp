@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "muxi_spmv/muxi_spmv_formats.h"
 #include "muxi_spmv/muxi_spmv_io.h"
 
 using namespace muxi;
@@ -195,6 +196,68 @@ spmv_status_t write_csr(const char* name, const muxi_csr_t* A, int symmetric) {
     return SPMV_SUCCESS;
 }
 
+// readMTXToCOO: the file's entries as stored (no symmetric mirroring), in
+// (row, column) order, rows expanded from the CSR read.
+template <typename T>
+spmv_status_t read_coo(const char* name, muxi_coo_t* out) {
+    if (!out) return SPMV_ERROR_INVALID_MATRIX;
+    muxi_csr_t c;
+    spmv_status_t st = read_csr<T>(name, 0, &c);
+    if (st != SPMV_SUCCESS) return st;
+    int* ri = (int*)std::malloc(sizeof(int) * (c.nnz ? (size_t)c.nnz : 1));
+    if (!ri) {
+        std::free(c.rowPtr);
+        std::free(c.colIdx);
+        std::free(c.values);
+        return SPMV_ERROR_MEMORY_ALLOC;
+    }
+    for (int r = 0; r < c.numRows; ++r)
+        for (int k = c.rowPtr[r]; k < c.rowPtr[r + 1]; ++k) ri[k] = r;
+    std::free(c.rowPtr);
+    std::memset(out, 0, sizeof(*out));
+    out->numRows = c.numRows;
+    out->numCols = c.numCols;
+    out->nnz = c.nnz;
+    out->rowIdx = ri;
+    out->colIdx = c.colIdx;
+    out->values = c.values;
+    out->ownsHostMemory = true;
+    return SPMV_SUCCESS;
+}
+
+// writeCOOToMTX: a "general" file with the entries in the given order.
+template <typename T>
+spmv_status_t write_coo(const char* name, const muxi_coo_t* A) {
+    if (!name || !A || A->numRows < 0 || A->numCols < 0 || A->nnz < 0 ||
+        (A->nnz > 0 && (!A->rowIdx || !A->colIdx || !A->values)))
+        return SPMV_ERROR_INVALID_MATRIX;
+    for (int k = 0; k < A->nnz; ++k)
+        if (A->rowIdx[k] < 0 || A->rowIdx[k] >= A->numRows || A->colIdx[k] < 0 || A->colIdx[k] >= A->numCols) {
+            set_error("mtx: COO entry outside the matrix");
+            return SPMV_ERROR_INVALID_MATRIX;
+        }
+    File file(name, "w");
+    if (!file.f) {
+        set_error(std::string("mtx: cannot open ") + name + " for writing");
+        return SPMV_ERROR_INVALID_MATRIX;
+    }
+    const T* v = (const T*)A->values;
+    std::fprintf(file.f, "%%%%MatrixMarket matrix coordinate real general\n");
+    std::fprintf(file.f, "%% written by muxi_spmv_amd\n");
+    std::fprintf(file.f, "%d %d %d\n", A->numRows, A->numCols, A->nnz);
+    for (int k = 0; k < A->nnz; ++k) {
+        if (sizeof(T) == 8)
+            std::fprintf(file.f, "%d %d %.17g\n", A->rowIdx[k] + 1, A->colIdx[k] + 1, (double)v[k]);
+        else
+            std::fprintf(file.f, "%d %d %.9g\n", A->rowIdx[k] + 1, A->colIdx[k] + 1, (double)v[k]);
+    }
+    if (std::ferror(file.f)) {
+        set_error(std::string("mtx: write failed on ") + name);
+        return SPMV_ERROR_INVALID_MATRIX;
+    }
+    return SPMV_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" spmv_status_t muxi_mtx_read_header(const char* filename, muxi_mtx_header_t* header) {
@@ -226,6 +289,31 @@ extern "C" spmv_status_t muxi_csr_free_host(muxi_csr_t* A) {
     std::free(A->colIdx);
     std::free(A->values);
     A->rowPtr = nullptr;
+    A->colIdx = nullptr;
+    A->values = nullptr;
+    A->ownsHostMemory = false;
+    return SPMV_SUCCESS;
+}
+
+extern "C" spmv_status_t muxi_mtx_read_coo_f32(const char* filename, muxi_coo_t* out) {
+    return read_coo<float>(filename, out);
+}
+extern "C" spmv_status_t muxi_mtx_read_coo_f64(const char* filename, muxi_coo_t* out) {
+    return read_coo<double>(filename, out);
+}
+extern "C" spmv_status_t muxi_mtx_write_coo_f32(const char* filename, const muxi_coo_t* A) {
+    return write_coo<float>(filename, A);
+}
+extern "C" spmv_status_t muxi_mtx_write_coo_f64(const char* filename, const muxi_coo_t* A) {
+    return write_coo<double>(filename, A);
+}
+
+extern "C" spmv_status_t muxi_coo_free_host(muxi_coo_t* A) {
+    if (!A) return SPMV_ERROR_INVALID_MATRIX;
+    std::free(A->rowIdx);
+    std::free(A->colIdx);
+    std::free(A->values);
+    A->rowIdx = nullptr;
     A->colIdx = nullptr;
     A->values = nullptr;
     A->ownsHostMemory = false;

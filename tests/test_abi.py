@@ -115,3 +115,24 @@ def test_unsupported_type_combination():
 def test_partition_count_matches_oracle(R, N, ePP, mult):
     assert _lib.load().muxi_merge_partition_count(R, N, ePP, mult, 104) == \
         oracle.merge_partition_count(R, N, ePP, mult, 104, 4)
+
+
+def test_format_conversion_host_side_errors():
+    """Argument errors of the conversions are reported before any device work."""
+    lib = _lib.load()
+    a = _lib.muxi_csr_t()
+    a.numRows, a.numCols, a.nnz = 10, 10, 5  # device arrays missing
+    coo, bsr, out = _lib.muxi_coo_t(), _lib.muxi_bsr_t(), _lib.muxi_csr_t()
+    assert lib.muxi_csr_to_coo(None, 0, C.byref(coo), None) == -2
+    assert lib.muxi_csr_to_coo(C.byref(a), 2, C.byref(coo), None) == -8  # HALF values: no
+    assert lib.muxi_csr_to_coo(C.byref(a), 0, C.byref(coo), None) == -2
+    assert lib.muxi_csr_transpose(C.byref(a), 0, C.byref(out), None) == -2
+    assert lib.muxi_csr_to_bsr(C.byref(a), 0, 0, C.byref(bsr), None) == -2  # block size 0
+    bsr.numRows, bsr.numCols, bsr.blockSize, bsr.numBlockRows, bsr.numBlockCols = 10, 10, 4, 2, 3  # wrong count
+    assert lib.muxi_bsr_to_csr(C.byref(bsr), 0, C.byref(out), None) == -2
+    coo.numRows, coo.numCols, coo.nnz = 3, 3, 2  # no arrays
+    assert lib.muxi_coo_to_csr(C.byref(coo), 0, C.byref(out), None) == -2
+    for f in (lib.muxi_csr_free_device, lib.muxi_coo_free_device, lib.muxi_bsr_free_device):
+        assert f(None) == -2
+    assert lib.muxi_csr_free_device(C.byref(a)) == 0  # not owned: untouched
+    assert C.sizeof(_lib.muxi_bsr_t) == 6 * 4 + 6 * 8 + 8 and C.sizeof(_lib.muxi_coo_t) == 4 * 3 + 4 + 6 * 8 + 8

@@ -154,3 +154,28 @@ def test_missing_file(tmp_path):
         mtx.read_header(tmp_path / "nope.mtx")
     with pytest.raises(SpmvError, match="cannot open"):
         mtx.read(tmp_path / "nope.mtx")
+
+
+def test_coo_read_keeps_stored_entries_sorted(tmp_path):
+    p = _put(tmp_path, "ps.mtx", PATTERN_SYM)
+    ri, ci, v, R, C = mtx.read_coo(p, np.float64)  # no mirroring, like readMTXToCOO
+    assert (R, C) == (3, 3)
+    assert ri.tolist() == [0, 1, 2, 2] and ci.tolist() == [0, 0, 1, 2] and v.tolist() == [1, 1, 1, 1]
+    ri, ci, v, _, _ = mtx.read_coo(_put(tmp_path, "g.mtx", GENERAL), np.float32)
+    assert ri.tolist() == [0, 0, 0, 2, 2, 3] and ci.tolist() == [0, 2, 4, 0, 1, 3]
+
+
+def test_coo_write_order_and_round_trip(tmp_path):
+    ri = np.array([2, 0, 1, 0], np.int32)
+    ci = np.array([1, 3, 0, 0], np.int32)
+    v = np.array([0.1, -2.5, 3e-8, 7.0], np.float64)
+    p = tmp_path / "c.mtx"
+    mtx.write_coo(p, ri, ci, v, 3, 4)
+    lines = [ln for ln in p.read_text().splitlines() if not ln.startswith("%")]
+    assert lines[0] == "3 4 4" and lines[1].split()[:2] == ["3", "2"]  # given order, 1-based
+    r2, c2, v2, R, C = mtx.read_coo(p, np.float64)
+    order = np.lexsort((ci, ri))
+    assert (R, C) == (3, 4) and np.array_equal(r2, ri[order]) and np.array_equal(c2, ci[order])
+    assert np.array_equal(v2, v[order])
+    with pytest.raises(SpmvError, match="INVALID_MATRIX"):
+        mtx.write_coo(tmp_path / "bad.mtx", [3], [0], np.ones(1), 3, 4)
