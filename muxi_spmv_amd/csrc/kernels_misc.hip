@@ -104,7 +104,7 @@ template spmv_status_t launch_scale_y<double>(double*, long, double, hipStream_t
 // y += alpha * A^T x, one lane per row scattering with atomics: the
 // counterpart of spmv_csr_scalar_kernel<T, true> (src/spmv/csr/spmv_csr.cuh:105-106).
 // The caller scales y by beta first. Atomic order makes the last bits
-// run-dependent; the plan API's transpose path is deterministic instead.
+// run-dependent; muxi_csr_transpose followed by a plan is deterministic instead.
 template <typename VT, typename XT>
 __global__ __launch_bounds__(256) void transpose_scalar_kernel(int R, const int* __restrict__ rp,
                                                                const int* __restrict__ ci, const VT* __restrict__ v,
