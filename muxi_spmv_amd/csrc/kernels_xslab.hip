@@ -251,12 +251,50 @@ __global__ __launch_bounds__(256) void xslab_combine_kernel(int R, const int* __
 }
 
 // ---- plan construction
-__device__ __forceinline__ int slab_of(int col, int S, int C) { return (int)(((long)col * S) / C); }
+// Slab q holds columns [cut[q], cut[q+1]). The cuts split the nonzeros into S
+// near-equal parts (column quantiles), so every slab, and so every XCD, gets
+// the same work whatever the column distribution (a row shard of a banded
+// matrix touches only a narrow column window, for instance).
+struct SlabCuts {
+    int c[kMaxSlabs + 1];
+};
+
+__device__ __forceinline__ int slab_of(int col, const SlabCuts& cut, int S) {
+    int q = 0;
+    for (int k = 1; k < S; ++k) q += col >= cut.c[k];
+    return q;
+}
+
+__global__ void xslab_colhist_kernel(const int* __restrict__ ci, long N, int C, int* __restrict__ hist) {
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < N; k += (long)gridDim.x * blockDim.x) {
+        const int c = ci[k];
+        if ((unsigned)c < (unsigned)C) atomicAdd(hist + c, 1);
+    }
+}
+
+// cut[q] = the first column c with (nonzeros in columns < c) >= ceil(q N / S); pre = exclusive prefix of the
+// column histogram, C + 1 entries.
+__global__ void xslab_cuts_kernel(const int* __restrict__ pre, int C, long N, int S, int* __restrict__ cut) {
+    const int q = threadIdx.x;
+    if (q > S) return;
+    if (q == 0 || q == S) {
+        cut[q] = q == 0 ? 0 : C;
+        return;
+    }
+    const long t = ((long)q * N + S - 1) / S;
+    int lo = 0, hi = C;
+    while (lo < hi) {
+        const int mid = (int)(((long)lo + hi) >> 1);
+        if (pre[mid] < t) lo = mid + 1;
+        else hi = mid;
+    }
+    cut[q] = lo;
+}
 
 // One wavefront per row (rows reach 10^5 nonzeros): the row is read in
 // 64-wide chunks and lane q < S counts slab q's nonzeros from ballots.
 __global__ __launch_bounds__(256) void xslab_count_kernel(const int* __restrict__ rp, const int* __restrict__ ci,
-                                                          int R, int C, int S, int* __restrict__ cnt,
+                                                          int R, SlabCuts cut, int S, int* __restrict__ cnt,
                                                           int* __restrict__ flag, int* __restrict__ npair) {
     const long r = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
@@ -264,7 +302,7 @@ __global__ __launch_bounds__(256) void xslab_count_kernel(const int* __restrict_
     const int b = rp[r], e = rp[r + 1];
     int c = 0;
     for (int i = b; i < e; i += 64) {
-        const int s = i + lane < e ? slab_of(ci[i + lane], S, C) : -1;
+        const int s = i + lane < e ? slab_of(ci[i + lane], cut, S) : -1;
         for (int q = 0; q < S; ++q) {
             const unsigned long long m = __ballot(s == q);
             if (lane == q) c += __popcll(m);
@@ -283,7 +321,7 @@ __global__ __launch_bounds__(256) void xslab_count_kernel(const int* __restrict_
 // order inside every slab.
 template <typename VT>
 __global__ __launch_bounds__(256) void xslab_fill_kernel(const int* __restrict__ rp, const int* __restrict__ ci,
-                                                         const VT* __restrict__ v, int R, int C, int S,
+                                                         const VT* __restrict__ v, int R, SlabCuts cut, int S,
                                                          const int* __restrict__ cnt, const int* __restrict__ pos,
                                                          const int* __restrict__ lrow,
                                                          const int* __restrict__ nnz_base,
@@ -310,7 +348,7 @@ __global__ __launch_bounds__(256) void xslab_fill_kernel(const int* __restrict__
     for (int i = b; i < e; i += 64) {
         const bool in = i + lane < e;
         const int col = in ? ci[i + lane] : 0;
-        const int s = in ? slab_of(col, S, C) : -1;
+        const int s = in ? slab_of(col, cut, S) : -1;
         int o = 0;
         for (int q = 0; q < S; ++q) {
             const unsigned long long m = __ballot(s == q);
@@ -379,6 +417,27 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
     }
     const int SIG = 64 * epl;
     const long SR = (long)S * R;
+    SlabCuts cut{};
+    {
+        int *hist, *pre, *d_cut;
+        MUXI_HIP(hipMallocAsync(&hist, sizeof(int) * ((size_t)C + 1), s));
+        MUXI_HIP(hipMallocAsync(&pre, sizeof(int) * ((size_t)C + 1), s));
+        MUXI_HIP(hipMallocAsync(&d_cut, sizeof(int) * (kMaxSlabs + 1), s));
+        MUXI_HIP(hipMemsetAsync(hist, 0, sizeof(int) * ((size_t)C + 1), s));
+        if (p->A.N > 0) {
+            unsigned g = grid_for(p->A.N, 256);
+            hipLaunchKernelGGL(xslab_colhist_kernel, dim3(g > 65536 ? 65536 : g), dim3(256), 0, s, p->A.ci,
+                               (long)p->A.N, C, hist);
+            MUXI_LAUNCH_CHECK();
+        }
+        spmv_status_t st = exscan(hist, pre, (long)C + 1, s);
+        if (st != SPMV_SUCCESS) return st;
+        hipLaunchKernelGGL(xslab_cuts_kernel, dim3(1), dim3(64), 0, s, pre, C, (long)p->A.N, S, d_cut);
+        MUXI_LAUNCH_CHECK();
+        MUXI_HIP(hipMemcpyAsync(cut.c, d_cut, sizeof(int) * (S + 1), hipMemcpyDeviceToHost, s));
+        MUXI_HIP(hipStreamSynchronize(s));
+        for (int* t : {hist, pre, d_cut}) (void)hipFreeAsync(t, s);
+    }
     int *cnt, *flag, *pos, *lrow, *npair;
     MUXI_HIP(hipMallocAsync(&cnt, sizeof(int) * (SR + 1), s));
     MUXI_HIP(hipMallocAsync(&flag, sizeof(int) * (SR + 1), s));
@@ -388,8 +447,8 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
     MUXI_HIP(hipMemsetAsync(cnt + SR, 0, sizeof(int), s));
     MUXI_HIP(hipMemsetAsync(flag + SR, 0, sizeof(int), s));
     MUXI_HIP(hipMemsetAsync(npair + R, 0, sizeof(int), s));
-    hipLaunchKernelGGL(xslab_count_kernel, dim3(grid_for((long)R * 64, 256)), dim3(256), 0, s, p->A.rp, p->A.ci, R, C,
-                       S, cnt, flag, npair);
+    hipLaunchKernelGGL(xslab_count_kernel, dim3(grid_for((long)R * 64, 256)), dim3(256), 0, s, p->A.rp, p->A.ci, R,
+                       cut, S, cnt, flag, npair);
     MUXI_LAUNCH_CHECK();
     // row-major pair offsets rpp[R+1] live in the plan
     MUXI_HIP(hipMalloc(&p->d_slab_rpp, sizeof(int) * ((size_t)R + 1)));
@@ -462,12 +521,12 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
         using T = decltype(tag);
         using VT = typename T::VT;
         hipLaunchKernelGGL(xslab_fill_kernel<VT>, dim3(grid_for((long)R * 64, 256)), dim3(256), 0, s, p->A.rp, p->A.ci,
-                           (const VT*)p->A.v, R, C, S, cnt, pos, lrow, d_nb, d_nb + S, p->d_slab_rpp, p->d_slab_rp,
+                           (const VT*)p->A.v, R, cut, S, cnt, pos, lrow, d_nb, d_nb + S, p->d_slab_rpp, p->d_slab_rp,
                            p->d_slab_ploc, p->d_slab_ci, (VT*)p->d_slab_v);
         MUXI_LAUNCH_CHECK();
         for (int q = 0; q < S; ++q) {
             const long from = (long)hpad[q] + sa.nnz[q], to = (long)hpad[q] + (long)sa.tiles[q] * SIG;
-            const int lo = (int)(((long)q * C + S - 1) / S);  // first column of slab q
+            const int lo = cut.c[q];  // first column of slab q (it has nonzeros, so lo < C)
             if (to > from) {
                 hipLaunchKernelGGL(xslab_pad_kernel<VT>, dim3(grid_for(to - from, 256)), dim3(256), 0, s,
                                    p->d_slab_ci, (VT*)p->d_slab_v, from, to, lo);
