@@ -156,6 +156,13 @@ SIGNATURES: dict[str, tuple] = {
     "muxi_mtx_write_coo_f32": (_I, [C.c_char_p, _P_COO]),
     "muxi_mtx_write_coo_f64": (_I, [C.c_char_p, _P_COO]),
     "muxi_coo_free_host": (_I, [_P_COO]),
+    "muxi_host_plan_create": (_I, [_P_CSR, _I, _I, _I, _I, _I, C.POINTER(_VP)]),
+    "muxi_host_plan_destroy": (_I, [_VP]),
+    "muxi_host_plan_exec": (_I, [_VP, _VP, _VP, _D, _D]),
+    "muxi_host_plan_timing": (_I, [_VP, C.POINTER(_F), C.POINTER(_F), C.POINTER(_F), C.POINTER(_F)]),
+    "muxi_host_plan_chunk": (_I, [_VP, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    "muxi_host_alloc": (_I, [C.POINTER(_VP), _Z]),
+    "muxi_host_free": (_I, [_VP]),
     "muxi_csr_to_coo": (_I, [_P_CSR, _I, _P_COO, _VP]),
     "muxi_coo_to_csr": (_I, [_P_COO, _I, _P_CSR, _VP]),
     "muxi_csr_to_bsr": (_I, [_P_CSR, _I, _I, _P_BSR, _VP]),

@@ -17,7 +17,11 @@ shard      one GPU's share of an 8-GPU row split (muxi_shard_rows: equal merge-p
            diagonals moved to row starts, x replicated) of the power-law and
            Laplacian matrices: the heaviest shard run with the AUTO plan, next
            to the whole matrix on one GPU.
-All timings are cold: each timed launch is preceded by a 512 MiB cache flush,
+host       x and y in host memory (muxi_spmv_host.h): pinned and pageable buffers,
+           1 / 2 / 4 / 8 row chunks overlapping PCIe copies with the kernels, on
+           the power-law matrix and the FP64 Laplacian; end-to-end time from the
+           host plan's own events (median of 10) next to the device-resident plan.
+All timings except host's are cold: each timed launch is preceded by a 512 MiB cache flush,
 and HIP events bracket the launch.
 """
 from __future__ import annotations
@@ -33,7 +37,7 @@ import time
 import numpy as np
 import torch
 
-from . import api, prep, tune
+from . import api, host, prep, tune
 from . import generators as G
 from .benchmark import HBM_PEAK_GBS, algorithmic_bytes
 from ._lib import check, load
@@ -239,13 +243,58 @@ def shard(args) -> list[dict]:
     return out
 
 
+def host_path(args) -> list[dict]:
+    tm = Timer()
+    out = []
+    for name, h, dt in (("powerlaw_2m_fp32", G.cfg_powerlaw_2m(), torch.float32),
+                        ("laplacian_4096_fp64", G.cfg_laplacian_4096(), torch.float64)):
+        vdt = np.float64 if dt == torch.float64 else np.float32
+        esz = 8 if dt == torch.float64 else 4
+        A = api.CSRMatrix(h.rowPtr, h.colIdx, torch.from_numpy(h.values).to(dt), h.numCols)
+        nb = algorithmic_bytes(h.numRows, h.numCols, h.nnz, esz, esz)
+        xs = G.uniform_vector(h.numCols, 3, vdt)
+        p = api.Plan(A, "auto", 0, xtype=dt)
+        xd = torch.from_numpy(xs).cuda()
+        yd = torch.empty(h.numRows, dtype=dt, device="cuda")
+        dev_ms = tm.time(lambda: p.exec(xd, yd), flush=False)
+        info = p.info()
+        p.destroy()
+        base = {"workload": name, "device_resident_ms": round(dev_ms, 4), "device_kernel": f"{info.kernel}:{info.param0}",
+                "pcie_bytes": (h.numCols + h.numRows) * esz}
+        for pinned in (True, False):
+            if pinned:
+                x = host.pinned_empty(h.numCols, vdt)
+                y = host.pinned_empty(h.numRows, vdt)
+            else:
+                x, y = np.empty(h.numCols, vdt), np.empty(h.numRows, vdt)
+            x[:] = xs
+            for chunks in (1, 2, 4, 8):
+                hp = host.HostPlan(A, chunks, xtype=dt)
+                hp.exec(x, y)
+                ts = []
+                for _ in range(10):
+                    hp.exec(x, y)
+                    ts.append(hp.timing())
+                t = sorted(ts, key=lambda v: v.total_ms)[len(ts) // 2]
+                rec = dict(base, pinned=pinned, chunks=chunks, total_ms=round(t.total_ms, 4),
+                           first_copy_in_ms=round(t.first_copy_in_ms, 4), kernels_ms=round(t.kernels_ms, 4),
+                           copy_out_tail_ms=round(t.copy_out_tail_ms, 4),
+                           end_to_end_GBps=round(nb / t.total_ms / 1e6, 1),
+                           kernels=[hp.chunk(k)[2] + ":" + str(hp.chunk(k)[3]) for k in range(chunks)])
+                hp.destroy()
+                log(json.dumps(rec))
+                out.append(rec)
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["precision", "batched", "spmm", "laplacian", "shard"])
+    ap.add_argument("what", choices=["precision", "batched", "spmm", "laplacian", "shard", "host"])
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     torch.cuda.set_device(0)
-    res = {"precision": precision, "batched": batched, "spmm": spmm, "laplacian": laplacian, "shard": shard}[a.what](a)
+    res = {"precision": precision, "batched": batched, "spmm": spmm, "laplacian": laplacian, "shard": shard,
+           "host": host_path}[a.what](a)
     txt = "\n".join(json.dumps(r) for r in res)
     print(txt, flush=True)
     if a.out:

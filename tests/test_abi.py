@@ -136,3 +136,18 @@ def test_format_conversion_host_side_errors():
         assert f(None) == -2
     assert lib.muxi_csr_free_device(C.byref(a)) == 0  # not owned: untouched
     assert C.sizeof(_lib.muxi_bsr_t) == 6 * 4 + 6 * 8 + 8 and C.sizeof(_lib.muxi_coo_t) == 4 * 3 + 4 + 6 * 8 + 8
+
+
+def test_host_plan_host_side_errors():
+    lib = _lib.load()
+    p = C.c_void_p()
+    assert lib.muxi_host_plan_create(None, 0, 0, 4, 0, 0, C.byref(p)) == -2
+    a = _lib.muxi_csr_t()
+    a.numRows, a.numCols, a.nnz = 10, 10, 5  # device arrays missing
+    assert lib.muxi_host_plan_create(C.byref(a), 0, 0, 4, 0, 0, C.byref(p)) == -2
+    assert lib.muxi_host_plan_create(C.byref(a), 0, 0, 4, 0, 0, None) == -1
+    assert lib.muxi_host_plan_exec(None, None, None, 1.0, 0.0) == -1
+    assert lib.muxi_host_plan_timing(None, None, None, None, None) == -1
+    assert lib.muxi_host_plan_destroy(None) == -1
+    assert lib.muxi_host_alloc(None, 16) == -3
+    assert lib.muxi_host_free(None) == 0
