@@ -5,9 +5,9 @@
 // has an 8 MB x) and the columns are spread, every XCD pulls all of x through
 // its L2: on that matrix the plain merge kernel moves ~1.1 GB through L2 for
 // 139 MB of matrix and vectors (TCC_MISS x 128 B). The plan splits the
-// columns into S slabs (column c is in slab floor(c*S/C)) and stores, per
-// slab, a compressed CSR of the nonzeros that fall in it: slab-local row
-// pointers over the rows that have any (so no slab row is empty), columns
+// columns into S slabs (column ranges holding equal shares of the nonzeros)
+// and stores, per slab, a compressed CSR of the nonzeros that fall in it:
+// slab-local row pointers over the rows that have any (no slab row is empty), columns
 // and values, padded to whole tiles, plus one row-end bit per nonzero and a
 // descriptor per tile (the slab row of its first nonzero). Workgroup b works
 // on slab b mod S, so an XCD gathers from one slice of x that stays in its
