@@ -85,11 +85,12 @@ spmv_status_t muxi_mlp_adam(float* d_params, const float* d_grad, float* d_m, fl
  * A library built with trained weights (the package's selector_weights.json)
  * starts with that selector installed ("built-in"). muxi_selector_set
  * installs a trained selector (process-wide, replaces any previous one):
- * params as above with in = 10, mu/sd the input normalisation, and the
+ * params as above with in = 10 or 11, mu/sd the input normalisation, and the
  * candidate each output scores (kernels[n_out], params0[n_out]). The model
  * input is log1p(rows), log1p(nnz), log1p(mean), log1p(sqrt(var)),
  * log1p(max), bandwidth, diagonal-block fill, empty fraction,
- * log1p(max/mean), log1p(nnz) - log1p(rows). */
+ * log1p(max/mean), log1p(nnz) - log1p(rows), and for in = 11 also 1 when x
+ * and y are FP64 (0 otherwise). */
 spmv_status_t muxi_selector_set(const float* params, int n_in, int hidden, int n_out, const float* mu,
                                 const float* sd, const int* kernels, const int* params0);
 /* Removes any selector: AUTO falls back to the fixed rule below. */

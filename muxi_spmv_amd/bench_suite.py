@@ -12,7 +12,7 @@ batched    256 CSR matrices, rows log-uniform in [1e3, 2e5], mean nnz/row
            launch (eager and hipGraph) against one CSR-vector launch per
            matrix (eager and hipGraph).
 spmm       Y = A X on the power-law matrix for k = 8 and 32 (row-major X).
-laplacian  4096^2 5-point Laplacian in FP64, CSR kernels and ELLPACK.
+laplacian  4096^2 5-point Laplacian in FP64, CSR kernels, ELLPACK and the AUTO pick.
 shard      one GPU's share of an 8-GPU row split (muxi_shard_rows: equal merge-path
            diagonals moved to row starts, x replicated) of the power-law and
            Laplacian matrices: the heaviest shard run with the AUTO plan, next
@@ -198,11 +198,12 @@ def laplacian(args) -> list[dict]:
     b = algorithmic_bytes(h.numRows, h.numCols, h.nnz, 8, 8)
     out = []
     for k, p in [("ell", 0), ("adaptive", 512), ("adaptive", 1024), ("vwave", 2), ("vwave", 4), ("csr5", 512),
-                 ("merge", 4), ("xslab", 0)]:
+                 ("merge", 4), ("xslab", 0), ("auto", 0)]:
         pl = api.Plan(A, k, p, xtype=torch.float64)
         ms = tm.time(lambda: pl.exec(x, y))
-        rec = {"workload": "laplacian_4096_fp64", "kernel": f"{k}:{p}", "ms": round(ms, 4),
-               "GBps": round(b / ms / 1e6, 1), "frac_hbm": round(b / ms / 1e6 / HBM_PEAK_GBS, 4)}
+        info = pl.info()
+        rec = {"workload": "laplacian_4096_fp64", "kernel": f"{k}:{p}", "runs": f"{info.kernel}:{info.param0}",
+               "ms": round(ms, 4), "GBps": round(b / ms / 1e6, 1), "frac_hbm": round(b / ms / 1e6 / HBM_PEAK_GBS, 4)}
         log(json.dumps(rec))
         out.append(rec)
         pl.destroy()

@@ -54,8 +54,8 @@ def write_selector_header() -> Path | None:
         inc.unlink(missing_ok=True)
         return None
     st = json.loads(WEIGHTS.read_text())
-    if st["n_in"] != 10 or len(st["candidates"]) != st["n_out"]:
-        raise RuntimeError(f"{WEIGHTS}: expected 10 inputs and one output per candidate")
+    if st["n_in"] not in (10, 11) or len(st["candidates"]) != st["n_out"]:
+        raise RuntimeError(f"{WEIGHTS}: expected 10 or 11 inputs and one output per candidate")
     ks, ps = [], []
     for name in st["candidates"]:
         k, p = name.split(":")

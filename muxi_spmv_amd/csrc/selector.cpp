@@ -56,8 +56,10 @@ void free_sel(Selector* s) {
 
 }  // namespace
 
-// Model inputs from the eight features (same transform as tune.feature_vector).
-static void feature_input(const double* f, float* out) {
+// Model inputs from the eight features and the x / y precision (same transform
+// as tune.feature_vector): ten size and shape inputs, then 1 for FP64 x and y
+// (read only by 11-input models).
+static void feature_input(const double* f, bool fp64, float* out) {
     const double rows = f[0], nnz = f[1], mean = f[2], var = f[3], mx = f[4];
     out[0] = (float)std::log1p(rows);
     out[1] = (float)std::log1p(nnz);
@@ -69,6 +71,7 @@ static void feature_input(const double* f, float* out) {
     out[7] = (float)f[7];
     out[8] = (float)std::log1p(mx / (mean > 1e-9 ? mean : 1e-9));
     out[9] = (float)(std::log1p(nnz) - std::log1p(rows));
+    out[10] = fp64 ? 1.0f : 0.0f;
 }
 
 static bool candidate_allowed(int kernel, const double* f) {
@@ -80,8 +83,8 @@ static bool candidate_allowed(int kernel, const double* f) {
 
 static spmv_status_t make_selector(const float* params, int n_in, int hidden, int n_out, const float* mu,
                                    const float* sd, const int* kernels, const int* params0, Selector** out) {
-    if (!params || !mu || !sd || !kernels || !params0 || n_in != 10 || hidden <= 0 || n_out <= 0) {
-        set_error("selector: invalid model (expects 10 inputs)");
+    if (!params || !mu || !sd || !kernels || !params0 || (n_in != 10 && n_in != 11) || hidden <= 0 || n_out <= 0) {
+        set_error("selector: invalid model (expects 10 inputs, or 11 with the FP64 flag)");
         return SPMV_ERROR_INVALID_VECTOR;
     }
     Selector* s = new Selector();
@@ -170,11 +173,11 @@ extern "C" spmv_status_t muxi_selector_pick(const muxi_csr_t* A, spmv_type_t xty
         return SPMV_SUCCESS;
     }
     Selector& s = *g_sel;
-    float in[10];
-    feature_input(f, in);
-    for (int i = 0; i < 10; ++i) in[i] = (in[i] - s.mu[i]) / s.sd[i];
+    float in[11];
+    feature_input(f, xtype == SPMV_TYPE_DOUBLE, in);
+    for (int i = 0; i < s.n_in; ++i) in[i] = (in[i] - s.mu[i]) / s.sd[i];
     hipStream_t hs = (hipStream_t)stream;
-    MUXI_HIP(hipMemcpyAsync(s.d_in, in, sizeof(in), hipMemcpyHostToDevice, hs));
+    MUXI_HIP(hipMemcpyAsync(s.d_in, in, sizeof(float) * s.n_in, hipMemcpyHostToDevice, hs));
     st = muxi_mlp_forward(s.d_params, s.n_in, s.h, s.n_out, s.d_in, 1, s.d_out, s.d_work, stream);
     if (st != SPMV_SUCCESS) return st;
     std::vector<float> out(s.n_out);

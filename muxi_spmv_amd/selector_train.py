@@ -5,7 +5,13 @@ python -m muxi_spmv_amd.selector_train [--train 2000] [--test 400] [--db PATH] [
 1. Matrices i = 0 .. train+test-1 come from five seeded families (uniform
    rows, power-law rows, banded, 32x32 block-diagonal, banded with random
    row and column permutations; family = i mod 5, seed 6 + i), rows
-   log-uniform in [1e4, 4e6], FP32, generated on the device.
+   log-uniform in [1e4, 4e6], FP32, generated on the device. Indices
+   BIG_BASE + j are further matrices of the same families with rows
+   log-uniform in [4e6, 2e7] (--big-train, then --big-test of them), so the
+   named workloads of up to 16.8 M rows are inside the training range. Index
+   FP64_BASE + i is matrix i again with FP64 values, x and y (the first
+   --fp64-train training and --fp64-test held-out matrices of both ranges),
+   so the network sees what the precision changes.
 2. The autotune sweep times every candidate kernel/configuration cold and
    appends one JSON record per matrix to the DB (resumable: a record is
    only topped up with the candidates it lacks; --skip leaves kernels out of
@@ -32,9 +38,18 @@ import torch
 from . import tune
 
 FAMILIES = ["uniform", "powerlaw", "banded", "blockdiag", "banded_perm"]
+BIG_BASE = 50_000
+BIG_ROWS = (4e6, 2e7)
+FP64_BASE = 100_000
 
 
 def matrix_spec(i: int, rows_min: float, rows_max: float) -> dict:
+    if i >= FP64_BASE:
+        spec = matrix_spec(i - FP64_BASE, rows_min, rows_max)
+        spec.update(i=i, dtype="fp64")
+        return spec
+    if i >= BIG_BASE:
+        rows_min, rows_max = BIG_ROWS
     rng = np.random.default_rng(6 + i)
     fam = FAMILIES[i % 5]
     R = int(math.exp(rng.uniform(math.log(rows_min), math.log(rows_max))))
@@ -56,7 +71,11 @@ def build(spec: dict):
     fam = spec["family"]
     gfam = "banded" if fam == "banded_perm" else fam
     return tune.gen_matrix(gfam, spec["rows"], spec["rows"], spec["p0"], spec["p1"], spec["p2"], seed=spec["seed"],
-                           permute=(fam == "banded_perm"))
+                           permute=(fam == "banded_perm"), dtype=_dtype(spec))
+
+
+def _dtype(spec: dict) -> torch.dtype:
+    return torch.float64 if spec.get("dtype") == "fp64" else torch.float32
 
 
 def load_db(path: Path) -> dict[int, dict]:
@@ -100,8 +119,8 @@ def run_sweeps(indices, db_path: Path, rows_min, rows_max, reps, log_every=25, t
                 break
             spec = matrix_spec(i, rows_min, rows_max)
             A = build(spec)
-            x = tune.gen_vector(A.numCols, seed=spec["seed"])
-            y = torch.empty(A.numRows, device="cuda")
+            x = tune.gen_vector(A.numCols, dtype=_dtype(spec), seed=spec["seed"])
+            y = torch.empty(A.numRows, dtype=_dtype(spec), device="cuda")
             feats = tune.features(A) if i not in done else np.array(done[i]["features"])
             times = dict(done[i]["times"]) if i in done else {}
             times.update(tune.sweep(A, x, y, fl, reps=reps, candidates=todo, f=feats))
@@ -120,7 +139,7 @@ def run_sweeps(indices, db_path: Path, rows_min, rows_max, reps, log_every=25, t
 def dataset(recs):
     X, T, M = [], [], []
     for r in recs:
-        X.append(tune.feature_vector(np.array(r["features"])))
+        X.append(tune.feature_vector(np.array(r["features"]), r.get("dtype") == "fp64"))
         t, m = tune.targets_from_sweep(r["times"])
         T.append(t)
         M.append(m)
@@ -152,10 +171,19 @@ def evaluate(sel: tune.Selector, train, test) -> dict:
     fixed = min(tot, key=tot.get)
     fix = regrets(test, [fixed] * len(test))
     q = lambda a, p: float(np.percentile(a[np.isfinite(a)], p)) if np.isfinite(a).any() else None  # noqa: E731
+    by = {}
+    groups = {"fp32": lambda r: r.get("dtype") != "fp64", "fp64": lambda r: r.get("dtype") == "fp64",
+              "rows_over_4e6": lambda r: r["i"] % FP64_BASE >= BIG_BASE}
+    for name, g in groups.items():
+        sel_ = np.array([g(r) for r in test])
+        if sel_.any():
+            by[name] = {"n": int(sel_.sum()), "median": q(reg[sel_], 50), "p95": q(reg[sel_], 95),
+                        "exact_best_frac": float(np.mean(reg[sel_] == 0.0)), "reference_rule_median": q(ref[sel_], 50)}
     return {
         "n_test": len(test), "n_train": len(train),
         "learned": {"median": q(reg, 50), "p95": q(reg, 95), "mean": float(np.mean(reg)),
                     "exact_best_frac": float(np.mean(reg == 0.0))},
+        "by_group": by,
         "reference_rule": {"median": q(ref, 50), "p95": q(ref, 95), "mean": float(np.mean(ref[np.isfinite(ref)]))},
         "best_fixed": {"config": fixed, "median": q(fix, 50), "p95": q(fix, 95), "mean": float(np.mean(fix))},
     }
@@ -173,7 +201,11 @@ def main(argv=None) -> int:
     ap.add_argument("--summary", default=None)
     ap.add_argument("--sweep-only", action="store_true")
     ap.add_argument("--train-only", action="store_true")
-    ap.add_argument("--epochs", type=int, default=4000)
+    ap.add_argument("--epochs", type=int, default=8000)
+    ap.add_argument("--fp64-train", type=int, default=2000, help="training matrices also swept with FP64 values")
+    ap.add_argument("--fp64-test", type=int, default=400, help="held-out matrices also swept with FP64 values")
+    ap.add_argument("--big-train", type=int, default=300, help="training matrices with 4e6 to 2e7 rows")
+    ap.add_argument("--big-test", type=int, default=60, help="held-out matrices with 4e6 to 2e7 rows")
     ap.add_argument("--time-budget", type=float, default=None, help="seconds of sweeping in this call")
     ap.add_argument("--skip", default="", help="comma-separated kernels not to time in this call")
     ap.add_argument("--retime", default="", help="comma-separated kernels to time again on existing records")
@@ -182,16 +214,20 @@ def main(argv=None) -> int:
     db = Path(a.db)
     db.parent.mkdir(parents=True, exist_ok=True)
     n = a.train + a.test
+    big_train = [BIG_BASE + j for j in range(a.big_train)]
+    big_test = [BIG_BASE + a.big_train + j for j in range(a.big_test)]
+    f64_train = [FP64_BASE + j for j in range(min(a.fp64_train, a.train))] + [FP64_BASE + i for i in big_train]
+    f64_test = [FP64_BASE + a.train + j for j in range(min(a.fp64_test, a.test))] + [FP64_BASE + i for i in big_test]
     if not a.train_only:
         # interleave so that a partial run covers both splits
-        order = list(range(a.train, n)) + list(range(a.train))
+        order = list(range(a.train, n)) + list(range(a.train)) + big_test + big_train + f64_test + f64_train
         run_sweeps(order, db, a.rows_min, a.rows_max, a.reps, time_budget_s=a.time_budget,
                    skip=tuple(k for k in a.skip.split(",") if k), retime=tuple(k for k in a.retime.split(",") if k))
         if a.sweep_only:
             return 0
     recs = load_db(db)
-    train = [recs[i] for i in range(a.train) if i in recs]
-    test = [recs[i] for i in range(a.train, n) if i in recs]
+    train = [recs[i] for i in list(range(a.train)) + big_train + f64_train if i in recs]
+    test = [recs[i] for i in list(range(a.train, n)) + big_test + f64_test if i in recs]
     print(f"[train] {len(train)} train / {len(test)} held-out records", flush=True)
     X, T, M = dataset(train)
     sel = tune.Selector(64, tune.N_INPUT, len(tune.CANDIDATES), seed=0)

@@ -102,15 +102,15 @@ def features(A) -> np.ndarray:
     return np.array(out[:], dtype=np.float64)
 
 
-def feature_vector(f: np.ndarray) -> np.ndarray:
-    """Model inputs: logs of the size-like features, the fractions as they are."""
+def feature_vector(f: np.ndarray, fp64: bool = False) -> np.ndarray:
+    """Model inputs: logs of the size-like features, the fractions as they are, and 1 for FP64 x and y."""
     rows, nnz, mean, var, mx, band, diag, empty = f
     return np.array([math.log1p(rows), math.log1p(nnz), math.log1p(mean), math.log1p(math.sqrt(max(var, 0.0))),
                      math.log1p(mx), band, diag, empty, math.log1p(mx / max(mean, 1e-9)),
-                     math.log1p(nnz) - math.log1p(rows)], dtype=np.float32)
+                     math.log1p(nnz) - math.log1p(rows), 1.0 if fp64 else 0.0], dtype=np.float32)
 
 
-N_INPUT = 10
+N_INPUT = 11
 
 
 def ell_feasible(f: np.ndarray, limit_slots: float = 4.0) -> bool:
@@ -195,7 +195,8 @@ class Selector:
         self.tmu = torch.zeros(n_out)
 
     def _norm(self, X: np.ndarray) -> torch.Tensor:
-        return ((torch.from_numpy(np.asarray(X, np.float32)) - self.mu) / self.sd).cuda().contiguous()
+        X = np.asarray(X, np.float32)[:, :self.n_in]  # a 10-input model ignores the FP64 flag
+        return ((torch.from_numpy(np.ascontiguousarray(X)) - self.mu) / self.sd).cuda().contiguous()
 
     def forward(self, Xn: torch.Tensor) -> torch.Tensor:
         B = Xn.shape[0]
@@ -222,7 +223,7 @@ class Selector:
             batch: int = 256, seed: int = 0) -> list[float]:
         """X [n, in] raw model inputs, T [n, out] log-times (relative to each
         matrix's best), M [n, out] masks. Adam on minibatches."""
-        X = np.asarray(X, np.float32)
+        X = np.asarray(X, np.float32)[:, :self.n_in]
         self.mu = torch.from_numpy(X.mean(0))
         self.sd = torch.from_numpy(X.std(0) + 1e-6)
         Xn = self._norm(X)

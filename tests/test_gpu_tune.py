@@ -144,24 +144,26 @@ def test_library_auto_pick_matches_python_selector():
     """The C-side pick (device features, model input, normalisation, HIP forward, feasibility mask) agrees with
     the Python selector on the same network; random weights suffice for this."""
     from muxi_spmv_amd._lib import load
-    sel = tune.Selector(64, seed=3)
     rng = np.random.default_rng(0)
-    sel.mu = torch.from_numpy(rng.standard_normal(10).astype(np.float32))
-    sel.sd = torch.from_numpy((1 + rng.random(10)).astype(np.float32))
-    sel.install()
-    try:
-        assert load().muxi_selector_state() == 2
-        for i, (fam, p) in enumerate([("uniform", (5, 15, 0)), ("powerlaw", (2.0, 1, 5000)), ("banded", (64, 16, 0)),
-                                      ("blockdiag", (32, 0, 0)), ("poisson", (30.0, 0, 0))]):
-            A = tune.gen_matrix(fam, 30000, 30000, *p, seed=20 + i)
-            f = tune.features(A)
-            P = sel.predict(tune.feature_vector(f)[None])[0]
-            P = np.where(tune.feasible_mask(f) > 0, P, np.inf)
-            name = tune.library_pick(A)
-            assert name in tune.CAND_NAMES
-            assert P[tune.CAND_NAMES.index(name)] <= P.min() + 1e-4 * (1 + abs(P.min()))
-    finally:
-        load().muxi_selector_reset()
+    for n_in in (10, 11):  # the 11th input is the FP64 flag
+        sel = tune.Selector(64, n_in=n_in, seed=3)
+        sel.mu = torch.from_numpy(rng.standard_normal(n_in).astype(np.float32))
+        sel.sd = torch.from_numpy((1 + rng.random(n_in)).astype(np.float32))
+        sel.install()
+        try:
+            assert load().muxi_selector_state() == 2
+            for i, (fam, p) in enumerate([("uniform", (5, 15, 0)), ("powerlaw", (2.0, 1, 5000)),
+                                          ("banded", (64, 16, 0)), ("blockdiag", (32, 0, 0)), ("poisson", (30.0, 0, 0))]):
+                A = tune.gen_matrix(fam, 30000, 30000, *p, seed=20 + i)
+                f = tune.features(A)
+                for xt in (torch.float32, torch.float64):
+                    P = sel.predict(tune.feature_vector(f, xt == torch.float64)[None])[0]
+                    P = np.where(tune.feasible_mask(f) > 0, P, np.inf)
+                    name = tune.library_pick(A, xt)
+                    assert name in tune.CAND_NAMES
+                    assert P[tune.CAND_NAMES.index(name)] <= P.min() + 1e-4 * (1 + abs(P.min()))
+        finally:
+            load().muxi_selector_reset()
 
 
 def test_selector_clear_falls_back_to_fixed_rule():

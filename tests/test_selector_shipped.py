@@ -1,7 +1,8 @@
 """The selector weights shipped in the package, scored on the held-out matrices of the sweep DB they were
-trained beside (muxi_spmv_amd/selector_db.jsonl.gz: 2,000 training + 400 held-out matrices, every candidate
-timed cold on an MI355X). Predictions come from the FP64 oracle network (oracle/mlp.py), so this runs on the
-CPU; the device forward pass is checked against the same oracle in tests/test_gpu_tune.py."""
+trained beside (muxi_spmv_amd/selector_db.jsonl.gz: 2,000 training + 400 held-out matrices of 1e4 to 4e6 rows
+and 300 + 60 of 4e6 to 2e7 rows, all in FP32 and again with FP64 values, x and y; every candidate timed cold on
+an MI355X). Predictions come from the FP64 oracle network (oracle/mlp.py), so this runs on the CPU; the device
+forward pass is checked against the same oracle in tests/test_gpu_tune.py."""
 import json
 import math
 from pathlib import Path
@@ -28,15 +29,23 @@ def test_shipped_selector_candidates_match_the_library():
     st, recs, tune = _load()
     assert st["candidates"] == tune.CAND_NAMES
     assert st["n_in"] == tune.N_INPUT and st["n_out"] == len(tune.CANDIDATES)
-    assert len(recs) == 2400
+    assert sum(1 for i in recs if i < FP64) == 2400 + 360
+    assert sum(1 for i in recs if i >= FP64) == 2400 + 360
+
+
+FP64 = 100_000  # selector_train.FP64_BASE: index FP64 + i is matrix i with FP64 values, x and y
+BIG = 50_000  # selector_train.BIG_BASE: 300 training then 60 held-out matrices of 4e6 to 2e7 rows
 
 
 @pytest.mark.skipif(not (WEIGHTS.exists() and DB.exists()), reason="no shipped selector")
-def test_shipped_selector_regret_on_held_out_matrices():
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_shipped_selector_regret_on_held_out_matrices(precision):
     st, recs, tune = _load()
-    test = [recs[i] for i in range(2000, 2400)]
-    train = [recs[i] for i in range(2000)]
-    X = np.array([tune.feature_vector(np.array(r["features"])) for r in test], np.float64)
+    off = FP64 if precision == "fp64" else 0
+    test = [recs[off + i] for i in list(range(2000, 2400)) + list(range(BIG + 300, BIG + 360))]
+    train = [recs[off + i] for i in list(range(2000)) + list(range(BIG, BIG + 300))]
+    fp64 = precision == "fp64"
+    X = np.array([tune.feature_vector(np.array(r["features"]), fp64) for r in test], np.float64)
     Xn = (X - np.array(st["mu"])) / np.array(st["sd"])
     Y, _ = omlp.forward(np.array(st["params"], np.float64), Xn, st["n_in"], st["hidden"], st["n_out"])
     names = st["candidates"]
