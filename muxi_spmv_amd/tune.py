@@ -32,6 +32,7 @@ CANDIDATES: list[tuple[str, int]] = [
     ("csr5", 256), ("csr5", 512), ("csr5", 1024),
     ("adaptive", 512), ("adaptive", 1024), ("adaptive", 2048),
     ("xslab", (2 << 8) | 4), ("xslab", (4 << 8) | 4), ("xslab", (8 << 8) | 4),
+    ("xslab", (2 << 8) | 8), ("xslab", (4 << 8) | 8), ("xslab", (8 << 8) | 8),
     ("ell", 0),
 ]
 CAND_NAMES = [f"{k}:{p}" for k, p in CANDIDATES]
