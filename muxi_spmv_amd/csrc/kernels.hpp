@@ -63,6 +63,12 @@ spmv_status_t launch_csr_to_ell(const DevCsr& A, int width, int* d_col, VT* d_va
 template <typename VT, typename XT, typename AT, bool COMP>
 spmv_status_t launch_ell(int R, int C, int width, const int* d_col, const VT* d_val, const XT* x, XT* y, AT alpha,
                          AT beta, hipStream_t s);
+// the plan's layout: a row's slots grouped by four (kernels_ell.hip)
+template <typename VT>
+spmv_status_t launch_csr_to_ellg(const DevCsr& A, int width, int* d_col, VT* d_val, hipStream_t s);
+template <typename VT, typename XT, typename AT, bool COMP>
+spmv_status_t launch_ellg(int R, int C, int width, const int* d_col, const VT* d_val, const XT* x, XT* y, AT alpha,
+                          AT beta, hipStream_t s);
 
 // ---- SpMM over merge-path partitions (kernels_spmm.hip); carry holds P*k elements.
 int spmm_groups(int k);

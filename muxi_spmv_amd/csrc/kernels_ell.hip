@@ -6,6 +6,8 @@
 // padding is col = -1, value = 0; rows longer than the width are truncated.
 // Unlike the reference, the packing runs on the device from the CSR device
 // arrays, one lane per row, instead of a host loop plus blocking copies.
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace muxi {
@@ -122,13 +124,192 @@ spmv_status_t launch_ell(int R, int C, int width, const int* d_col, const VT* d_
     return SPMV_SUCCESS;
 }
 
+// ---- the plan's ELL layout: a row's slots grouped by four --------------------
+// Slots 4g .. 4g+3 of row r sit together at 4*g*R + 4*r, so a lane loads its
+// row's four columns with one 16-byte load (and its four values with one
+// vector load); after the n4 = width/4 groups come a pair (2 slots at
+// base2 + 2r, when width % 4 >= 2) and a single (base1 + r, when width is
+// odd). Lane-adjacent rows stay address-adjacent within every group, so the
+// x gathers of one slot still come from neighbouring rows.
+struct EllGroups {
+    int n4, two, one;
+    long R, base2, base1;
+};
+
+inline EllGroups ell_groups(int R, int width) {
+    EllGroups g;
+    g.R = R;
+    g.n4 = width / 4;
+    g.two = (width % 4) >= 2;
+    g.one = width % 2;
+    g.base2 = 4L * R * g.n4;
+    g.base1 = g.base2 + (g.two ? 2L * R : 0);
+    return g;
+}
+
+__device__ __forceinline__ long ellg_index(const EllGroups& g, long r, int i) {
+    if (i < 4 * g.n4) return 4L * g.R * (i / 4) + 4 * r + (i % 4);
+    if (g.two && i < 4 * g.n4 + 2) return g.base2 + 2 * r + (i - 4 * g.n4);
+    return g.base1 + r;
+}
+
+template <typename VT>
+__global__ void csr_to_ellg_kernel(int R, int width, EllGroups eg, const int* __restrict__ rp,
+                                   const int* __restrict__ ci, const VT* __restrict__ v, int* __restrict__ ecol,
+                                   VT* __restrict__ eval) {
+    const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= R) return;
+    const int b = rp[row], len = rp[row + 1] - b;
+    for (int i = 0; i < width; ++i) {
+        const long idx = ellg_index(eg, row, i);
+        if (i < len) {
+            ecol[idx] = ci[b + i];
+            eval[idx] = v[b + i];
+        } else {
+            ecol[idx] = -1;
+            VT z;
+            __builtin_memset(&z, 0, sizeof(VT));
+            eval[idx] = z;
+        }
+    }
+}
+
+template <typename VT>
+spmv_status_t launch_csr_to_ellg(const DevCsr& A, int width, int* d_col, VT* d_val, hipStream_t s) {
+    if (A.R <= 0 || width <= 0) return SPMV_SUCCESS;
+    hipLaunchKernelGGL(csr_to_ellg_kernel<VT>, dim3(grid_for(A.R, 256)), dim3(256), 0, s, A.R, width,
+                       ell_groups(A.R, width), A.rp, A.ci, (const VT*)A.v, d_col, d_val);
+    MUXI_LAUNCH_CHECK();
+    return SPMV_SUCCESS;
+}
+
+// N consecutive elements with one non-temporal vector load.
+template <typename T, int N>
+struct Vec {
+    T v[N];
+};
+template <typename T, int N>
+__device__ __forceinline__ Vec<T, N> ldv(const T* p) {
+    typedef T V __attribute__((ext_vector_type(N)));
+    const V w = __builtin_nontemporal_load((const V*)p);
+    Vec<T, N> q;
+#pragma unroll
+    for (int j = 0; j < N; ++j) q.v[j] = w[j];
+    return q;
+}
+template <int N>
+__device__ __forceinline__ Vec<bf16_t, N> ldv_bf16(const bf16_t* p) {
+    typedef uint16_t V __attribute__((ext_vector_type(N)));
+    const V w = __builtin_nontemporal_load((const V*)p);
+    Vec<bf16_t, N> q;
+#pragma unroll
+    for (int j = 0; j < N; ++j) q.v[j].bits = w[j];
+    return q;
+}
+template <typename T, int N>
+__device__ __forceinline__ Vec<T, N> ldvec(const T* p) {
+    if constexpr (std::is_same<T, bf16_t>::value) return ldv_bf16<N>(p);
+    else return ldv<T, N>(p);
+}
+
+// One lane per row over the grouped layout: per round two groups of four
+// slots (plus the pair and single in the last round), all column and value
+// loads first, then all gathers, then the products. Padding (col < 0) gathers
+// x[0] and adds nothing.
+template <typename VT, typename XT, typename AT, bool COMP>
+__global__ __launch_bounds__(256) void ellg_spmv_kernel(int R, int C, EllGroups eg, const int* __restrict__ ecol,
+                                                        const VT* __restrict__ eval, const XT* __restrict__ x,
+                                                        XT* __restrict__ y, AT alpha, AT beta) {
+    const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= R) return;
+    Acc<AT, COMP> a;
+    a.zero();
+    if (C > 0) {
+        for (int g = 0;; g += 2) {
+            const bool hasA = g < eg.n4, hasB = g + 1 < eg.n4, last = g + 2 >= eg.n4;
+            const bool has2 = last && eg.two, has1 = last && eg.one;
+            Vec<int, 4> cA{{-1, -1, -1, -1}}, cB{{-1, -1, -1, -1}};
+            Vec<VT, 4> vA{}, vB{};
+            Vec<int, 2> c2{{-1, -1}};
+            Vec<VT, 2> v2{};
+            int c1 = -1;
+            VT v1{};
+            if (hasA) {
+                cA = ldvec<int, 4>(ecol + 4L * eg.R * g + 4 * row);
+                vA = ldvec<VT, 4>(eval + 4L * eg.R * g + 4 * row);
+            }
+            if (hasB) {
+                cB = ldvec<int, 4>(ecol + 4L * eg.R * (g + 1) + 4 * row);
+                vB = ldvec<VT, 4>(eval + 4L * eg.R * (g + 1) + 4 * row);
+            }
+            if (has2) {
+                c2 = ldvec<int, 2>(ecol + eg.base2 + 2 * row);
+                v2 = ldvec<VT, 2>(eval + eg.base2 + 2 * row);
+            }
+            if (has1) {
+                c1 = ntl(ecol + eg.base1 + row);
+                v1 = ntl(eval + eg.base1 + row);
+            }
+            // gathers only for the segments present (uniform branches: no gather instruction is wasted)
+            XT xA[4] = {}, xB[4] = {}, x2[2] = {}, x1 = XT(0);
+            if (hasA) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) xA[j] = x[cA.v[j] > 0 ? cA.v[j] : 0];
+            }
+            if (hasB) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) xB[j] = x[cB.v[j] > 0 ? cB.v[j] : 0];
+            }
+            if (has2) {
+#pragma unroll
+                for (int j = 0; j < 2; ++j) x2[j] = x[c2.v[j] > 0 ? c2.v[j] : 0];
+            }
+            if (has1) x1 = x[c1 > 0 ? c1 : 0];
+            if (hasA) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (cA.v[j] >= 0) a.fma(vA.v[j], xA[j]);
+            }
+            if (hasB) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (cB.v[j] >= 0) a.fma(vB.v[j], xB[j]);
+            }
+            if (has2) {
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    if (c2.v[j] >= 0) a.fma(v2.v[j], x2[j]);
+            }
+            if (has1 && c1 >= 0) a.fma(v1, x1);
+            if (last) break;
+        }
+    }
+    store_y(y, row, a.value(), alpha, beta);
+}
+
+template <typename VT, typename XT, typename AT, bool COMP>
+spmv_status_t launch_ellg(int R, int C, int width, const int* d_col, const VT* d_val, const XT* x, XT* y, AT alpha,
+                          AT beta, hipStream_t s) {
+    if (R <= 0) return SPMV_SUCCESS;
+    hipLaunchKernelGGL((ellg_spmv_kernel<VT, XT, AT, COMP>), dim3(grid_for(R, 256)), dim3(256), 0, s, R, C,
+                       ell_groups(R, width), d_col, d_val, x, y, alpha, beta);
+    MUXI_LAUNCH_CHECK();
+    return SPMV_SUCCESS;
+}
+
+template spmv_status_t launch_csr_to_ellg<float>(const DevCsr&, int, int*, float*, hipStream_t);
+template spmv_status_t launch_csr_to_ellg<double>(const DevCsr&, int, int*, double*, hipStream_t);
+template spmv_status_t launch_csr_to_ellg<bf16_t>(const DevCsr&, int, int*, bf16_t*, hipStream_t);
+
 template spmv_status_t launch_csr_to_ell<float>(const DevCsr&, int, int*, float*, hipStream_t);
 template spmv_status_t launch_csr_to_ell<double>(const DevCsr&, int, int*, double*, hipStream_t);
 template spmv_status_t launch_csr_to_ell<bf16_t>(const DevCsr&, int, int*, bf16_t*, hipStream_t);
 
 #define INST(VT, XT, AT, COMP)                                                                                       \
     template spmv_status_t launch_ell<VT, XT, AT, COMP>(int, int, int, const int*, const VT*, const XT*, XT*, AT, AT, \
-                                                        hipStream_t);
+                                                        hipStream_t);                                                    \
+    template spmv_status_t launch_ellg<VT, XT, AT, COMP>(int, int, int, const int*, const VT*, const XT*, XT*, AT, AT, \
+                                                         hipStream_t);
 MUXI_FOR_EACH_COMBO(INST)
 #undef INST
 

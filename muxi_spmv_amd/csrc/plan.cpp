@@ -224,7 +224,7 @@ extern "C" spmv_status_t muxi_plan_create(const muxi_csr_t* Ain, spmv_type_t vt,
                 p->scratch_bytes = (sizeof(int) + vsize(vt)) * slots;
                 PLAN_TRY(with_types(vt, xt, p->comp, [&](auto tag) {
                     using T = decltype(tag);
-                    return launch_csr_to_ell<typename T::VT>(p->A, width, p->d_ecol, (typename T::VT*)p->d_eval, s);
+                    return launch_csr_to_ellg<typename T::VT>(p->A, width, p->d_ecol, (typename T::VT*)p->d_eval, s);
                 }));
             }
             break;
@@ -370,8 +370,8 @@ static spmv_status_t plan_exec(const muxi_plan* p, const void* x, void* y, doubl
                 return launch_csr5<VT, XT, AT, C>(p->A, p->param0, p->d_tile_row, p->T, p->d_carry, xx, yy, a, b, s,
                                                   ev1);
             case MUXI_KERNEL_ELL:
-                return launch_ell<VT, XT, AT, C>(p->A.R, p->A.C, p->width, p->d_ecol, (const VT*)p->d_eval, xx, yy, a, b,
-                                                 s);
+                return launch_ellg<VT, XT, AT, C>(p->A.R, p->A.C, p->width, p->d_ecol, (const VT*)p->d_eval, xx, yy, a,
+                                                  b, s);
             case MUXI_KERNEL_XSLAB:
                 if constexpr (C) {
                     set_error("xslab: no compensated variant");
