@@ -109,11 +109,12 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
 
 
 CPP_TESTS = [ROOT / "tests" / "cpp" / "test_spmv_known_answer.cpp", ROOT / "tests" / "cpp" / "test_formats_io.cpp"]
-PROBES = [PKG / "probe" / "gather_probe.hip"]
+PROBES = [PKG / "probe" / "gather_probe.hip", PKG / "probe" / "stream_probe.hip"]
 
 
 def build_probes(verbose: bool = True) -> list[Path]:
-    """Stand-alone measurement programs (the random-gather rate that bounds the scattered-column SpMV)."""
+    """Stand-alone measurement programs (the random-gather rate that bounds the scattered-column SpMV, the HBM
+    streaming rate that bounds the stencil SpMV)."""
     out = []
     for src in PROBES:
         exe = src.with_suffix("")
