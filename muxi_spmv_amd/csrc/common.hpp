@@ -11,6 +11,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <exception>
+#include <new>
 #include <string>
 
 #include "muxi_spmv/muxi_spmv.h"
@@ -185,6 +187,22 @@ const char* get_error();
             return SPMV_ERROR_DEVICE;                                              \
         }                                                                          \
     } while (0)
+
+// Runs f() and turns a C++ exception (host containers on absurd sizes) into a
+// status, so that none crosses the C ABI. An object half built when the
+// exception came is leaked rather than freed on an unknown path.
+template <typename F>
+spmv_status_t no_throw(const char* where, F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        set_error(std::string(where) + ": out of host memory");
+        return SPMV_ERROR_MEMORY_ALLOC;
+    } catch (const std::exception& e) {
+        set_error(std::string(where) + ": " + e.what());
+        return SPMV_ERROR_INTERNAL;
+    }
+}
 
 inline unsigned grid_for(long work, int per_block) {
     return (unsigned)((work + per_block - 1) / per_block);

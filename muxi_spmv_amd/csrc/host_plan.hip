@@ -88,7 +88,7 @@ static void host_plan_free(muxi_host_plan* h) {
         }                              \
     } while (0)
 
-extern "C" spmv_status_t muxi_host_plan_create(const muxi_csr_t* A, spmv_type_t vt, spmv_type_t xt, int chunks,
+static spmv_status_t host_plan_create_impl(const muxi_csr_t* A, spmv_type_t vt, spmv_type_t xt, int chunks,
                                                muxi_kernel_t kernel, int param0, muxi_host_plan** out) {
     if (!out) {
         set_error("host plan output pointer is NULL");
@@ -299,4 +299,9 @@ extern "C" spmv_status_t muxi_host_alloc(void** ptr, size_t bytes) {
 extern "C" spmv_status_t muxi_host_free(void* ptr) {
     if (ptr) MUXI_HIP(hipHostFree(ptr));
     return SPMV_SUCCESS;
+}
+
+extern "C" spmv_status_t muxi_host_plan_create(const muxi_csr_t* A, spmv_type_t vt, spmv_type_t xt, int chunks,
+                                               muxi_kernel_t kernel, int param0, muxi_host_plan** out) {
+    return no_throw("muxi_host_plan_create", [&] { return host_plan_create_impl(A, vt, xt, chunks, kernel, param0, out); });
 }

@@ -155,7 +155,7 @@ static void batch_free(muxi_batch_plan* p) {
         }                                                                 \
     } while (0)
 
-extern "C" spmv_status_t muxi_batch_plan_create(const muxi_csr_t* mats, int count, spmv_type_t vt, spmv_type_t xt,
+static spmv_status_t batch_plan_create_impl(const muxi_csr_t* mats, int count, spmv_type_t vt, spmv_type_t xt,
                                                 const void* const* xs, void* const* ys, int ipt, void* stream,
                                                 muxi_batch_plan** out) {
     if (!out) return SPMV_ERROR_INVALID_HANDLE;
@@ -365,4 +365,11 @@ extern "C" spmv_status_t muxi_graph_destroy(muxi_graph* g) {
     (void)hipGraphDestroy(g->g);
     delete g;
     return SPMV_SUCCESS;
+}
+
+extern "C" spmv_status_t muxi_batch_plan_create(const muxi_csr_t* mats, int count, spmv_type_t vt, spmv_type_t xt,
+                                                const void* const* xs, void* const* ys, int ipt, void* stream,
+                                                muxi_batch_plan** out) {
+    return no_throw("muxi_batch_plan_create",
+                    [&] { return batch_plan_create_impl(mats, count, vt, xt, xs, ys, ipt, stream, out); });
 }

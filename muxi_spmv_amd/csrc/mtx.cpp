@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <exception>
+#include <new>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -101,9 +103,11 @@ spmv_status_t read_csr(const char* name, int expand, muxi_csr_t* out) {
     const bool mirror = h.isSymmetric && expand;
     std::vector<int> ri, cj;
     std::vector<T> vv;
-    ri.reserve((size_t)h.nnz * (mirror ? 2 : 1));
-    cj.reserve(ri.capacity());
-    vv.reserve(ri.capacity());
+    // reserve what the header declares, up to 2^26 entries (more grow as they are read)
+    const size_t want = std::min<size_t>((size_t)h.nnz * (mirror ? 2 : 1), (size_t)1 << 26);
+    ri.reserve(want);
+    cj.reserve(want);
+    vv.reserve(want);
     for (int e = 0; e < h.nnz; ++e) {
         int r, c;
         double v = 1.0;  // pattern entries are 1
@@ -270,17 +274,32 @@ extern "C" spmv_status_t muxi_mtx_read_header(const char* filename, muxi_mtx_hea
     return read_head(file.f, filename, header);
 }
 
+// Host containers may throw (a file that declares billions of entries); no
+// exception crosses the C ABI.
+template <typename F>
+static spmv_status_t guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        set_error("mtx: out of host memory");
+        return SPMV_ERROR_MEMORY_ALLOC;
+    } catch (const std::exception& e) {
+        set_error(std::string("mtx: ") + e.what());
+        return SPMV_ERROR_INTERNAL;
+    }
+}
+
 extern "C" spmv_status_t muxi_mtx_read_csr_f32(const char* filename, int expandSymmetric, muxi_csr_t* out) {
-    return read_csr<float>(filename, expandSymmetric, out);
+    return guarded([&] { return read_csr<float>(filename, expandSymmetric, out); });
 }
 extern "C" spmv_status_t muxi_mtx_read_csr_f64(const char* filename, int expandSymmetric, muxi_csr_t* out) {
-    return read_csr<double>(filename, expandSymmetric, out);
+    return guarded([&] { return read_csr<double>(filename, expandSymmetric, out); });
 }
 extern "C" spmv_status_t muxi_mtx_write_csr_f32(const char* filename, const muxi_csr_t* A, int writeSymmetric) {
-    return write_csr<float>(filename, A, writeSymmetric);
+    return guarded([&] { return write_csr<float>(filename, A, writeSymmetric); });
 }
 extern "C" spmv_status_t muxi_mtx_write_csr_f64(const char* filename, const muxi_csr_t* A, int writeSymmetric) {
-    return write_csr<double>(filename, A, writeSymmetric);
+    return guarded([&] { return write_csr<double>(filename, A, writeSymmetric); });
 }
 
 extern "C" spmv_status_t muxi_csr_free_host(muxi_csr_t* A) {
@@ -296,10 +315,10 @@ extern "C" spmv_status_t muxi_csr_free_host(muxi_csr_t* A) {
 }
 
 extern "C" spmv_status_t muxi_mtx_read_coo_f32(const char* filename, muxi_coo_t* out) {
-    return read_coo<float>(filename, out);
+    return guarded([&] { return read_coo<float>(filename, out); });
 }
 extern "C" spmv_status_t muxi_mtx_read_coo_f64(const char* filename, muxi_coo_t* out) {
-    return read_coo<double>(filename, out);
+    return guarded([&] { return read_coo<double>(filename, out); });
 }
 extern "C" spmv_status_t muxi_mtx_write_coo_f32(const char* filename, const muxi_coo_t* A) {
     return write_coo<float>(filename, A);

@@ -47,7 +47,7 @@ static void plan_free(muxi_plan* p) {
 
 static bool pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; }
 
-extern "C" spmv_status_t muxi_plan_create(const muxi_csr_t* Ain, spmv_type_t vt, spmv_type_t xt, muxi_kernel_t kernel,
+static spmv_status_t plan_create_impl(const muxi_csr_t* Ain, spmv_type_t vt, spmv_type_t xt, muxi_kernel_t kernel,
                                           int param0, int compensated, void* stream, muxi_plan** out) {
     if (!out) {
         set_error("plan output pointer is NULL");
@@ -238,7 +238,7 @@ extern "C" spmv_status_t muxi_plan_create(const muxi_csr_t* Ain, spmv_type_t vt,
     return SPMV_SUCCESS;
 }
 
-extern "C" spmv_status_t muxi_plan_create_spmm(const muxi_csr_t* Ain, spmv_type_t vt, spmv_type_t xt, int nrhs,
+static spmv_status_t plan_create_spmm_impl(const muxi_csr_t* Ain, spmv_type_t vt, spmv_type_t xt, int nrhs,
                                                int ipt, void* stream, muxi_plan** out) {
     if (!out) {
         set_error("plan output pointer is NULL");
@@ -428,4 +428,14 @@ extern "C" spmv_status_t muxi_event_elapsed_ms(void* start, void* end, float* ms
     MUXI_HIP(hipEventSynchronize((hipEvent_t)end));
     MUXI_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
     return SPMV_SUCCESS;
+}
+
+extern "C" spmv_status_t muxi_plan_create(const muxi_csr_t* Ain, spmv_type_t vt, spmv_type_t xt, muxi_kernel_t kernel, int param0,
+                                          int compensated, void* stream, muxi_plan** out) {
+    return no_throw("muxi_plan_create", [&] { return plan_create_impl(Ain, vt, xt, kernel, param0, compensated, stream, out); });
+}
+
+extern "C" spmv_status_t muxi_plan_create_spmm(const muxi_csr_t* Ain, spmv_type_t vt, spmv_type_t xt, int nrhs, int ipt,
+                                               void* stream, muxi_plan** out) {
+    return no_throw("muxi_plan_create_spmm", [&] { return plan_create_spmm_impl(Ain, vt, xt, nrhs, ipt, stream, out); });
 }

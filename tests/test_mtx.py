@@ -179,3 +179,10 @@ def test_coo_write_order_and_round_trip(tmp_path):
     assert np.array_equal(v2, v[order])
     with pytest.raises(SpmvError, match="INVALID_MATRIX"):
         mtx.write_coo(tmp_path / "bad.mtx", [3], [0], np.ones(1), 3, 4)
+
+
+def test_huge_declared_size_is_an_error_not_a_crash(tmp_path):
+    # the header claims 2e9 entries; the file holds one
+    p = _put(tmp_path, "h.mtx", "%%MatrixMarket matrix coordinate real general\n100 100 2000000000\n1 1 1\n")
+    with pytest.raises(SpmvError, match="INVALID_MATRIX"):
+        mtx.read(p)
