@@ -15,7 +15,9 @@ configuration.
 Prints one JSON line: value = algorithmic GB/s over the K timed steps
 (bytes = N*(4+4) + 4*(R+1) + 4*C + 4*R per step); roofline = the main
 kernel's average duration from HIP events recorded around it in every
-timed step; cpu_baseline = the oracle's FP32 CSR loop on the host cores.
+fourth timed step (a sample spread over the timed region; markers on every
+step would add their own cost to a short step); cpu_baseline = the oracle's
+FP32 CSR loop on the host cores.
 """
 from __future__ import annotations
 
@@ -31,7 +33,8 @@ sys.path.insert(0, str(ROOT))
 
 CANDIDATES = [("merge", 4), ("merge", 8), ("merge", 16), ("csr5", 256), ("csr5", 512), ("csr5", 1024),
               ("adaptive", 512), ("adaptive", 1024), ("adaptive", 2048), ("ell", 0),
-              ("xslab", (2 << 8) | 4), ("xslab", (4 << 8) | 4), ("xslab", (8 << 8) | 4), ("xslab", (4 << 8) | 8),
+              ("xslab", (2 << 8) | 4), ("xslab", (4 << 8) | 4), ("xslab", (8 << 8) | 4),
+              ("xslab", (2 << 8) | 8), ("xslab", (4 << 8) | 8), ("xslab", (8 << 8) | 8),
               ("vwave", 4), ("vwave", 8), ("vector", 0), ("scalar", 0)]
 
 
