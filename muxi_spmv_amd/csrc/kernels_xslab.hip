@@ -250,6 +250,27 @@ __global__ __launch_bounds__(256) void xslab_combine_kernel(int R, const int* __
     store_y(y, r, s, alpha, beta);
 }
 
+// Combine for plans of at most four slabs: a row has at most four pairs, so
+// their P slots sit in one 16-byte entry per row (-1 = none): one vector load,
+// then the four partial loads together (no row-offset lookup in between).
+template <typename XT, typename AT>
+__global__ __launch_bounds__(256) void xslab_combine4_kernel(int R, const int* __restrict__ ploc4,
+                                                             const AT* __restrict__ P, XT* __restrict__ y, AT alpha,
+                                                             AT beta) {
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    typedef int I4 __attribute__((ext_vector_type(4)));
+    const I4 q = __builtin_nontemporal_load((const I4*)ploc4 + r);
+    AT v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = ntl(P + (q[k] >= 0 ? q[k] : 0));
+    AT s = AT(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (q[k] >= 0) s += v[k];
+    store_y(y, r, s, alpha, beta);
+}
+
 // ---- plan construction
 // Slab q holds columns [cut[q], cut[q+1]). The cuts split the nonzeros into S
 // near-equal parts (column quantiles), so every slab, and so every XCD, gets
@@ -327,7 +348,8 @@ __global__ __launch_bounds__(256) void xslab_fill_kernel(const int* __restrict__
                                                          const int* __restrict__ nnz_base,
                                                          const int* __restrict__ pad_base,
                                                          const int* __restrict__ rpp, int* __restrict__ srp,
-                                                         int* __restrict__ ploc, int* __restrict__ sci,
+                                                         int* __restrict__ ploc, int* __restrict__ ploc4,
+                                                         int* __restrict__ sci,
                                                          VT* __restrict__ sv) {
     const long r = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
@@ -340,7 +362,9 @@ __global__ __launch_bounds__(256) void xslab_fill_kernel(const int* __restrict__
         if (lane < S) off = pos[k];
         if (has) {
             const int lr = lrow[k];
-            ploc[rpp[r] + __popcll(hm & __lanemask_lt())] = lr;  // the row's pairs in slab order
+            const int rank = __popcll(hm & __lanemask_lt());
+            ploc[rpp[r] + rank] = lr;  // the row's pairs in slab order
+            if (ploc4) ploc4[r * 4 + rank] = lr;
             srp[lr + lane] = off - nnz_base[lane];
         }
     }
@@ -517,12 +541,16 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
     for (int q = 0; q < S; ++q)
         MUXI_HIP(hipMemcpyAsync(p->d_slab_rp + sa.row_base[q] + q + sa.rows[q], &term[q], sizeof(int),
                                 hipMemcpyHostToDevice, s));
+    if (S <= 4) {  // a row has at most S pairs: their slots in one 16-byte entry per row
+        MUXI_HIP(hipMalloc(&p->d_slab_ploc4, sizeof(int) * 4 * (size_t)R));
+        MUXI_HIP(hipMemsetAsync(p->d_slab_ploc4, 0xff, sizeof(int) * 4 * (size_t)R, s));
+    }
     st = with_types(vt, p->xtype, p->comp, [&](auto tag) -> spmv_status_t {
         using T = decltype(tag);
         using VT = typename T::VT;
         hipLaunchKernelGGL(xslab_fill_kernel<VT>, dim3(grid_for((long)R * 64, 256)), dim3(256), 0, s, p->A.rp, p->A.ci,
                            (const VT*)p->A.v, R, cut, S, cnt, pos, lrow, d_nb, d_nb + S, p->d_slab_rpp, p->d_slab_rp,
-                           p->d_slab_ploc, p->d_slab_ci, (VT*)p->d_slab_v);
+                           p->d_slab_ploc, p->d_slab_ploc4, p->d_slab_ci, (VT*)p->d_slab_v);
         MUXI_LAUNCH_CHECK();
         for (int q = 0; q < S; ++q) {
             const long from = (long)hpad[q] + sa.nnz[q], to = (long)hpad[q] + (long)sa.tiles[q] * SIG;
@@ -554,7 +582,8 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
     for (int* t : {cnt, flag, pos, lrow, npair, d_nb}) (void)hipFreeAsync(t, s);
     MUXI_HIP(hipStreamSynchronize(s));
     p->scratch_bytes = sizeof(int) * ints + vs * (size_t)tp + 4 * nflag + 2 * sizeof(int) * (size_t)(ntiles + 1) +
-                       acc * (size_t)(ncarry + total_rows) + sizeof(int) * ((size_t)ncarry + R + 1);
+                       acc * (size_t)(ncarry + total_rows) + sizeof(int) * ((size_t)ncarry + R + 1) +
+                       (p->d_slab_ploc4 ? sizeof(int) * 4 * (size_t)R : 0);
     return SPMV_SUCCESS;
 }
 
@@ -579,8 +608,12 @@ spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT 
     } else if (ev_end) {
         MUXI_HIP(hipEventRecord(ev_end, s));
     }
-    hipLaunchKernelGGL((xslab_combine_kernel<XT, AT>), dim3(grid_for(p->A.R, 256)), dim3(256), 0, s, p->A.R,
-                       p->d_slab_rpp, p->d_slab_ploc, P, y, alpha, beta);
+    if (p->d_slab_ploc4)
+        hipLaunchKernelGGL((xslab_combine4_kernel<XT, AT>), dim3(grid_for(p->A.R, 256)), dim3(256), 0, s, p->A.R,
+                           p->d_slab_ploc4, P, y, alpha, beta);
+    else
+        hipLaunchKernelGGL((xslab_combine_kernel<XT, AT>), dim3(grid_for(p->A.R, 256)), dim3(256), 0, s, p->A.R,
+                           p->d_slab_rpp, p->d_slab_ploc, P, y, alpha, beta);
     MUXI_LAUNCH_CHECK();
     return SPMV_SUCCESS;
 }

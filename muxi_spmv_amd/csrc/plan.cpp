@@ -23,6 +23,7 @@ static void plan_free(muxi_plan* p) {
     (void)hipFree(p->d_slab);
     (void)hipFree(p->d_slab_rpp);
     (void)hipFree(p->d_slab_ckey);
+    (void)hipFree(p->d_slab_ploc4);
     delete p;
 }
 

@@ -53,6 +53,7 @@ struct muxi_plan {
     int* d_slab = nullptr;
     int* d_slab_rp = nullptr;
     int* d_slab_ploc = nullptr;  // P slot of each pair, pairs in row-major order
+    int* d_slab_ploc4 = nullptr; // with at most 4 slabs: the P slots of row r at [4r, 4r+4), -1 = none
     int* d_slab_ci = nullptr;
     void* d_slab_v = nullptr;
     uint32_t* d_slab_flags = nullptr;
