@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void xslab_kernel(SlabArgs sa, const int* __re
                                                     const uint32_t* __restrict__ sflags,
                                                     const int* __restrict__ tile_row, const XT* __restrict__ x,
                                                     AT* __restrict__ P, AT* __restrict__ carry,
-                                                    int* __restrict__ carry_key) {
+                                                    int* __restrict__ carry_key, AT* __restrict__ PC) {
     constexpr int SIG = 64 * EPL, NW = SIG / 32;  // nonzeros and flag words per tile
     const int s = blockIdx.x % sa.S;
     const int t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x / sa.S) * 4 + (int)(threadIdx.x >> 6));
@@ -129,7 +129,8 @@ __global__ __launch_bounds__(256) void xslab_kernel(SlabArgs sa, const int* __re
     uint32_t w[NW];
 #pragma unroll
     for (int i = 0; i < NW; ++i) w[i] = fw[i];  // the tile's flags: uniform, scalar loads
-    const int row0 = tile_row[sa.tile_base[s] + t];
+    const int desc = tile_row[sa.tile_base[s] + t];
+    const int row0 = desc & 0x7fffffff;  // bit 31: the open row's only carry goes to its PC slot
     XT xv[EPL];
 #pragma unroll
     for (int j = 0; j < EPL; ++j) xv[j] = x[c[j]];
@@ -181,6 +182,7 @@ __global__ __launch_bounds__(256) void xslab_kernel(SlabArgs sa, const int* __re
         const bool open = orow < sa.rows[s] && !((f >> (EPL - 1)) & 1u);
         carry[sa.tile_base[s] + t] = open ? sc : AT(0);
         carry_key[sa.tile_base[s] + t] = open ? sa.row_base[s] + orow : -1;
+        if (desc < 0 && open) PC[sa.row_base[s] + orow] = sc;
     }
 }
 
@@ -250,25 +252,117 @@ __global__ __launch_bounds__(256) void xslab_combine_kernel(int R, const int* __
     store_y(y, r, s, alpha, beta);
 }
 
-// Combine for plans of at most four slabs: a row has at most four pairs, so
-// their P slots sit in one 16-byte entry per row (-1 = none): one vector load,
-// then the four partial loads together (no row-offset lookup in between).
+// Combine for plans of at most four slabs, with no fix-up launch before it.
+// A row has at most four pairs, so their P slots sit in one 16-byte entry per
+// row (-1 = none): one vector load, then the partial loads together. A pair
+// whose row crosses exactly one tile end has that carry in PC[slot] (the slab
+// kernel writes it there; PC is zero for every other slot), so it is
+// P[slot] + PC[slot]. A row with a pair that crosses several tile ends is
+// "heavy": its first entry is INT_MIN (the entry itself is in hfirst), the
+// crossing pair is -(j + 2) with ct[3j..3j+2] = (slot, t0, t1), and one of the
+// first workgroups' wavefronts sums carry[t0 .. t1) 64 at a time.
 template <typename XT, typename AT>
 __global__ __launch_bounds__(256) void xslab_combine4_kernel(int R, const int* __restrict__ ploc4,
-                                                             const AT* __restrict__ P, XT* __restrict__ y, AT alpha,
-                                                             AT beta) {
-    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+                                                             const AT* __restrict__ P, const AT* __restrict__ PC,
+                                                             const int* __restrict__ ct, const AT* __restrict__ carry,
+                                                             int H, const int* __restrict__ hrow,
+                                                             const int* __restrict__ hfirst, XT* __restrict__ y,
+                                                             AT alpha, AT beta) {
+    const int HB = (H + 3) / 4;
+    if ((int)blockIdx.x < HB) {  // heavy rows: one wavefront each
+        const int h = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+        if (h >= H) return;  // whole wavefront
+        const int r = hrow[h];
+        AT s = AT(0);
+        for (int k = 0; k < 4; ++k) {
+            const int q = k == 0 ? hfirst[h] : ploc4[4L * r + k];
+            if (q == -1) break;  // no more pairs
+            if (q >= 0) {
+                s += P[q] + PC[q];
+            } else {
+                const int j = -q - 2, t0 = ct[3 * j + 1], t1 = ct[3 * j + 2];
+                AT c = AT(0);
+                for (int t = t0 + lane; t < t1; t += 64) c += carry[t];
+#pragma unroll
+                for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m);
+                s += c + P[ct[3 * j]];
+            }
+        }
+        if (lane == 0) store_y(y, r, s, alpha, beta);
+        return;
+    }
+    const long r = (long)(blockIdx.x - HB) * blockDim.x + threadIdx.x;
     if (r >= R) return;
     typedef int I4 __attribute__((ext_vector_type(4)));
     const I4 q = __builtin_nontemporal_load((const I4*)ploc4 + r);
-    AT v[4];
+    if (q[0] == INT_MIN) return;  // a heavy row
+    AT v[4], c[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = ntl(P + (q[k] >= 0 ? q[k] : 0));
+    for (int k = 0; k < 4; ++k) {
+        v[k] = ntl(P + (q[k] >= 0 ? q[k] : 0));
+        c[k] = PC[q[k] >= 0 ? q[k] : 0];
+    }
     AT s = AT(0);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        if (q[k] >= 0) s += v[k];
+        if (q[k] >= 0) s += v[k] + c[k];
     store_y(y, r, s, alpha, beta);
+}
+
+// The pair open at the end of each tile of one slab (-1 = none): tile t ends
+// inside slab row r = tile_row[t + 1] when that row starts before the next
+// tile does; the slab's last tile ends its last row.
+__global__ void xslab_carry_keys_kernel(const int* __restrict__ srp, const int* __restrict__ tile_row, int tiles,
+                                        int SIG, int row_base, int* __restrict__ keys) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tiles) return;
+    int k = -1;
+    if (t + 1 < tiles) {
+        const int r = tile_row[t + 1];
+        if (srp[r] < (t + 1) * SIG) k = row_base + r;
+    }
+    keys[t] = k;
+}
+
+// Marks the descriptors of tiles whose carry is their open row's only one.
+__global__ void xslab_mark_single_kernel(const int* __restrict__ tiles, int n, int* __restrict__ tile_row) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) tile_row[tiles[i]] |= (int)0x80000000u;
+}
+
+__global__ void xslab_cross_id_kernel(const int* __restrict__ ct, long nct, int* __restrict__ cross_id) {
+    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nct) cross_id[ct[3 * j]] = (int)j;
+}
+
+// ploc4 entries of pairs crossing several tile ends become -(j + 2); their rows get hflag = 1.
+__global__ void xslab_mark_heavy_kernel(int R, int* __restrict__ ploc4, const int* __restrict__ cross_id,
+                                        int* __restrict__ hflag) {
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    int heavy = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int q = ploc4[4 * r + k];
+        if (q < 0) continue;
+        const int j = cross_id[q];
+        if (j >= 0) {
+            ploc4[4 * r + k] = -(j + 2);
+            heavy = 1;
+        }
+    }
+    hflag[r] = heavy;
+}
+
+// Heavy row h = hrank[r]: hrow[h] = r, its first ploc4 entry moves to hfirst[h].
+__global__ void xslab_heavy_rows_kernel(int R, const int* __restrict__ hflag, const int* __restrict__ hrank,
+                                        int* __restrict__ ploc4, int* __restrict__ hrow, int* __restrict__ hfirst) {
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R || !hflag[r]) return;
+    const int h = hrank[r];
+    hrow[h] = (int)r;
+    hfirst[h] = ploc4[4 * r];
+    ploc4[4 * r] = INT_MIN;
 }
 
 // ---- plan construction
@@ -422,7 +516,8 @@ static spmv_status_t exscan(const int* in, int* out, long n, hipStream_t s) {
 }
 
 template <typename VT, typename XT, typename AT>
-using XslabFn = void (*)(SlabArgs, const int*, const VT*, const uint32_t*, const int*, const XT*, AT*, AT*, int*);
+using XslabFn = void (*)(SlabArgs, const int*, const VT*, const uint32_t*, const int*, const XT*, AT*, AT*, int*,
+                         AT*);
 
 template <typename VT, typename XT, typename AT>
 static XslabFn<VT, XT, AT> xslab_fn(int epl) {
@@ -567,7 +662,8 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
     const size_t acc = p->xtype == SPMV_TYPE_DOUBLE ? 8 : 4;
     MUXI_HIP(hipMalloc(&p->d_tile_row, 2 * sizeof(int) * (size_t)(ntiles + 1)));  // descriptors, then offsets
     p->d_tile_off = p->d_tile_row + ntiles + 1;
-    MUXI_HIP(hipMalloc(&p->d_carry, acc * (size_t)(ncarry + total_rows + 1)));  // carries, then P
+    // carries, then P; one spare element so that P[0] exists even with no pairs (the combines' clamped loads)
+    MUXI_HIP(hipMalloc(&p->d_carry, acc * (size_t)(ncarry + total_rows + 2)));
     MUXI_HIP(hipMalloc(&p->d_slab_ckey, sizeof(int) * (size_t)(ncarry + 1)));
     for (int q = 0; q < S; ++q) {
         if (sa.tiles[q] == 0) continue;
@@ -578,6 +674,84 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
         st = launch_csr5_tiles(srp_q, sa.rows[q], sa.nnz[q], SIG, sa.tiles[q], p->d_tile_row + sa.tile_base[q],
                                p->d_tile_off + sa.tile_base[q], s);
         if (st != SPMV_SUCCESS) return st;
+    }
+    if (p->d_slab_ploc4 && ncarry > 0) {
+        // Carries without a fix-up launch: a tile whose open row crosses only this tile end writes the carry to
+        // the row's PC slot (descriptor bit 31); a pair crossing several tile ends gets a row (slot, t0, t1) of
+        // ct and its row is summed by a wavefront in the combine.
+        MUXI_HIP(hipMalloc(&p->d_slab_pc, acc * ((size_t)total_rows + 1)));
+        MUXI_HIP(hipMemsetAsync(p->d_slab_pc, 0, acc * ((size_t)total_rows + 1), s));
+        int* keys = nullptr;
+        MUXI_HIP(hipMallocAsync(&keys, sizeof(int) * (size_t)(ncarry + 1), s));
+        for (int q = 0; q < S; ++q) {
+            if (sa.tiles[q] == 0) continue;
+            hipLaunchKernelGGL(xslab_carry_keys_kernel, dim3(grid_for(sa.tiles[q], 256)), dim3(256), 0, s,
+                               p->d_slab_rp + sa.row_base[q] + q, p->d_tile_row + sa.tile_base[q], sa.tiles[q], SIG,
+                               sa.row_base[q], keys + sa.tile_base[q]);
+            MUXI_LAUNCH_CHECK();
+        }
+        std::vector<int> hk((size_t)ncarry);
+        MUXI_HIP(hipMemcpyAsync(hk.data(), keys, sizeof(int) * ncarry, hipMemcpyDeviceToHost, s));
+        MUXI_HIP(hipStreamSynchronize(s));
+        (void)hipFreeAsync(keys, s);
+        std::vector<int> single, ct;
+        for (long t = 0; t < ncarry;) {
+            if (hk[t] < 0) {
+                ++t;
+                continue;
+            }
+            long u = t;
+            while (u + 1 < ncarry && hk[u + 1] == hk[t]) ++u;
+            if (u == t) {
+                single.push_back((int)t);
+            } else {
+                ct.push_back(hk[t]);
+                ct.push_back((int)t);
+                ct.push_back((int)(u + 1));
+            }
+            t = u + 1;
+        }
+        if (!single.empty()) {
+            int* d_single = nullptr;
+            MUXI_HIP(hipMallocAsync(&d_single, sizeof(int) * single.size(), s));
+            MUXI_HIP(hipMemcpyAsync(d_single, single.data(), sizeof(int) * single.size(), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(xslab_mark_single_kernel, dim3(grid_for((long)single.size(), 256)), dim3(256), 0, s,
+                               d_single, (int)single.size(), p->d_tile_row);
+            MUXI_LAUNCH_CHECK();
+            MUXI_HIP(hipStreamSynchronize(s));
+            (void)hipFree(d_single);
+        }
+        const long nct = (long)ct.size() / 3;
+        p->nheavy = 0;
+        if (nct > 0) {
+            MUXI_HIP(hipMalloc(&p->d_slab_ct, sizeof(int) * ct.size()));
+            MUXI_HIP(hipMemcpyAsync(p->d_slab_ct, ct.data(), sizeof(int) * ct.size(), hipMemcpyHostToDevice, s));
+            int *cross_id = nullptr, *hflag = nullptr, *hrank = nullptr;
+            MUXI_HIP(hipMallocAsync(&cross_id, sizeof(int) * ((size_t)total_rows + 1), s));
+            MUXI_HIP(hipMallocAsync(&hflag, sizeof(int) * ((size_t)R + 1), s));
+            MUXI_HIP(hipMallocAsync(&hrank, sizeof(int) * ((size_t)R + 1), s));
+            MUXI_HIP(hipMemsetAsync(cross_id, 0xff, sizeof(int) * ((size_t)total_rows + 1), s));
+            MUXI_HIP(hipMemsetAsync(hflag + R, 0, sizeof(int), s));
+            hipLaunchKernelGGL(xslab_cross_id_kernel, dim3(grid_for(nct, 256)), dim3(256), 0, s, p->d_slab_ct, nct,
+                               cross_id);
+            MUXI_LAUNCH_CHECK();
+            hipLaunchKernelGGL(xslab_mark_heavy_kernel, dim3(grid_for(R, 256)), dim3(256), 0, s, R,
+                               p->d_slab_ploc4, cross_id, hflag);
+            MUXI_LAUNCH_CHECK();
+            st = exscan(hflag, hrank, (long)R + 1, s);
+            if (st != SPMV_SUCCESS) return st;
+            int H = 0;
+            MUXI_HIP(hipMemcpyAsync(&H, hrank + R, sizeof(int), hipMemcpyDeviceToHost, s));
+            MUXI_HIP(hipStreamSynchronize(s));
+            if (H > 0) {
+                MUXI_HIP(hipMalloc(&p->d_slab_heavy, sizeof(int) * 2 * (size_t)H));
+                hipLaunchKernelGGL(xslab_heavy_rows_kernel, dim3(grid_for(R, 256)), dim3(256), 0, s, R, hflag, hrank,
+                                   p->d_slab_ploc4, p->d_slab_heavy, p->d_slab_heavy + H);
+                MUXI_LAUNCH_CHECK();
+            }
+            p->nheavy = H;
+            for (int* t : {cross_id, hflag, hrank}) (void)hipFreeAsync(t, s);
+        }
     }
     for (int* t : {cnt, flag, pos, lrow, npair, d_nb}) (void)hipFreeAsync(t, s);
     MUXI_HIP(hipStreamSynchronize(s));
@@ -599,21 +773,28 @@ spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT 
         const dim3 grid((unsigned)(((long)p->P + 3) / 4 * sa.S));  // four tiles per workgroup
         const XslabFn<VT, XT, AT> fn = xslab_fn<VT, XT, AT>(p->param0 & 0xff);
         hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, sa, p->d_slab_ci, (const VT*)p->d_slab_v, p->d_slab_flags,
-                           p->d_tile_row, x, P, carry, p->d_slab_ckey);
+                           p->d_tile_row, x, P, carry, p->d_slab_ckey, (AT*)p->d_slab_pc);
         MUXI_LAUNCH_CHECK();
         if (ev_end) MUXI_HIP(hipEventRecord(ev_end, s));
-        hipLaunchKernelGGL(keyed_fixup_kernel<AT>, dim3(grid_for(ncarry, 256)), dim3(256), 0, s, p->d_slab_ckey,
-                           ncarry, carry, P);
-        MUXI_LAUNCH_CHECK();
+        if (!p->d_slab_ploc4) {
+            hipLaunchKernelGGL(keyed_fixup_kernel<AT>, dim3(grid_for(ncarry, 256)), dim3(256), 0, s, p->d_slab_ckey,
+                               ncarry, carry, P);
+            MUXI_LAUNCH_CHECK();
+        }
     } else if (ev_end) {
         MUXI_HIP(hipEventRecord(ev_end, s));
     }
-    if (p->d_slab_ploc4)
-        hipLaunchKernelGGL((xslab_combine4_kernel<XT, AT>), dim3(grid_for(p->A.R, 256)), dim3(256), 0, s, p->A.R,
-                           p->d_slab_ploc4, P, y, alpha, beta);
-    else
+    if (p->d_slab_ploc4) {
+        const int H = p->nheavy;
+        // with no tiles there are no pairs: PC is never read then (every ploc4 entry is -1)
+        const AT* PC = p->d_slab_pc ? (const AT*)p->d_slab_pc : P;
+        hipLaunchKernelGGL((xslab_combine4_kernel<XT, AT>), dim3((H + 3) / 4 + grid_for(p->A.R, 256)), dim3(256), 0,
+                           s, p->A.R, p->d_slab_ploc4, P, PC, p->d_slab_ct, carry, H, p->d_slab_heavy,
+                           p->d_slab_heavy + H, y, alpha, beta);
+    } else {
         hipLaunchKernelGGL((xslab_combine_kernel<XT, AT>), dim3(grid_for(p->A.R, 256)), dim3(256), 0, s, p->A.R,
                            p->d_slab_rpp, p->d_slab_ploc, P, y, alpha, beta);
+    }
     MUXI_LAUNCH_CHECK();
     return SPMV_SUCCESS;
 }

@@ -24,6 +24,9 @@ static void plan_free(muxi_plan* p) {
     (void)hipFree(p->d_slab_rpp);
     (void)hipFree(p->d_slab_ckey);
     (void)hipFree(p->d_slab_ploc4);
+    (void)hipFree(p->d_slab_pc);
+    (void)hipFree(p->d_slab_ct);
+    (void)hipFree(p->d_slab_heavy);
     delete p;
 }
 

@@ -54,6 +54,10 @@ struct muxi_plan {
     int* d_slab_rp = nullptr;
     int* d_slab_ploc = nullptr;  // P slot of each pair, pairs in row-major order
     int* d_slab_ploc4 = nullptr; // with at most 4 slabs: the P slots of row r at [4r, 4r+4), -1 = none
+    void* d_slab_pc = nullptr;   // with ploc4: the single carry of each pair crossing one tile end (else 0)
+    int* d_slab_ct = nullptr;    // pairs crossing several tile ends: (P slot, first carry tile, end tile)
+    int* d_slab_heavy = nullptr; // rows holding such a pair, then their first ploc4 entries
+    int nheavy = 0;
     int* d_slab_ci = nullptr;
     void* d_slab_v = nullptr;
     uint32_t* d_slab_flags = nullptr;
