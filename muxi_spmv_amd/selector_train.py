@@ -17,7 +17,9 @@ python -m muxi_spmv_amd.selector_train [--train 2000] [--test 400] [--db PATH] [
    only topped up with the candidates it lacks; --skip leaves kernels out of
    a call, --retime times kernels again after they changed).
 3. The MLP (HIP forward/backward/Adam kernels) is fit on the training
-   records to predict log(t_candidate / t_best); its pick is the argmin.
+   records to predict log(t_candidate / t_best), --ensemble times from
+   different seeds; the shipped network averages them (one wider MLP with
+   block-diagonal hidden layers). Its pick is the argmin.
 4. On the held-out matrices it reports the median and p95 regret
    t_pick / t_best - 1 against the sweep, next to the reference's fixed rule
    and the best single configuration.
@@ -202,6 +204,7 @@ def main(argv=None) -> int:
     ap.add_argument("--sweep-only", action="store_true")
     ap.add_argument("--train-only", action="store_true")
     ap.add_argument("--epochs", type=int, default=8000)
+    ap.add_argument("--ensemble", type=int, default=4, help="networks averaged into the shipped one")
     ap.add_argument("--fp64-train", type=int, default=2000, help="training matrices also swept with FP64 values")
     ap.add_argument("--fp64-test", type=int, default=400, help="held-out matrices also swept with FP64 values")
     ap.add_argument("--big-train", type=int, default=300, help="training matrices with 4e6 to 2e7 rows")
@@ -230,9 +233,13 @@ def main(argv=None) -> int:
     test = [recs[i] for i in list(range(a.train, n)) + big_test + f64_test if i in recs]
     print(f"[train] {len(train)} train / {len(test)} held-out records", flush=True)
     X, T, M = dataset(train)
-    sel = tune.Selector(64, tune.N_INPUT, len(tune.CANDIDATES), seed=0)
-    losses = sel.fit(X, T, M, epochs=a.epochs, lr=2e-3, batch=256)
-    print(f"[train] loss {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
+    members = []
+    for k in range(max(1, a.ensemble)):
+        m = tune.Selector(64, tune.N_INPUT, len(tune.CANDIDATES), seed=k)
+        losses = m.fit(X, T, M, epochs=a.epochs, lr=2e-3, batch=256, seed=k)
+        print(f"[train] member {k}: loss {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
+        members.append(m)
+    sel = tune.Selector.ensemble(members) if len(members) > 1 else members[0]
     res = evaluate(sel, train, test)
     res["loss_first_last"] = [losses[0], losses[-1]]
     print(json.dumps(res), flush=True)

@@ -254,6 +254,43 @@ class Selector:
             P = np.where(mask > 0, P, np.inf)
         return P.argmin(1)
 
+    @staticmethod
+    def _layout(n_in: int, h: int, n_out: int) -> list[tuple[str, tuple]]:
+        """Parameter blocks in the library's order: W1 (h x in), b1, W2 (h x h), b2, W3 (out x h), b3."""
+        return [("w1", (h, n_in)), ("b1", (h,)), ("w2", (h, h)), ("b2", (h,)), ("w3", (n_out, h)), ("b3", (n_out,))]
+
+    def _blocks(self) -> dict[str, np.ndarray]:
+        p = self.params.detach().cpu().numpy().astype(np.float64)
+        out, k = {}, 0
+        for name, shape in self._layout(self.n_in, self.h, self.n_out):
+            n = int(np.prod(shape))
+            out[name] = p[k:k + n].reshape(shape)
+            k += n
+        return out
+
+    @classmethod
+    def ensemble(cls, members: list["Selector"]) -> "Selector":
+        """One network whose output is the mean of the members' outputs: the members' first layers stacked, their
+        second layers block-diagonal, their output layers side by side and scaled by 1/K. The members share the
+        input normalisation (trained on the same inputs)."""
+        K = len(members)
+        m0 = members[0]
+        n_in, h, n_out = m0.n_in, m0.h, m0.n_out
+        B = [m._blocks() for m in members]
+        w1 = np.concatenate([b["w1"] for b in B], 0)
+        b1 = np.concatenate([b["b1"] for b in B])
+        w2 = np.zeros((K * h, K * h))
+        for k, b in enumerate(B):
+            w2[k * h:(k + 1) * h, k * h:(k + 1) * h] = b["w2"]
+        b2 = np.concatenate([b["b2"] for b in B])
+        w3 = np.concatenate([b["w3"] for b in B], 1) / K
+        b3 = np.mean([b["b3"] for b in B], 0)
+        e = cls(K * h, n_in, n_out)
+        e.params = torch.from_numpy(np.concatenate([w1.ravel(), b1, w2.ravel(), b2, w3.ravel(), b3])
+                                    .astype(np.float32)).cuda()
+        e.mu, e.sd = m0.mu.clone(), m0.sd.clone()
+        return e
+
     def state(self) -> dict:
         return {"n_in": self.n_in, "hidden": self.h, "n_out": self.n_out, "candidates": CAND_NAMES,
                 "params": self.params.cpu().tolist(), "mu": self.mu.tolist(), "sd": self.sd.tolist()}

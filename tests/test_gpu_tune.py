@@ -179,3 +179,21 @@ def test_selector_clear_falls_back_to_fixed_rule():
     finally:
         lib.muxi_selector_reset()
     assert lib.muxi_selector_state() in (0, 1)
+
+
+def test_selector_ensemble_is_the_mean_of_its_members():
+    """Selector.ensemble (stacked / block-diagonal / averaged layers) predicts the mean of the members' predictions."""
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((40, tune.N_INPUT)).astype(np.float32)
+    members = []
+    for k in range(3):
+        m = tune.Selector(16, seed=10 + k)
+        m.mu = torch.from_numpy(rng.standard_normal(tune.N_INPUT).astype(np.float32))
+        m.sd = torch.from_numpy((1 + rng.random(tune.N_INPUT)).astype(np.float32))
+        members.append(m)
+    for m in members[1:]:
+        m.mu, m.sd = members[0].mu.clone(), members[0].sd.clone()
+    e = tune.Selector.ensemble(members)
+    assert e.h == 48
+    mean = np.mean([m.predict(X) for m in members], 0)
+    assert np.allclose(e.predict(X), mean, rtol=1e-5, atol=1e-5)
