@@ -252,16 +252,17 @@ __global__ __launch_bounds__(256) void xslab_combine_kernel(int R, const int* __
     store_y(y, r, s, alpha, beta);
 }
 
-// Combine for plans of at most four slabs, with no fix-up launch before it.
-// A row has at most four pairs, so their P slots sit in one 16-byte entry per
-// row (-1 = none): one vector load, then the partial loads together. A pair
+// Combine for plans of at most eight slabs, with no fix-up launch before it.
+// A row has at most K (4 or 8, the slab count rounded up) pairs, so their P
+// slots sit in one K-int entry per row (-1 = none): one or two 16-byte loads,
+// then the partial loads together. A pair
 // whose row crosses exactly one tile end has that carry in PC[slot] (the slab
 // kernel writes it there; PC is zero for every other slot), so it is
 // P[slot] + PC[slot]. A row with a pair that crosses several tile ends is
 // "heavy": its first entry is INT_MIN (the entry itself is in hfirst), the
 // crossing pair is -(j + 2) with ct[3j..3j+2] = (slot, t0, t1), and one of the
 // first workgroups' wavefronts sums carry[t0 .. t1) 64 at a time.
-template <typename XT, typename AT>
+template <typename XT, typename AT, int K>
 __global__ __launch_bounds__(256) void xslab_combine4_kernel(int R, const int* __restrict__ ploc4,
                                                              const AT* __restrict__ P, const AT* __restrict__ PC,
                                                              const int* __restrict__ ct, const AT* __restrict__ carry,
@@ -274,8 +275,8 @@ __global__ __launch_bounds__(256) void xslab_combine4_kernel(int R, const int* _
         if (h >= H) return;  // whole wavefront
         const int r = hrow[h];
         AT s = AT(0);
-        for (int k = 0; k < 4; ++k) {
-            const int q = k == 0 ? hfirst[h] : ploc4[4L * r + k];
+        for (int k = 0; k < K; ++k) {
+            const int q = k == 0 ? hfirst[h] : ploc4[(long)K * r + k];
             if (q == -1) break;  // no more pairs
             if (q >= 0) {
                 s += P[q] + PC[q];
@@ -294,17 +295,23 @@ __global__ __launch_bounds__(256) void xslab_combine4_kernel(int R, const int* _
     const long r = (long)(blockIdx.x - HB) * blockDim.x + threadIdx.x;
     if (r >= R) return;
     typedef int I4 __attribute__((ext_vector_type(4)));
-    const I4 q = __builtin_nontemporal_load((const I4*)ploc4 + r);
-    if (q[0] == INT_MIN) return;  // a heavy row
-    AT v[4], c[4];
+    int q[K];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int g = 0; g < K / 4; ++g) {
+        const I4 w = __builtin_nontemporal_load((const I4*)(ploc4 + (long)K * r) + g);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q[4 * g + j] = w[j];
+    }
+    if (q[0] == INT_MIN) return;  // a heavy row
+    AT v[K], c[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
         v[k] = ntl(P + (q[k] >= 0 ? q[k] : 0));
         c[k] = PC[q[k] >= 0 ? q[k] : 0];
     }
     AT s = AT(0);
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < K; ++k)
         if (q[k] >= 0) s += v[k] + c[k];
     store_y(y, r, s, alpha, beta);
 }
@@ -336,18 +343,17 @@ __global__ void xslab_cross_id_kernel(const int* __restrict__ ct, long nct, int*
 }
 
 // ploc4 entries of pairs crossing several tile ends become -(j + 2); their rows get hflag = 1.
-__global__ void xslab_mark_heavy_kernel(int R, int* __restrict__ ploc4, const int* __restrict__ cross_id,
+__global__ void xslab_mark_heavy_kernel(int R, int K, int* __restrict__ ploc4, const int* __restrict__ cross_id,
                                         int* __restrict__ hflag) {
     const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     int heavy = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int q = ploc4[4 * r + k];
+    for (int k = 0; k < K; ++k) {
+        const int q = ploc4[K * r + k];
         if (q < 0) continue;
         const int j = cross_id[q];
         if (j >= 0) {
-            ploc4[4 * r + k] = -(j + 2);
+            ploc4[K * r + k] = -(j + 2);
             heavy = 1;
         }
     }
@@ -355,14 +361,14 @@ __global__ void xslab_mark_heavy_kernel(int R, int* __restrict__ ploc4, const in
 }
 
 // Heavy row h = hrank[r]: hrow[h] = r, its first ploc4 entry moves to hfirst[h].
-__global__ void xslab_heavy_rows_kernel(int R, const int* __restrict__ hflag, const int* __restrict__ hrank,
+__global__ void xslab_heavy_rows_kernel(int R, int K, const int* __restrict__ hflag, const int* __restrict__ hrank,
                                         int* __restrict__ ploc4, int* __restrict__ hrow, int* __restrict__ hfirst) {
     const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R || !hflag[r]) return;
     const int h = hrank[r];
     hrow[h] = (int)r;
-    hfirst[h] = ploc4[4 * r];
-    ploc4[4 * r] = INT_MIN;
+    hfirst[h] = ploc4[K * r];
+    ploc4[K * r] = INT_MIN;
 }
 
 // ---- plan construction
@@ -442,7 +448,7 @@ __global__ __launch_bounds__(256) void xslab_fill_kernel(const int* __restrict__
                                                          const int* __restrict__ nnz_base,
                                                          const int* __restrict__ pad_base,
                                                          const int* __restrict__ rpp, int* __restrict__ srp,
-                                                         int* __restrict__ ploc, int* __restrict__ ploc4,
+                                                         int* __restrict__ ploc, int* __restrict__ ploc4, int K,
                                                          int* __restrict__ sci,
                                                          VT* __restrict__ sv) {
     const long r = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -458,7 +464,7 @@ __global__ __launch_bounds__(256) void xslab_fill_kernel(const int* __restrict__
             const int lr = lrow[k];
             const int rank = __popcll(hm & __lanemask_lt());
             ploc[rpp[r] + rank] = lr;  // the row's pairs in slab order
-            if (ploc4) ploc4[r * 4 + rank] = lr;
+            if (ploc4) ploc4[r * K + rank] = lr;
             srp[lr + lane] = off - nnz_base[lane];
         }
     }
@@ -636,16 +642,17 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
     for (int q = 0; q < S; ++q)
         MUXI_HIP(hipMemcpyAsync(p->d_slab_rp + sa.row_base[q] + q + sa.rows[q], &term[q], sizeof(int),
                                 hipMemcpyHostToDevice, s));
-    if (S <= 4) {  // a row has at most S pairs: their slots in one 16-byte entry per row
-        MUXI_HIP(hipMalloc(&p->d_slab_ploc4, sizeof(int) * 4 * (size_t)R));
-        MUXI_HIP(hipMemsetAsync(p->d_slab_ploc4, 0xff, sizeof(int) * 4 * (size_t)R, s));
+    p->slab_k = S <= 4 ? 4 : (S <= 8 ? 8 : 0);
+    if (p->slab_k) {  // a row has at most S pairs: their slots in one K-int entry per row
+        MUXI_HIP(hipMalloc(&p->d_slab_ploc4, sizeof(int) * p->slab_k * (size_t)R));
+        MUXI_HIP(hipMemsetAsync(p->d_slab_ploc4, 0xff, sizeof(int) * p->slab_k * (size_t)R, s));
     }
     st = with_types(vt, p->xtype, p->comp, [&](auto tag) -> spmv_status_t {
         using T = decltype(tag);
         using VT = typename T::VT;
         hipLaunchKernelGGL(xslab_fill_kernel<VT>, dim3(grid_for((long)R * 64, 256)), dim3(256), 0, s, p->A.rp, p->A.ci,
                            (const VT*)p->A.v, R, cut, S, cnt, pos, lrow, d_nb, d_nb + S, p->d_slab_rpp, p->d_slab_rp,
-                           p->d_slab_ploc, p->d_slab_ploc4, p->d_slab_ci, (VT*)p->d_slab_v);
+                           p->d_slab_ploc, p->d_slab_ploc4, p->slab_k, p->d_slab_ci, (VT*)p->d_slab_v);
         MUXI_LAUNCH_CHECK();
         for (int q = 0; q < S; ++q) {
             const long from = (long)hpad[q] + sa.nnz[q], to = (long)hpad[q] + (long)sa.tiles[q] * SIG;
@@ -735,7 +742,7 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
             hipLaunchKernelGGL(xslab_cross_id_kernel, dim3(grid_for(nct, 256)), dim3(256), 0, s, p->d_slab_ct, nct,
                                cross_id);
             MUXI_LAUNCH_CHECK();
-            hipLaunchKernelGGL(xslab_mark_heavy_kernel, dim3(grid_for(R, 256)), dim3(256), 0, s, R,
+            hipLaunchKernelGGL(xslab_mark_heavy_kernel, dim3(grid_for(R, 256)), dim3(256), 0, s, R, p->slab_k,
                                p->d_slab_ploc4, cross_id, hflag);
             MUXI_LAUNCH_CHECK();
             st = exscan(hflag, hrank, (long)R + 1, s);
@@ -745,8 +752,8 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
             MUXI_HIP(hipStreamSynchronize(s));
             if (H > 0) {
                 MUXI_HIP(hipMalloc(&p->d_slab_heavy, sizeof(int) * 2 * (size_t)H));
-                hipLaunchKernelGGL(xslab_heavy_rows_kernel, dim3(grid_for(R, 256)), dim3(256), 0, s, R, hflag, hrank,
-                                   p->d_slab_ploc4, p->d_slab_heavy, p->d_slab_heavy + H);
+                hipLaunchKernelGGL(xslab_heavy_rows_kernel, dim3(grid_for(R, 256)), dim3(256), 0, s, R, p->slab_k,
+                                   hflag, hrank, p->d_slab_ploc4, p->d_slab_heavy, p->d_slab_heavy + H);
                 MUXI_LAUNCH_CHECK();
             }
             p->nheavy = H;
@@ -757,7 +764,7 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
     MUXI_HIP(hipStreamSynchronize(s));
     p->scratch_bytes = sizeof(int) * ints + vs * (size_t)tp + 4 * nflag + 2 * sizeof(int) * (size_t)(ntiles + 1) +
                        acc * (size_t)(ncarry + total_rows) + sizeof(int) * ((size_t)ncarry + R + 1) +
-                       (p->d_slab_ploc4 ? sizeof(int) * 4 * (size_t)R : 0);
+                       (p->d_slab_ploc4 ? sizeof(int) * p->slab_k * (size_t)R : 0);
     return SPMV_SUCCESS;
 }
 
@@ -788,9 +795,13 @@ spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT 
         const int H = p->nheavy;
         // with no tiles there are no pairs: PC is never read then (every ploc4 entry is -1)
         const AT* PC = p->d_slab_pc ? (const AT*)p->d_slab_pc : P;
-        hipLaunchKernelGGL((xslab_combine4_kernel<XT, AT>), dim3((H + 3) / 4 + grid_for(p->A.R, 256)), dim3(256), 0,
-                           s, p->A.R, p->d_slab_ploc4, P, PC, p->d_slab_ct, carry, H, p->d_slab_heavy,
-                           p->d_slab_heavy + H, y, alpha, beta);
+        const dim3 grid((H + 3) / 4 + grid_for(p->A.R, 256));
+        if (p->slab_k == 4)
+            hipLaunchKernelGGL((xslab_combine4_kernel<XT, AT, 4>), grid, dim3(256), 0, s, p->A.R, p->d_slab_ploc4, P,
+                               PC, p->d_slab_ct, carry, H, p->d_slab_heavy, p->d_slab_heavy + H, y, alpha, beta);
+        else
+            hipLaunchKernelGGL((xslab_combine4_kernel<XT, AT, 8>), grid, dim3(256), 0, s, p->A.R, p->d_slab_ploc4, P,
+                               PC, p->d_slab_ct, carry, H, p->d_slab_heavy, p->d_slab_heavy + H, y, alpha, beta);
     } else {
         hipLaunchKernelGGL((xslab_combine_kernel<XT, AT>), dim3(grid_for(p->A.R, 256)), dim3(256), 0, s, p->A.R,
                            p->d_slab_rpp, p->d_slab_ploc, P, y, alpha, beta);
