@@ -223,12 +223,28 @@ static spmv_status_t plan_create_impl(const muxi_csr_t* Ain, spmv_type_t vt, spm
             p->param0 = width;
             const size_t slots = (size_t)R * (size_t)width;
             if (slots > 0) {
-                PLAN_HIP(hipMalloc(&p->d_ecol, sizeof(int) * slots));
+                // 16-bit column offsets when every column is within 32767 of its row
+                int bad = 1;
+                {
+                    int* d_bad = nullptr;
+                    PLAN_HIP(hipMalloc(&d_bad, sizeof(int)));
+                    spmv_status_t st = launch_ell_offsets_fit(p->A, width, d_bad, s);
+                    if (st == SPMV_SUCCESS &&
+                        (hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                         hipStreamSynchronize(s) != hipSuccess))
+                        st = SPMV_ERROR_MEMORY_COPY;
+                    (void)hipFree(d_bad);
+                    PLAN_TRY(st);
+                }
+                p->ecol16 = bad == 0;
+                const size_t csize = p->ecol16 ? sizeof(short) : sizeof(int);
+                PLAN_HIP(hipMalloc(&p->d_ecol, csize * slots));
                 PLAN_HIP(hipMalloc(&p->d_eval, vsize(vt) * slots));
-                p->scratch_bytes = (sizeof(int) + vsize(vt)) * slots;
+                p->scratch_bytes = (csize + vsize(vt)) * slots;
                 PLAN_TRY(with_types(vt, xt, p->comp, [&](auto tag) {
                     using T = decltype(tag);
-                    return launch_csr_to_ellg<typename T::VT>(p->A, width, p->d_ecol, (typename T::VT*)p->d_eval, s);
+                    return launch_csr_to_ellg<typename T::VT>(p->A, width, p->ecol16, p->d_ecol,
+                                                              (typename T::VT*)p->d_eval, s);
                 }));
             }
             break;
@@ -374,8 +390,8 @@ static spmv_status_t plan_exec(const muxi_plan* p, const void* x, void* y, doubl
                 return launch_csr5<VT, XT, AT, C>(p->A, p->param0, p->d_tile_row, p->T, p->d_carry, xx, yy, a, b, s,
                                                   ev1);
             case MUXI_KERNEL_ELL:
-                return launch_ellg<VT, XT, AT, C>(p->A.R, p->A.C, p->width, p->d_ecol, (const VT*)p->d_eval, xx, yy, a,
-                                                  b, s);
+                return launch_ellg<VT, XT, AT, C>(p->A.R, p->A.C, p->width, p->ecol16, p->d_ecol,
+                                                  (const VT*)p->d_eval, xx, yy, a, b, s);
             case MUXI_KERNEL_XSLAB:
                 if constexpr (C) {
                     set_error("xslab: no compensated variant");

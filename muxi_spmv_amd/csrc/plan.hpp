@@ -36,7 +36,8 @@ struct muxi_plan {
     void* d_carry = nullptr;
     // ELL
     int width = 0;
-    int* d_ecol = nullptr;
+    void* d_ecol = nullptr;  // int columns, or short offsets col - row when ecol16
+    bool ecol16 = false;
     void* d_eval = nullptr;
     // adaptive row blocks: blk_row[nblocks+1], blk_chunk[nblocks], long_row[nlong], long_first[2*nlong]
     int nblocks = 0;

@@ -345,3 +345,46 @@ def test_full_size_laplacian_exact():
     api.spmv_ellpack(E, x, y)
     torch.cuda.synchronize()
     assert torch.equal(y, exp)
+
+
+def _offset_matrix(R, far, dtype=np.float32, seed=5):
+    """Rows r hold columns r - far, r, r + far (those inside [0, R)) plus up to two
+    random columns within 100 of r, so row lengths (and the ELL padding) vary."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for r in range(R):
+        c = {r}
+        if r - far >= 0:
+            c.add(r - far)
+        if r + far < R:
+            c.add(r + far)
+        for _ in range(int(rng.integers(0, 3))):
+            c.add(int(min(R - 1, max(0, r + rng.integers(-100, 101)))))
+        rows.append(sorted(c))
+    rp = np.zeros(R + 1, np.int32)
+    rp[1:] = np.cumsum([len(c) for c in rows])
+    ci = np.concatenate([np.asarray(c, np.int32) for c in rows])
+    v = rng.uniform(-1, 1, ci.size).astype(dtype)
+    return G.HostCSR(rp, ci, v, R, R)
+
+
+@pytest.mark.parametrize("far,off16", [(32767, True), (32768, False)])
+@pytest.mark.parametrize("vdt", [torch.float32, torch.float64])
+def test_ell_column_offsets(far, off16, vdt):
+    """The ELL plan stores 16-bit column offsets exactly when every column lies
+    within 32767 of its row (offset -32768 is the padding mark, so a column
+    32768 below its row, like one 32768 above, keeps int columns)."""
+    npdt = np.float64 if vdt == torch.float64 else np.float32
+    h = _offset_matrix(70_000, far, npdt)
+    y, x, _, p = run_plan(h, "ell", 0, vdt, vdt)
+    width = int(h.row_lengths().max())
+    csize = 2 if off16 else 4
+    assert p.info().scratch_bytes == (csize + np.dtype(npdt).itemsize) * h.numRows * width
+    (assert_close_fp64 if vdt == torch.float64 else assert_close_fp32)(y, h, x)
+    # alpha / beta through the same plan
+    y0 = torch.from_numpy(xvec(h.numRows, seed=3, dtype=npdt)).cuda()
+    y2, x2, _, _ = run_plan(h, "ell", 0, vdt, vdt, alpha=2.0, beta=-0.5, y0=y0)
+    ref = 2.0 * oracle.spmv_csr_f64_plain(h.rowPtr, h.colIdx, h.values.astype(np.float64), x2.astype(np.float64)) \
+        - 0.5 * y0.cpu().numpy().astype(np.float64)
+    tol = 1e-5 if vdt == torch.float32 else 1e-12
+    assert np.max(np.abs(y2.cpu().numpy() - ref)) <= tol * (1 + np.max(np.abs(ref)))
