@@ -100,9 +100,11 @@ def precision(args) -> list[dict]:
         y64 = api.oracle_f64(A64, x)  # FP64 values, FP32 x, exact-ish FP64 sum
         for vt in (torch.float64, torch.float32, torch.bfloat16):
             Av = A64.with_values(api.convert_values(A64.d_values, vt))
-            for kernel, param in (("merge", 4), ("xslab", 0)):
+            for kernel, param in (("merge", 4), ("xslab", 0), ("ell", 0), ("auto", 0)):
+                if kernel == "ell" and name != "banded_4m_rcm":
+                    continue  # ELL pads every row of the power-law matrix to its longest
                 for comp in (False, True):
-                    if kernel == "xslab" and comp:
+                    if kernel in ("xslab", "auto") and comp:
                         continue
                     p = api.Plan(Av, kernel, param, compensated=comp)
                     y = torch.empty(Av.numRows, device="cuda")
