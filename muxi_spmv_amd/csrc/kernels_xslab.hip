@@ -252,12 +252,18 @@ __global__ __launch_bounds__(256) void xslab_combine_kernel(int R, const int* __
     store_y(y, r, s, alpha, beta);
 }
 
+// Slot entries: bits 0-29 the P slot, bit 30 set when the pair has a carry
+// in PC[slot], bit 31 the sign of the marks below.
+constexpr int XSLAB_SLOT_MASK = 0x3fffffff;
+constexpr int XSLAB_HAS_PC = 0x40000000;
+
 // Combine for plans of at most eight slabs, with no fix-up launch before it.
 // A row has at most K (4 or 8, the slab count rounded up) pairs, so their P
 // slots sit in one K-int entry per row (-1 = none): one or two 16-byte loads,
 // then the partial loads together. A pair
 // whose row crosses exactly one tile end has that carry in PC[slot] (the slab
-// kernel writes it there; PC is zero for every other slot), so it is
+// kernel writes it there; PC is zero for every other slot, and the entry
+// carries XSLAB_HAS_PC, so only these pairs load PC), so it is
 // P[slot] + PC[slot]. A row with a pair that crosses several tile ends is
 // "heavy": its first entry is INT_MIN (the entry itself is in hfirst), the
 // crossing pair is -(j + 2) with ct[3j..3j+2] = (slot, t0, t1), and one of the
@@ -279,7 +285,7 @@ __global__ __launch_bounds__(256) void xslab_combine4_kernel(int R, const int* _
             const int q = k == 0 ? hfirst[h] : ploc4[(long)K * r + k];
             if (q == -1) break;  // no more pairs
             if (q >= 0) {
-                s += P[q] + PC[q];
+                s += P[q & XSLAB_SLOT_MASK] + PC[q & XSLAB_SLOT_MASK];  // PC is 0 where no carry lands
             } else {
                 const int j = -q - 2, t0 = ct[3 * j + 1], t1 = ct[3 * j + 2];
                 AT c = AT(0);
@@ -306,8 +312,9 @@ __global__ __launch_bounds__(256) void xslab_combine4_kernel(int R, const int* _
     AT v[K], c[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        v[k] = ntl(P + (q[k] >= 0 ? q[k] : 0));
-        c[k] = PC[q[k] >= 0 ? q[k] : 0];
+        const int qi = q[k] & XSLAB_SLOT_MASK;
+        v[k] = ntl(P + (q[k] >= 0 ? qi : 0));
+        c[k] = q[k] >= 0 && (q[k] & XSLAB_HAS_PC) ? PC[qi] : AT(0);  // only pairs with a carry slot read it
     }
     AT s = AT(0);
 #pragma unroll
@@ -329,6 +336,20 @@ __global__ void xslab_carry_keys_kernel(const int* __restrict__ srp, const int* 
         if (srp[r] < (t + 1) * SIG) k = row_base + r;
     }
     keys[t] = k;
+}
+
+// Sets a[idx[i]] = 1.
+__global__ void xslab_set_kernel(const int* __restrict__ idx, int n, int* __restrict__ a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[idx[i]] = 1;
+}
+
+// Slot entries (of n = K * R) whose pair has a carry slot get XSLAB_HAS_PC.
+__global__ void xslab_flag_pc_kernel(long n, int* __restrict__ ploc4, const int* __restrict__ has_pc) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int q = ploc4[i];
+    if (q >= 0 && has_pc[q]) ploc4[i] = q | XSLAB_HAS_PC;
 }
 
 // Marks the descriptors of tiles whose carry is their open row's only one.
@@ -642,7 +663,8 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
     for (int q = 0; q < S; ++q)
         MUXI_HIP(hipMemcpyAsync(p->d_slab_rp + sa.row_base[q] + q + sa.rows[q], &term[q], sizeof(int),
                                 hipMemcpyHostToDevice, s));
-    p->slab_k = S <= 4 ? 4 : (S <= 8 ? 8 : 0);
+    // (slot indices keep bits 30 and 31 free: XSLAB_HAS_PC and the sign)
+    p->slab_k = total_rows >= XSLAB_SLOT_MASK ? 0 : (S <= 4 ? 4 : (S <= 8 ? 8 : 0));
     if (p->slab_k) {  // a row has at most S pairs: their slots in one K-int entry per row
         MUXI_HIP(hipMalloc(&p->d_slab_ploc4, sizeof(int) * p->slab_k * (size_t)R));
         MUXI_HIP(hipMemsetAsync(p->d_slab_ploc4, 0xff, sizeof(int) * p->slab_k * (size_t)R, s));
@@ -758,6 +780,24 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
             }
             p->nheavy = H;
             for (int* t : {cross_id, hflag, hrank}) (void)hipFreeAsync(t, s);
+        }
+        if (!single.empty()) {  // the slot entries of pairs with a carry slot say so (the others skip PC)
+            std::vector<int> skey(single.size());
+            for (size_t i = 0; i < single.size(); ++i) skey[i] = hk[(size_t)single[i]];
+            int *d_sk = nullptr, *has_pc = nullptr;
+            MUXI_HIP(hipMallocAsync(&d_sk, sizeof(int) * skey.size(), s));
+            MUXI_HIP(hipMallocAsync(&has_pc, sizeof(int) * ((size_t)total_rows + 1), s));
+            MUXI_HIP(hipMemsetAsync(has_pc, 0, sizeof(int) * ((size_t)total_rows + 1), s));
+            MUXI_HIP(hipMemcpyAsync(d_sk, skey.data(), sizeof(int) * skey.size(), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(xslab_set_kernel, dim3(grid_for((long)skey.size(), 256)), dim3(256), 0, s, d_sk,
+                               (int)skey.size(), has_pc);
+            MUXI_LAUNCH_CHECK();
+            const long n = (long)p->slab_k * R;
+            hipLaunchKernelGGL(xslab_flag_pc_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, p->d_slab_ploc4,
+                               has_pc);
+            MUXI_LAUNCH_CHECK();
+            MUXI_HIP(hipStreamSynchronize(s));  // skey is host memory the copy reads
+            for (int* t : {d_sk, has_pc}) (void)hipFreeAsync(t, s);
         }
     }
     for (int* t : {cnt, flag, pos, lrow, npair, d_nb}) (void)hipFreeAsync(t, s);
