@@ -252,29 +252,34 @@ __global__ __launch_bounds__(256) void xslab_combine_kernel(int R, const int* __
     store_y(y, r, s, alpha, beta);
 }
 
-// Slot entries: bits 0-29 the P slot, bit 30 set when the pair has a carry
-// in PC[slot], bit 31 the sign of the marks below.
-constexpr int XSLAB_SLOT_MASK = 0x3fffffff;
-constexpr int XSLAB_HAS_PC = 0x40000000;
+// Row masks of the fix-up-free combine: bits 0-7 the slabs holding a pair of
+// the row, bit 8 + s set when the row's pair in slab s has a carry in PC,
+// bit 31 set for a heavy row. Slot entries of heavy rows: the P slot, or
+// -(j + 2) for a pair crossing several tile ends, -1 after the last pair.
+constexpr unsigned XSLAB_HEAVY = 0x80000000u;
 
 // Combine for plans of at most eight slabs, with no fix-up launch before it.
-// A row has at most K (4 or 8, the slab count rounded up) pairs, so their P
-// slots sit in one K-int entry per row (-1 = none): one or two 16-byte loads,
-// then the partial loads together. A pair
-// whose row crosses exactly one tile end has that carry in PC[slot] (the slab
-// kernel writes it there; PC is zero for every other slot, and the entry
-// carries XSLAB_HAS_PC, so only these pairs load PC), so it is
-// P[slot] + PC[slot]. A row with a pair that crosses several tile ends is
-// "heavy": its first entry is INT_MIN (the entry itself is in hfirst), the
-// crossing pair is -(j + 2) with ct[3j..3j+2] = (slot, t0, t1), and one of the
-// first workgroups' wavefronts sums carry[t0 .. t1) 64 at a time.
+// The P slot of row r's pair in slab s is wbase[w * S + s] (the slot of the
+// first pair in slab s at or after row 64w, w = r / 64) plus the number of
+// rows of r's wavefront before r that hold a pair in slab s, which a ballot
+// of the mask bit gives: the combine reads one 4-byte mask per row, and the
+// pairs of consecutive rows in one slab are consecutive slots. A pair whose
+// row crosses exactly one tile end has that carry in PC[slot] (the slab kernel
+// writes it there; PC is zero for every other slot), so it is P[slot] +
+// PC[slot]; only pairs with the mask bit load PC. A row with a pair that
+// crosses several tile ends is "heavy": its K slot entries are in hent,
+// the crossing pair is -(j + 2) with ct[3j..3j+2] = (slot, t0, t1), and one
+// of the first workgroups' wavefronts sums carry[t0 .. t1) 64 at a time.
+// Pairs are added in slab order, as in every other combine.
 template <typename XT, typename AT, int K>
-__global__ __launch_bounds__(256) void xslab_combine4_kernel(int R, const int* __restrict__ ploc4,
-                                                             const AT* __restrict__ P, const AT* __restrict__ PC,
-                                                             const int* __restrict__ ct, const AT* __restrict__ carry,
-                                                             int H, const int* __restrict__ hrow,
-                                                             const int* __restrict__ hfirst, XT* __restrict__ y,
-                                                             AT alpha, AT beta) {
+__global__ __launch_bounds__(256) void xslab_combine_mask_kernel(int R, int S, const unsigned* __restrict__ rmask,
+                                                                 const int* __restrict__ wbase,
+                                                                 const AT* __restrict__ P, const AT* __restrict__ PC,
+                                                                 const int* __restrict__ ct,
+                                                                 const AT* __restrict__ carry, int H,
+                                                                 const int* __restrict__ hrow,
+                                                                 const int* __restrict__ hent, XT* __restrict__ y,
+                                                                 AT alpha, AT beta) {
     const int HB = (H + 3) / 4;
     if ((int)blockIdx.x < HB) {  // heavy rows: one wavefront each
         const int h = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -282,10 +287,10 @@ __global__ __launch_bounds__(256) void xslab_combine4_kernel(int R, const int* _
         const int r = hrow[h];
         AT s = AT(0);
         for (int k = 0; k < K; ++k) {
-            const int q = k == 0 ? hfirst[h] : ploc4[(long)K * r + k];
+            const int q = hent[(long)K * h + k];
             if (q == -1) break;  // no more pairs
             if (q >= 0) {
-                s += P[q & XSLAB_SLOT_MASK] + PC[q & XSLAB_SLOT_MASK];  // PC is 0 where no carry lands
+                s += P[q] + PC[q];  // PC is 0 where no carry lands
             } else {
                 const int j = -q - 2, t0 = ct[3 * j + 1], t1 = ct[3 * j + 2];
                 AT c = AT(0);
@@ -298,29 +303,35 @@ __global__ __launch_bounds__(256) void xslab_combine4_kernel(int R, const int* _
         if (lane == 0) store_y(y, r, s, alpha, beta);
         return;
     }
+    // whole wavefronts stay to the end (the ballots); rows past R have no bits
     const long r = (long)(blockIdx.x - HB) * blockDim.x + threadIdx.x;
-    if (r >= R) return;
-    typedef int I4 __attribute__((ext_vector_type(4)));
-    int q[K];
+    const unsigned m = r < R ? __builtin_nontemporal_load(rmask + r) : 0u;
+    const bool mine = r < R && !(m & XSLAB_HEAVY);
+    // the wavefront's slot bases, loaded beside the mask (a wavefront wholly past R reads wavefront 0's)
+    const long w = (r & ~63L) < R ? r >> 6 : 0;
+    int base[K];
 #pragma unroll
-    for (int g = 0; g < K / 4; ++g) {
-        const I4 w = __builtin_nontemporal_load((const I4*)(ploc4 + (long)K * r) + g);
+    for (int q = 0; q < K; ++q) base[q] = q < S ? wbase[w * S + q] : 0;
+    const unsigned long long lt = __lanemask_lt();
+    int slot[K];
+    bool use[K];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) q[4 * g + j] = w[j];
+    for (int q = 0; q < K; ++q) {
+        const bool has = q < S && ((m >> q) & 1u);
+        slot[q] = base[q] + __popcll(__ballot(has) & lt);
+        use[q] = has && mine;
     }
-    if (q[0] == INT_MIN) return;  // a heavy row
     AT v[K], c[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int qi = q[k] & XSLAB_SLOT_MASK;
-        v[k] = ntl(P + (q[k] >= 0 ? qi : 0));
-        c[k] = q[k] >= 0 && (q[k] & XSLAB_HAS_PC) ? PC[qi] : AT(0);  // only pairs with a carry slot read it
+    for (int q = 0; q < K; ++q) {
+        v[q] = use[q] ? ntl(P + slot[q]) : AT(0);
+        c[q] = use[q] && ((m >> (8 + q)) & 1u) ? PC[slot[q]] : AT(0);
     }
     AT s = AT(0);
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-        if (q[k] >= 0) s += v[k] + c[k];
-    store_y(y, r, s, alpha, beta);
+    for (int q = 0; q < K; ++q)
+        if (use[q]) s += v[q] + c[q];
+    if (mine) store_y(y, r, s, alpha, beta);
 }
 
 // The pair open at the end of each tile of one slab (-1 = none): tile t ends
@@ -344,12 +355,30 @@ __global__ void xslab_set_kernel(const int* __restrict__ idx, int n, int* __rest
     if (i < n) a[idx[i]] = 1;
 }
 
-// Slot entries (of n = K * R) whose pair has a carry slot get XSLAB_HAS_PC.
-__global__ void xslab_flag_pc_kernel(long n, int* __restrict__ ploc4, const int* __restrict__ has_pc) {
+// Row r's pairs with a carry slot (has_pc[slot]) set bit 8 + s of its mask,
+// s the slab of the pair (the k-th set bit of the mask for entry k).
+__global__ void xslab_flag_pc_kernel(int R, int K, const int* __restrict__ ploc4, const int* __restrict__ has_pc,
+                                     unsigned* __restrict__ rmask) {
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    unsigned m = rmask[r], rest = m & 0xffu, add = 0;
+    for (int k = 0; k < K && rest; ++k) {
+        const int sl = __builtin_ctz(rest);
+        rest &= rest - 1;
+        const int q = ploc4[K * r + k];
+        if (q >= 0 && has_pc[q]) add |= 1u << (8 + sl);
+    }
+    if (add) rmask[r] = m | add;
+}
+
+// wbase[w * S + s]: the P slot of the first pair in slab s at or after row
+// 64w, from the slab-major exclusive scan lrow of the pair flags.
+__global__ void xslab_wbase_kernel(int R, int S, const int* __restrict__ lrow, int* __restrict__ wbase) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int q = ploc4[i];
-    if (q >= 0 && has_pc[q]) ploc4[i] = q | XSLAB_HAS_PC;
+    const long nw = ((long)R + 63) / 64;
+    if (i >= nw * S) return;
+    const long w = i / S, q = i % S;
+    wbase[i] = lrow[q * R + 64 * w];
 }
 
 // Marks the descriptors of tiles whose carry is their open row's only one.
@@ -381,15 +410,17 @@ __global__ void xslab_mark_heavy_kernel(int R, int K, int* __restrict__ ploc4, c
     hflag[r] = heavy;
 }
 
-// Heavy row h = hrank[r]: hrow[h] = r, its first ploc4 entry moves to hfirst[h].
+// Heavy row h = hrank[r]: hrow[h] = r, its K slot entries go to hent[Kh ..],
+// and its mask gets XSLAB_HEAVY.
 __global__ void xslab_heavy_rows_kernel(int R, int K, const int* __restrict__ hflag, const int* __restrict__ hrank,
-                                        int* __restrict__ ploc4, int* __restrict__ hrow, int* __restrict__ hfirst) {
+                                        const int* __restrict__ ploc4, int* __restrict__ hrow, int* __restrict__ hent,
+                                        unsigned* __restrict__ rmask) {
     const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R || !hflag[r]) return;
     const int h = hrank[r];
     hrow[h] = (int)r;
-    hfirst[h] = ploc4[K * r];
-    ploc4[K * r] = INT_MIN;
+    for (int k = 0; k < K; ++k) hent[(long)K * h + k] = ploc4[K * r + k];
+    rmask[r] |= XSLAB_HEAVY;
 }
 
 // ---- plan construction
@@ -470,7 +501,7 @@ __global__ __launch_bounds__(256) void xslab_fill_kernel(const int* __restrict__
                                                          const int* __restrict__ pad_base,
                                                          const int* __restrict__ rpp, int* __restrict__ srp,
                                                          int* __restrict__ ploc, int* __restrict__ ploc4, int K,
-                                                         int* __restrict__ sci,
+                                                         unsigned* __restrict__ rmask, int* __restrict__ sci,
                                                          VT* __restrict__ sv) {
     const long r = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
@@ -481,6 +512,7 @@ __global__ __launch_bounds__(256) void xslab_fill_kernel(const int* __restrict__
         const bool has = lane < S && cnt[k] > 0;  // row r has nonzeros in slab `lane`
         const unsigned long long hm = __ballot(has);
         if (lane < S) off = pos[k];
+        if (rmask && lane == 0) rmask[r] = (unsigned)hm;  // S <= 8 here: bits 0-7
         if (has) {
             const int lr = lrow[k];
             const int rank = __popcll(hm & __lanemask_lt());
@@ -663,18 +695,26 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
     for (int q = 0; q < S; ++q)
         MUXI_HIP(hipMemcpyAsync(p->d_slab_rp + sa.row_base[q] + q + sa.rows[q], &term[q], sizeof(int),
                                 hipMemcpyHostToDevice, s));
-    // (slot indices keep bits 30 and 31 free: XSLAB_HAS_PC and the sign)
-    p->slab_k = total_rows >= XSLAB_SLOT_MASK ? 0 : (S <= 4 ? 4 : (S <= 8 ? 8 : 0));
-    if (p->slab_k) {  // a row has at most S pairs: their slots in one K-int entry per row
-        MUXI_HIP(hipMalloc(&p->d_slab_ploc4, sizeof(int) * p->slab_k * (size_t)R));
-        MUXI_HIP(hipMemsetAsync(p->d_slab_ploc4, 0xff, sizeof(int) * p->slab_k * (size_t)R, s));
+    // At most eight slabs: the combine works from row masks and per-wavefront slot bases, with no fix-up
+    // launch. ploc4 (a row's pair slots in one K-int entry, -1 = none) only serves the plan build.
+    p->slab_k = S <= 4 ? 4 : (S <= 8 ? 8 : 0);
+    int* ploc4 = nullptr;
+    if (p->slab_k) {
+        MUXI_HIP(hipMallocAsync(&ploc4, sizeof(int) * p->slab_k * (size_t)R, s));
+        MUXI_HIP(hipMemsetAsync(ploc4, 0xff, sizeof(int) * p->slab_k * (size_t)R, s));
+        MUXI_HIP(hipMalloc(&p->d_slab_rmask, sizeof(unsigned) * (size_t)R));
+        const long nwb = ((long)R + 63) / 64 * S;
+        MUXI_HIP(hipMalloc(&p->d_slab_wbase, sizeof(int) * (size_t)nwb));
+        hipLaunchKernelGGL(xslab_wbase_kernel, dim3(grid_for(nwb, 256)), dim3(256), 0, s, R, S, lrow,
+                           p->d_slab_wbase);
+        MUXI_LAUNCH_CHECK();
     }
     st = with_types(vt, p->xtype, p->comp, [&](auto tag) -> spmv_status_t {
         using T = decltype(tag);
         using VT = typename T::VT;
         hipLaunchKernelGGL(xslab_fill_kernel<VT>, dim3(grid_for((long)R * 64, 256)), dim3(256), 0, s, p->A.rp, p->A.ci,
                            (const VT*)p->A.v, R, cut, S, cnt, pos, lrow, d_nb, d_nb + S, p->d_slab_rpp, p->d_slab_rp,
-                           p->d_slab_ploc, p->d_slab_ploc4, p->slab_k, p->d_slab_ci, (VT*)p->d_slab_v);
+                           p->d_slab_ploc, ploc4, p->slab_k, p->d_slab_rmask, p->d_slab_ci, (VT*)p->d_slab_v);
         MUXI_LAUNCH_CHECK();
         for (int q = 0; q < S; ++q) {
             const long from = (long)hpad[q] + sa.nnz[q], to = (long)hpad[q] + (long)sa.tiles[q] * SIG;
@@ -704,7 +744,7 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
                                p->d_tile_off + sa.tile_base[q], s);
         if (st != SPMV_SUCCESS) return st;
     }
-    if (p->d_slab_ploc4 && ncarry > 0) {
+    if (ploc4 && ncarry > 0) {
         // Carries without a fix-up launch: a tile whose open row crosses only this tile end writes the carry to
         // the row's PC slot (descriptor bit 31); a pair crossing several tile ends gets a row (slot, t0, t1) of
         // ct and its row is summed by a wavefront in the combine.
@@ -765,7 +805,7 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
                                cross_id);
             MUXI_LAUNCH_CHECK();
             hipLaunchKernelGGL(xslab_mark_heavy_kernel, dim3(grid_for(R, 256)), dim3(256), 0, s, R, p->slab_k,
-                               p->d_slab_ploc4, cross_id, hflag);
+                               ploc4, cross_id, hflag);
             MUXI_LAUNCH_CHECK();
             st = exscan(hflag, hrank, (long)R + 1, s);
             if (st != SPMV_SUCCESS) return st;
@@ -773,15 +813,15 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
             MUXI_HIP(hipMemcpyAsync(&H, hrank + R, sizeof(int), hipMemcpyDeviceToHost, s));
             MUXI_HIP(hipStreamSynchronize(s));
             if (H > 0) {
-                MUXI_HIP(hipMalloc(&p->d_slab_heavy, sizeof(int) * 2 * (size_t)H));
+                MUXI_HIP(hipMalloc(&p->d_slab_heavy, sizeof(int) * (1 + p->slab_k) * (size_t)H));
                 hipLaunchKernelGGL(xslab_heavy_rows_kernel, dim3(grid_for(R, 256)), dim3(256), 0, s, R, p->slab_k,
-                                   hflag, hrank, p->d_slab_ploc4, p->d_slab_heavy, p->d_slab_heavy + H);
+                                   hflag, hrank, ploc4, p->d_slab_heavy, p->d_slab_heavy + H, p->d_slab_rmask);
                 MUXI_LAUNCH_CHECK();
             }
             p->nheavy = H;
             for (int* t : {cross_id, hflag, hrank}) (void)hipFreeAsync(t, s);
         }
-        if (!single.empty()) {  // the slot entries of pairs with a carry slot say so (the others skip PC)
+        if (!single.empty()) {  // the masks say which pairs have a carry slot (the others skip PC)
             std::vector<int> skey(single.size());
             for (size_t i = 0; i < single.size(); ++i) skey[i] = hk[(size_t)single[i]];
             int *d_sk = nullptr, *has_pc = nullptr;
@@ -792,19 +832,21 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
             hipLaunchKernelGGL(xslab_set_kernel, dim3(grid_for((long)skey.size(), 256)), dim3(256), 0, s, d_sk,
                                (int)skey.size(), has_pc);
             MUXI_LAUNCH_CHECK();
-            const long n = (long)p->slab_k * R;
-            hipLaunchKernelGGL(xslab_flag_pc_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, p->d_slab_ploc4,
-                               has_pc);
+            hipLaunchKernelGGL(xslab_flag_pc_kernel, dim3(grid_for(R, 256)), dim3(256), 0, s, R, p->slab_k, ploc4,
+                               has_pc, p->d_slab_rmask);
             MUXI_LAUNCH_CHECK();
             MUXI_HIP(hipStreamSynchronize(s));  // skey is host memory the copy reads
             for (int* t : {d_sk, has_pc}) (void)hipFreeAsync(t, s);
         }
     }
-    for (int* t : {cnt, flag, pos, lrow, npair, d_nb}) (void)hipFreeAsync(t, s);
+    for (int* t : {cnt, flag, pos, lrow, npair, d_nb, ploc4})
+        if (t) (void)hipFreeAsync(t, s);
     MUXI_HIP(hipStreamSynchronize(s));
     p->scratch_bytes = sizeof(int) * ints + vs * (size_t)tp + 4 * nflag + 2 * sizeof(int) * (size_t)(ntiles + 1) +
                        acc * (size_t)(ncarry + total_rows) + sizeof(int) * ((size_t)ncarry + R + 1) +
-                       (p->d_slab_ploc4 ? sizeof(int) * p->slab_k * (size_t)R : 0);
+                       (p->slab_k ? sizeof(unsigned) * (size_t)R + sizeof(int) * (((size_t)R + 63) / 64 * S) +
+                                        sizeof(int) * (1 + p->slab_k) * (size_t)p->nheavy
+                                  : 0);
     return SPMV_SUCCESS;
 }
 
@@ -823,7 +865,7 @@ spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT 
                            p->d_tile_row, x, P, carry, p->d_slab_ckey, (AT*)p->d_slab_pc);
         MUXI_LAUNCH_CHECK();
         if (ev_end) MUXI_HIP(hipEventRecord(ev_end, s));
-        if (!p->d_slab_ploc4) {
+        if (!p->slab_k) {
             hipLaunchKernelGGL(keyed_fixup_kernel<AT>, dim3(grid_for(ncarry, 256)), dim3(256), 0, s, p->d_slab_ckey,
                                ncarry, carry, P);
             MUXI_LAUNCH_CHECK();
@@ -831,17 +873,19 @@ spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT 
     } else if (ev_end) {
         MUXI_HIP(hipEventRecord(ev_end, s));
     }
-    if (p->d_slab_ploc4) {
+    if (p->slab_k) {
         const int H = p->nheavy;
-        // with no tiles there are no pairs: PC is never read then (every ploc4 entry is -1)
+        // with no tiles there are no pairs: PC is never read then (every mask is 0)
         const AT* PC = p->d_slab_pc ? (const AT*)p->d_slab_pc : P;
         const dim3 grid((H + 3) / 4 + grid_for(p->A.R, 256));
         if (p->slab_k == 4)
-            hipLaunchKernelGGL((xslab_combine4_kernel<XT, AT, 4>), grid, dim3(256), 0, s, p->A.R, p->d_slab_ploc4, P,
-                               PC, p->d_slab_ct, carry, H, p->d_slab_heavy, p->d_slab_heavy + H, y, alpha, beta);
+            hipLaunchKernelGGL((xslab_combine_mask_kernel<XT, AT, 4>), grid, dim3(256), 0, s, p->A.R, sa.S,
+                               p->d_slab_rmask, p->d_slab_wbase, P, PC, p->d_slab_ct, carry, H, p->d_slab_heavy,
+                               p->d_slab_heavy + H, y, alpha, beta);
         else
-            hipLaunchKernelGGL((xslab_combine4_kernel<XT, AT, 8>), grid, dim3(256), 0, s, p->A.R, p->d_slab_ploc4, P,
-                               PC, p->d_slab_ct, carry, H, p->d_slab_heavy, p->d_slab_heavy + H, y, alpha, beta);
+            hipLaunchKernelGGL((xslab_combine_mask_kernel<XT, AT, 8>), grid, dim3(256), 0, s, p->A.R, sa.S,
+                               p->d_slab_rmask, p->d_slab_wbase, P, PC, p->d_slab_ct, carry, H, p->d_slab_heavy,
+                               p->d_slab_heavy + H, y, alpha, beta);
     } else {
         hipLaunchKernelGGL((xslab_combine_kernel<XT, AT>), dim3(grid_for(p->A.R, 256)), dim3(256), 0, s, p->A.R,
                            p->d_slab_rpp, p->d_slab_ploc, P, y, alpha, beta);

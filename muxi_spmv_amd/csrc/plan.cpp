@@ -23,7 +23,8 @@ static void plan_free(muxi_plan* p) {
     (void)hipFree(p->d_slab);
     (void)hipFree(p->d_slab_rpp);
     (void)hipFree(p->d_slab_ckey);
-    (void)hipFree(p->d_slab_ploc4);
+    (void)hipFree(p->d_slab_rmask);
+    (void)hipFree(p->d_slab_wbase);
     (void)hipFree(p->d_slab_pc);
     (void)hipFree(p->d_slab_ct);
     (void)hipFree(p->d_slab_heavy);

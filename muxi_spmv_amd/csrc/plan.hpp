@@ -54,11 +54,12 @@ struct muxi_plan {
     int* d_slab = nullptr;
     int* d_slab_rp = nullptr;
     int* d_slab_ploc = nullptr;  // P slot of each pair, pairs in row-major order
-    int* d_slab_ploc4 = nullptr; // with at most 8 slabs: the P slots of row r at [Kr, Kr+K), -1 = none
-    int slab_k = 0;              // K: 4 (up to 4 slabs), 8 (up to 8), 0 (more: row offsets and a fix-up)
-    void* d_slab_pc = nullptr;   // with ploc4: the single carry of each pair crossing one tile end (else 0)
+    unsigned* d_slab_rmask = nullptr; // with at most 8 slabs: per row, its slabs, carry-slot and heavy bits
+    int* d_slab_wbase = nullptr;      // per wavefront of 64 rows and slab: the P slot of its first pair there
+    int slab_k = 0;                   // K: 4 (up to 4 slabs), 8 (up to 8), 0 (more: row offsets and a fix-up)
+    void* d_slab_pc = nullptr;   // with row masks: the single carry of each pair crossing one tile end (else 0)
     int* d_slab_ct = nullptr;    // pairs crossing several tile ends: (P slot, first carry tile, end tile)
-    int* d_slab_heavy = nullptr; // rows holding such a pair, then their first ploc4 entries
+    int* d_slab_heavy = nullptr; // rows holding such a pair (nheavy), then their K slot entries each
     int nheavy = 0;
     int* d_slab_ci = nullptr;
     void* d_slab_v = nullptr;
