@@ -271,7 +271,7 @@ constexpr unsigned XSLAB_HEAVY = 0x80000000u;
 // the crossing pair is -(j + 2) with ct[3j..3j+2] = (slot, t0, t1), and one
 // of the first workgroups' wavefronts sums carry[t0 .. t1) 64 at a time.
 // Pairs are added in slab order, as in every other combine.
-template <typename XT, typename AT, int K>
+template <typename XT, typename AT, int K, int J>
 __global__ __launch_bounds__(256) void xslab_combine_mask_kernel(int R, int S, const unsigned* __restrict__ rmask,
                                                                  const int* __restrict__ wbase,
                                                                  const AT* __restrict__ P, const AT* __restrict__ PC,
@@ -303,35 +303,53 @@ __global__ __launch_bounds__(256) void xslab_combine_mask_kernel(int R, int S, c
         if (lane == 0) store_y(y, r, s, alpha, beta);
         return;
     }
-    // whole wavefronts stay to the end (the ballots); rows past R have no bits
-    const long r = (long)(blockIdx.x - HB) * blockDim.x + threadIdx.x;
-    const unsigned m = r < R ? __builtin_nontemporal_load(rmask + r) : 0u;
-    const bool mine = r < R && !(m & XSLAB_HEAVY);
-    // the wavefront's slot bases, loaded beside the mask (a wavefront wholly past R reads wavefront 0's)
-    const long w = (r & ~63L) < R ? r >> 6 : 0;
-    int base[K];
-#pragma unroll
-    for (int q = 0; q < K; ++q) base[q] = q < S ? wbase[w * S + q] : 0;
+    // Whole wavefronts stay to the end (the ballots); rows past R have no bits. A wavefront takes J groups of
+    // 64 consecutive rows, so that J masks, then J * K partial loads, are in flight together.
+    const int lane = threadIdx.x & 63;
+    const long g0 = ((long)(blockIdx.x - HB) * (blockDim.x >> 6) + (threadIdx.x >> 6)) * J;  // first row group
     const unsigned long long lt = __lanemask_lt();
-    int slot[K];
-    bool use[K];
+    unsigned m[J];
 #pragma unroll
-    for (int q = 0; q < K; ++q) {
-        const bool has = q < S && ((m >> q) & 1u);
-        slot[q] = base[q] + __popcll(__ballot(has) & lt);
-        use[q] = has && mine;
+    for (int j = 0; j < J; ++j) {
+        const long r = (g0 + j) * 64 + lane;
+        m[j] = r < R ? __builtin_nontemporal_load(rmask + r) : 0u;
     }
-    AT v[K], c[K];
+    int slot[J][K];
 #pragma unroll
-    for (int q = 0; q < K; ++q) {
-        v[q] = use[q] ? ntl(P + slot[q]) : AT(0);
-        c[q] = use[q] && ((m >> (8 + q)) & 1u) ? PC[slot[q]] : AT(0);
+    for (int j = 0; j < J; ++j) {
+        const long w = (g0 + j) * 64 < R ? g0 + j : 0;  // a group wholly past R reads group 0's bases
+#pragma unroll
+        for (int q = 0; q < K; ++q) slot[j][q] = q < S ? wbase[w * S + q] : 0;
     }
-    AT s = AT(0);
+    bool use[J][K];
 #pragma unroll
-    for (int q = 0; q < K; ++q)
-        if (use[q]) s += v[q] + c[q];
-    if (mine) store_y(y, r, s, alpha, beta);
+    for (int j = 0; j < J; ++j) {
+        const bool mine = !(m[j] & XSLAB_HEAVY);
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const bool has = q < S && ((m[j] >> q) & 1u);
+            slot[j][q] += __popcll(__ballot(has) & lt);
+            use[j][q] = has && mine;
+        }
+    }
+    AT v[J][K], c[J][K];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            v[j][q] = use[j][q] ? ntl(P + slot[j][q]) : AT(0);
+            c[j][q] = use[j][q] && ((m[j] >> (8 + q)) & 1u) ? PC[slot[j][q]] : AT(0);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const long r = (g0 + j) * 64 + lane;
+        AT s = AT(0);
+#pragma unroll
+        for (int q = 0; q < K; ++q)
+            if (use[j][q]) s += v[j][q] + c[j][q];
+        if (r < R && !(m[j] & XSLAB_HEAVY)) store_y(y, r, s, alpha, beta);
+    }
 }
 
 // The pair open at the end of each tile of one slab (-1 = none): tile t ends
@@ -877,13 +895,15 @@ spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT 
         const int H = p->nheavy;
         // with no tiles there are no pairs: PC is never read then (every mask is 0)
         const AT* PC = p->d_slab_pc ? (const AT*)p->d_slab_pc : P;
-        const dim3 grid((H + 3) / 4 + grid_for(p->A.R, 256));
+        // J row groups of 64 per wavefront: 4 with four slot bases, 2 with eight
         if (p->slab_k == 4)
-            hipLaunchKernelGGL((xslab_combine_mask_kernel<XT, AT, 4>), grid, dim3(256), 0, s, p->A.R, sa.S,
+            hipLaunchKernelGGL((xslab_combine_mask_kernel<XT, AT, 4, 4>),
+                               dim3((H + 3) / 4 + grid_for(p->A.R, 256 * 4)), dim3(256), 0, s, p->A.R, sa.S,
                                p->d_slab_rmask, p->d_slab_wbase, P, PC, p->d_slab_ct, carry, H, p->d_slab_heavy,
                                p->d_slab_heavy + H, y, alpha, beta);
         else
-            hipLaunchKernelGGL((xslab_combine_mask_kernel<XT, AT, 8>), grid, dim3(256), 0, s, p->A.R, sa.S,
+            hipLaunchKernelGGL((xslab_combine_mask_kernel<XT, AT, 8, 2>),
+                               dim3((H + 3) / 4 + grid_for(p->A.R, 256 * 2)), dim3(256), 0, s, p->A.R, sa.S,
                                p->d_slab_rmask, p->d_slab_wbase, P, PC, p->d_slab_ct, carry, H, p->d_slab_heavy,
                                p->d_slab_heavy + H, y, alpha, beta);
     } else {
