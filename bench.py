@@ -43,8 +43,8 @@ CANDIDATES = [("merge", 4), ("merge", 8), ("merge", 16), ("csr5", 256), ("csr5",
 # kernel reads); the runs and commands are in profiles/xslab_spmv_powerlaw_2m_fp32.md and
 # profiles/ell_spmv_laplacian_4096_fp64.md. Configurations not measured report null.
 MEASURED_TRAFFIC = {
-    ("powerlaw_2m_fp32", "xslab", 1028): int((66338.6 * 2.0 + 15487.5 * 1.0) * 1024),
-    ("powerlaw_2m_fp32", "xslab", 1032): int((68212.4 * 2.0 + 15875.7 * 1.0) * 1024),
+    ("powerlaw_2m_fp32", "xslab", 1028): int((66336.6 * 2.0 + 15477.4 * 1.0) * 1024),
+    ("powerlaw_2m_fp32", "xslab", 1032): int((68168.4 * 2.0 + 15791.2 * 1.0) * 1024),
     ("laplacian_4096_fp64", "ell", 5): int((483539.5 * 2.0 + 131072.0 * 1.0) * 1024),
 }
 
