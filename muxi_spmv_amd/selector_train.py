@@ -15,7 +15,8 @@ python -m muxi_spmv_amd.selector_train [--train 2000] [--test 400] [--db PATH] [
 2. The autotune sweep times every candidate kernel/configuration cold and
    appends one JSON record per matrix to the DB (resumable: a record is
    only topped up with the candidates it lacks; --skip leaves kernels out of
-   a call, --retime times kernels again after they changed).
+   a call, --retime times kernels again after they changed; with
+   --retime-tag the records re-timed are tagged, so a later call resumes).
 3. The MLP (HIP forward/backward/Adam kernels) is fit on the training
    records to predict log(t_candidate / t_best), --ensemble times from
    different seeds; the shipped network averages them (one wider MLP with
@@ -92,17 +93,20 @@ def load_db(path: Path) -> dict[int, dict]:
     return recs
 
 
-def _todo(rec, cands, retime):
-    """Candidates still to time on a record: feasible ones it lacks, plus those named for re-timing."""
+def _todo(rec, cands, retime, retime_tag=""):
+    """Candidates still to time on a record: feasible ones it lacks, plus those named for re-timing (unless the
+    record already carries retime_tag, so that a re-timing split over several calls resumes)."""
     if rec is None:
         return list(cands)
     f = np.array(rec["features"])
+    if retime_tag and retime_tag in rec.get("retimed", ()):
+        retime = ()
     return [(k, p) for k, p in cands
             if tune.feasible(k, f) and (f"{k}:{p}" not in rec["times"] or k in retime)]
 
 
 def run_sweeps(indices, db_path: Path, rows_min, rows_max, reps, log_every=25, time_budget_s=None,
-               skip=(), retime=()):
+               skip=(), retime=(), retime_tag=""):
     """Sweeps matrices into the DB (one JSON line per matrix; a later line for the same index replaces an
     earlier one). Kernels in `skip` are not timed; kernels in `retime` are timed again on records that
     already hold them; candidates a record lacks are added to it."""
@@ -113,7 +117,7 @@ def run_sweeps(indices, db_path: Path, rows_min, rows_max, reps, log_every=25, t
     n_new = 0
     with open(db_path, "a") as f:
         for i in indices:
-            todo = _todo(done.get(i), cands, retime)
+            todo = _todo(done.get(i), cands, retime, retime_tag)
             if not todo:
                 continue
             if time_budget_s is not None and time.time() - t0 > time_budget_s:
@@ -127,6 +131,11 @@ def run_sweeps(indices, db_path: Path, rows_min, rows_max, reps, log_every=25, t
             times = dict(done[i]["times"]) if i in done else {}
             times.update(tune.sweep(A, x, y, fl, reps=reps, candidates=todo, f=feats))
             rec = dict(spec, nnz=int(A.nnz), features=feats.tolist(), times=times)
+            tags = list(done[i].get("retimed", [])) if i in done else []
+            if retime_tag and retime and retime_tag not in tags:
+                tags.append(retime_tag)
+            if tags:
+                rec["retimed"] = tags
             f.write(json.dumps(rec) + "\n")
             f.flush()
             done[i] = rec
@@ -212,6 +221,7 @@ def main(argv=None) -> int:
     ap.add_argument("--time-budget", type=float, default=None, help="seconds of sweeping in this call")
     ap.add_argument("--skip", default="", help="comma-separated kernels not to time in this call")
     ap.add_argument("--retime", default="", help="comma-separated kernels to time again on existing records")
+    ap.add_argument("--retime-tag", default="", help="records re-timed under this tag are not re-timed again")
     a = ap.parse_args(argv)
     torch.cuda.set_device(0)
     db = Path(a.db)
@@ -225,7 +235,8 @@ def main(argv=None) -> int:
         # interleave so that a partial run covers both splits
         order = list(range(a.train, n)) + list(range(a.train)) + big_test + big_train + f64_test + f64_train
         run_sweeps(order, db, a.rows_min, a.rows_max, a.reps, time_budget_s=a.time_budget,
-                   skip=tuple(k for k in a.skip.split(",") if k), retime=tuple(k for k in a.retime.split(",") if k))
+                   skip=tuple(k for k in a.skip.split(",") if k), retime=tuple(k for k in a.retime.split(",") if k),
+                   retime_tag=a.retime_tag)
         if a.sweep_only:
             return 0
     recs = load_db(db)

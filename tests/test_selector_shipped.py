@@ -72,3 +72,26 @@ def test_shipped_selector_regret_on_held_out_matrices(precision):
     assert np.median(learned) < np.median(fixed[np.isfinite(fixed)])
     assert np.median(learned) < np.median(rule[np.isfinite(rule)])
     assert np.mean(learned) < 0.5 * np.mean(fixed[np.isfinite(fixed)])
+
+
+def test_retime_tag_resumes():
+    """--retime times the named kernels again on every record, except records already tagged with the
+    --retime-tag of the same re-timing (so a re-timing split over several calls resumes where it stopped);
+    candidates a record lacks are always added."""
+    from muxi_spmv_amd import selector_train as st
+    from muxi_spmv_amd import tune
+
+    f = [1e5, 1e6, 10.0, 4.0, 15.0, 0.01, 0.1, 0.0]
+    cands = [("xslab", 1028), ("merge", 4), ("ell", 0)]
+    times = {"xslab:1028": 0.1, "merge:4": 0.2, "ell:0": 0.3}
+    rec = {"features": f, "times": dict(times)}
+    assert st._todo(None, cands, ()) == cands
+    assert st._todo(rec, cands, ()) == []
+    assert st._todo(rec, cands, ("xslab",)) == [("xslab", 1028)]
+    assert st._todo(rec, cands, ("xslab", "ell"), "t1") == [("xslab", 1028), ("ell", 0)]
+    tagged = dict(rec, retimed=["t1"])
+    assert st._todo(tagged, cands, ("xslab", "ell"), "t1") == []
+    assert st._todo(tagged, cands, ("xslab",), "t2") == [("xslab", 1028)]
+    missing = dict(tagged, times={"merge:4": 0.2})
+    assert st._todo(missing, cands, ("xslab",), "t1") == [("xslab", 1028), ("ell", 0)]
+    assert tune.feasible("ell", np.array(f))
