@@ -18,7 +18,7 @@ ZOO = matrix_zoo()
 KCONF = [("scalar", 0), ("vector", 0)] + [("vwave", k) for k in (1, 2, 4, 8, 16, 32, 64)] + \
         [("merge", k) for k in (2, 4, 8, 16)] + [("csr5", s) for s in (64, 128, 256, 512, 1024)] + [("ell", 0)] + \
         [("adaptive", b) for b in (256, 512, 1024, 2048)] + \
-        [("xslab", (S << 8) | ipt) for S, ipt in ((8, 4), (8, 8), (1, 8), (3, 8), (16, 4), (2, 4), (4, 4))] + \
+        [("xslab", (S << 8) | ipt) for S, ipt in ((8, 4), (8, 8), (1, 8), (3, 8), (6, 4), (16, 4), (2, 4), (4, 4))] + \
         [("xslab", 0)]
 
 
