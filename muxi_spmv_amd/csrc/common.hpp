@@ -209,3 +209,12 @@ inline unsigned grid_for(long work, int per_block) {
 }
 
 }  // namespace muxi
+
+namespace muxi {
+// A row block of a larger matrix: rows [row_base, row_base + numRows) of a matrix of rows_full rows, numbered
+// from 0 within the block, with the full matrix's columns (the host path's chunks). The features measure the
+// band against the full matrix's diagonal, so a block of a banded matrix looks banded to the selector.
+spmv_status_t matrix_features_rows(const muxi_csr_t* A, void* stream, long row_base, long rows_full, double* out);
+spmv_status_t selector_pick_rows(const muxi_csr_t* A, spmv_type_t xtype, void* stream, long row_base,
+                                 long rows_full, int* kernel, int* param0);
+}  // namespace muxi

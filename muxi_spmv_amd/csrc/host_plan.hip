@@ -10,6 +10,7 @@
 
 #include "common.hpp"
 #include "muxi_spmv/muxi_spmv_host.h"
+#include "plan.hpp"
 #include "muxi_spmv/muxi_spmv_prep.h"
 
 using namespace muxi;
@@ -170,7 +171,8 @@ static spmv_status_t host_plan_create_impl(const muxi_csr_t* A, spmv_type_t vt, 
         sub.d_rowPtr = rpk;
         sub.d_colIdx = A->d_colIdx + n0;
         sub.d_values = (char*)A->d_values + vs * (size_t)n0;
-        HP_TRY(muxi_plan_create(&sub, vt, xt, kernel, param0, 0, h->s_c, &h->plans[k]));
+        // a row block of A: AUTO sees its band against A's diagonal, ELL offsets come from A's rows
+        HP_TRY(plan_create_rows(&sub, vt, xt, kernel, param0, 0, h->s_c, a, R, &h->plans[k]));
     }
     std::vector<int> cmax(K, -1);
     HP_HIP(hipMemcpyAsync(cmax.data(), h->d_colmax, sizeof(int) * K, hipMemcpyDeviceToHost, h->s_c));

@@ -66,12 +66,14 @@ spmv_status_t launch_ell(int R, int C, int width, const int* d_col, const VT* d_
 // the plan's layout: a row's slots grouped by four (kernels_ell.hip); with
 // off16 the columns are 16-bit offsets from the row (d_col holds shorts),
 // allowed when launch_ell_offsets_fit left *d_bad == 0
-spmv_status_t launch_ell_offsets_fit(const DevCsr& A, int width, int* d_bad, hipStream_t s);
+// (offsets from row + rb: rb is a row block's first row in its full matrix, 0 for a whole matrix)
+spmv_status_t launch_ell_offsets_fit(const DevCsr& A, int width, int rb, int* d_bad, hipStream_t s);
 template <typename VT>
-spmv_status_t launch_csr_to_ellg(const DevCsr& A, int width, bool off16, void* d_col, VT* d_val, hipStream_t s);
+spmv_status_t launch_csr_to_ellg(const DevCsr& A, int width, bool off16, int rb, void* d_col, VT* d_val,
+                                 hipStream_t s);
 template <typename VT, typename XT, typename AT, bool COMP>
-spmv_status_t launch_ellg(int R, int C, int width, bool off16, const void* d_col, const VT* d_val, const XT* x,
-                          XT* y, AT alpha, AT beta, hipStream_t s);
+spmv_status_t launch_ellg(int R, int C, int width, bool off16, int rb, const void* d_col, const VT* d_val,
+                          const XT* x, XT* y, AT alpha, AT beta, hipStream_t s);
 
 // ---- SpMM over merge-path partitions (kernels_spmm.hip); carry holds P*k elements.
 int spmm_groups(int k);

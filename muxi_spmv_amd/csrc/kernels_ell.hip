@@ -168,32 +168,33 @@ __device__ __forceinline__ void ell_put(short* p, int col, long row) {
 }
 
 // *bad = 1 when a column among the first `width` of some row lies further
-// than 32767 from its row (the 16-bit offsets cannot hold it).
-__global__ void ell_offsets_fit_kernel(int R, int width, const int* __restrict__ rp, const int* __restrict__ ci,
-                                       int* __restrict__ bad) {
+// than 32767 from its row (the 16-bit offsets cannot hold it). Row r of a
+// row block is row r + rb of its full matrix; offsets are taken from that.
+__global__ void ell_offsets_fit_kernel(int R, int width, int rb, const int* __restrict__ rp,
+                                       const int* __restrict__ ci, int* __restrict__ bad) {
     const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int b = 0;
     if (row < R) {
         const int s0 = rp[row], n = rp[row + 1] - s0, len = n < width ? n : width;
         for (int i = 0; i < len && !b; ++i) {
-            const long d = (long)ci[s0 + i] - row;
+            const long d = (long)ci[s0 + i] - (row + rb);
             b = d < -32767 || d > 32767;
         }
     }
     if (__any(b) && (threadIdx.x & 63) == 0) atomicOr(bad, 1);
 }
 
-spmv_status_t launch_ell_offsets_fit(const DevCsr& A, int width, int* d_bad, hipStream_t s) {
+spmv_status_t launch_ell_offsets_fit(const DevCsr& A, int width, int rb, int* d_bad, hipStream_t s) {
     MUXI_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), s));
     if (A.R <= 0 || width <= 0) return SPMV_SUCCESS;
-    hipLaunchKernelGGL(ell_offsets_fit_kernel, dim3(grid_for(A.R, 256)), dim3(256), 0, s, A.R, width, A.rp, A.ci,
-                       d_bad);
+    hipLaunchKernelGGL(ell_offsets_fit_kernel, dim3(grid_for(A.R, 256)), dim3(256), 0, s, A.R, width, rb, A.rp,
+                       A.ci, d_bad);
     MUXI_LAUNCH_CHECK();
     return SPMV_SUCCESS;
 }
 
 template <typename VT, typename CT>
-__global__ void csr_to_ellg_kernel(int R, int width, EllGroups eg, const int* __restrict__ rp,
+__global__ void csr_to_ellg_kernel(int R, int width, int rb, EllGroups eg, const int* __restrict__ rp,
                                    const int* __restrict__ ci, const VT* __restrict__ v, CT* __restrict__ ecol,
                                    VT* __restrict__ eval) {
     const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -202,10 +203,10 @@ __global__ void csr_to_ellg_kernel(int R, int width, EllGroups eg, const int* __
     for (int i = 0; i < width; ++i) {
         const long idx = ellg_index(eg, row, i);
         if (i < len) {
-            ell_put(ecol + idx, ci[b + i], row);
+            ell_put(ecol + idx, ci[b + i], row + rb);
             eval[idx] = v[b + i];
         } else {
-            ell_put(ecol + idx, -1, row);
+            ell_put(ecol + idx, -1, row + rb);
             VT z;
             __builtin_memset(&z, 0, sizeof(VT));
             eval[idx] = z;
@@ -214,14 +215,15 @@ __global__ void csr_to_ellg_kernel(int R, int width, EllGroups eg, const int* __
 }
 
 template <typename VT>
-spmv_status_t launch_csr_to_ellg(const DevCsr& A, int width, bool off16, void* d_col, VT* d_val, hipStream_t s) {
+spmv_status_t launch_csr_to_ellg(const DevCsr& A, int width, bool off16, int rb, void* d_col, VT* d_val,
+                                 hipStream_t s) {
     if (A.R <= 0 || width <= 0) return SPMV_SUCCESS;
     if (off16)
         hipLaunchKernelGGL((csr_to_ellg_kernel<VT, short>), dim3(grid_for(A.R, 256)), dim3(256), 0, s, A.R, width,
-                           ell_groups(A.R, width), A.rp, A.ci, (const VT*)A.v, (short*)d_col, d_val);
+                           rb, ell_groups(A.R, width), A.rp, A.ci, (const VT*)A.v, (short*)d_col, d_val);
     else
         hipLaunchKernelGGL((csr_to_ellg_kernel<VT, int>), dim3(grid_for(A.R, 256)), dim3(256), 0, s, A.R, width,
-                           ell_groups(A.R, width), A.rp, A.ci, (const VT*)A.v, (int*)d_col, d_val);
+                           rb, ell_groups(A.R, width), A.rp, A.ci, (const VT*)A.v, (int*)d_col, d_val);
     MUXI_LAUNCH_CHECK();
     return SPMV_SUCCESS;
 }
@@ -260,7 +262,8 @@ __device__ __forceinline__ Vec<T, N> ldvec(const T* p) {
 // loads first, then all gathers, then the products. Padding (col < 0) gathers
 // x[0] and adds nothing. CT is the stored column type (int or short offsets).
 template <typename VT, typename XT, typename AT, bool COMP, typename CT>
-__global__ __launch_bounds__(256) void ellg_spmv_kernel(int R, int C, EllGroups eg, const CT* __restrict__ ecol,
+__global__ __launch_bounds__(256) void ellg_spmv_kernel(int R, int C, int rb, EllGroups eg,
+                                                        const CT* __restrict__ ecol,
                                                         const VT* __restrict__ eval, const XT* __restrict__ x,
                                                         XT* __restrict__ y, AT alpha, AT beta) {
     const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -280,23 +283,23 @@ __global__ __launch_bounds__(256) void ellg_spmv_kernel(int R, int C, EllGroups 
             if (hasA) {
                 const Vec<CT, 4> w = ldvec<CT, 4>(ecol + 4L * eg.R * g + 4 * row);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) cA.v[j] = ell_col(w.v[j], row);
+                for (int j = 0; j < 4; ++j) cA.v[j] = ell_col(w.v[j], row + rb);
                 vA = ldvec<VT, 4>(eval + 4L * eg.R * g + 4 * row);
             }
             if (hasB) {
                 const Vec<CT, 4> w = ldvec<CT, 4>(ecol + 4L * eg.R * (g + 1) + 4 * row);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) cB.v[j] = ell_col(w.v[j], row);
+                for (int j = 0; j < 4; ++j) cB.v[j] = ell_col(w.v[j], row + rb);
                 vB = ldvec<VT, 4>(eval + 4L * eg.R * (g + 1) + 4 * row);
             }
             if (has2) {
                 const Vec<CT, 2> w = ldvec<CT, 2>(ecol + eg.base2 + 2 * row);
 #pragma unroll
-                for (int j = 0; j < 2; ++j) c2.v[j] = ell_col(w.v[j], row);
+                for (int j = 0; j < 2; ++j) c2.v[j] = ell_col(w.v[j], row + rb);
                 v2 = ldvec<VT, 2>(eval + eg.base2 + 2 * row);
             }
             if (has1) {
-                c1 = ell_col(ntl(ecol + eg.base1 + row), row);
+                c1 = ell_col(ntl(ecol + eg.base1 + row), row + rb);
                 v1 = ntl(eval + eg.base1 + row);
             }
             // gathers only for the segments present (uniform branches: no gather instruction is wasted)
@@ -337,22 +340,22 @@ __global__ __launch_bounds__(256) void ellg_spmv_kernel(int R, int C, EllGroups 
 }
 
 template <typename VT, typename XT, typename AT, bool COMP>
-spmv_status_t launch_ellg(int R, int C, int width, bool off16, const void* d_col, const VT* d_val, const XT* x,
-                          XT* y, AT alpha, AT beta, hipStream_t s) {
+spmv_status_t launch_ellg(int R, int C, int width, bool off16, int rb, const void* d_col, const VT* d_val,
+                          const XT* x, XT* y, AT alpha, AT beta, hipStream_t s) {
     if (R <= 0) return SPMV_SUCCESS;
     if (off16)
         hipLaunchKernelGGL((ellg_spmv_kernel<VT, XT, AT, COMP, short>), dim3(grid_for(R, 256)), dim3(256), 0, s, R, C,
-                           ell_groups(R, width), (const short*)d_col, d_val, x, y, alpha, beta);
+                           rb, ell_groups(R, width), (const short*)d_col, d_val, x, y, alpha, beta);
     else
         hipLaunchKernelGGL((ellg_spmv_kernel<VT, XT, AT, COMP, int>), dim3(grid_for(R, 256)), dim3(256), 0, s, R, C,
-                           ell_groups(R, width), (const int*)d_col, d_val, x, y, alpha, beta);
+                           rb, ell_groups(R, width), (const int*)d_col, d_val, x, y, alpha, beta);
     MUXI_LAUNCH_CHECK();
     return SPMV_SUCCESS;
 }
 
-template spmv_status_t launch_csr_to_ellg<float>(const DevCsr&, int, bool, void*, float*, hipStream_t);
-template spmv_status_t launch_csr_to_ellg<double>(const DevCsr&, int, bool, void*, double*, hipStream_t);
-template spmv_status_t launch_csr_to_ellg<bf16_t>(const DevCsr&, int, bool, void*, bf16_t*, hipStream_t);
+template spmv_status_t launch_csr_to_ellg<float>(const DevCsr&, int, bool, int, void*, float*, hipStream_t);
+template spmv_status_t launch_csr_to_ellg<double>(const DevCsr&, int, bool, int, void*, double*, hipStream_t);
+template spmv_status_t launch_csr_to_ellg<bf16_t>(const DevCsr&, int, bool, int, void*, bf16_t*, hipStream_t);
 
 template spmv_status_t launch_csr_to_ell<float>(const DevCsr&, int, int*, float*, hipStream_t);
 template spmv_status_t launch_csr_to_ell<double>(const DevCsr&, int, int*, double*, hipStream_t);
@@ -361,8 +364,8 @@ template spmv_status_t launch_csr_to_ell<bf16_t>(const DevCsr&, int, int*, bf16_
 #define INST(VT, XT, AT, COMP)                                                                                       \
     template spmv_status_t launch_ell<VT, XT, AT, COMP>(int, int, int, const int*, const VT*, const XT*, XT*, AT, AT, \
                                                         hipStream_t);                                                    \
-    template spmv_status_t launch_ellg<VT, XT, AT, COMP>(int, int, int, bool, const void*, const VT*, const XT*, XT*,  \
-                                                         AT, AT, hipStream_t);
+    template spmv_status_t launch_ellg<VT, XT, AT, COMP>(int, int, int, bool, int, const void*, const VT*, const XT*, \
+                                                         XT*, AT, AT, hipStream_t);
 MUXI_FOR_EACH_COMBO(INST)
 #undef INST
 

@@ -173,7 +173,10 @@ struct FeatPart {
     int len_max, pad;
 };
 
-__global__ __launch_bounds__(kFeatBlock) void features_kernel(int R, int C, const int* __restrict__ rp,
+// Row r of the block is row r + row_base of the full matrix, whose diagonal runs through column
+// (r + row_base) * cen_scale (cen_scale = C / rows of the full matrix).
+__global__ __launch_bounds__(kFeatBlock) void features_kernel(int R, long row_base, double cen_scale,
+                                                              const int* __restrict__ rp,
                                                               const int* __restrict__ ci, FeatPart* __restrict__ out) {
     __shared__ double s_d[5][kFeatBlock / 64];
     __shared__ int s_m[kFeatBlock / 64];
@@ -186,8 +189,8 @@ __global__ __launch_bounds__(kFeatBlock) void features_kernel(int R, int C, cons
         lq = (double)len * len;
         ec = len == 0;
         mx = len;
-        const double cen = (double)r * (double)C / (double)R;
-        const long rb = r / 32;
+        const double cen = (double)(r + row_base) * cen_scale;
+        const long rb = (r + row_base) / 32;
         for (int i = b; i < e; ++i) {
             const int c = ci[i];
             bs += fabs((double)c - cen);
@@ -395,8 +398,10 @@ extern "C" spmv_status_t muxi_gen_vector(void* x, spmv_type_t t, long n, uint64_
     return SPMV_SUCCESS;
 }
 
-extern "C" spmv_status_t muxi_matrix_features(const muxi_csr_t* A, void* stream, double* out) {
+spmv_status_t muxi::matrix_features_rows(const muxi_csr_t* A, void* stream, long row_base, long rows_full,
+                                         double* out) {
     if (!A || !out || A->numRows < 0 || (A->numRows > 0 && !A->d_rowPtr)) return SPMV_ERROR_INVALID_MATRIX;
+    if (row_base < 0 || rows_full < row_base + A->numRows) return SPMV_ERROR_INVALID_MATRIX;
     const int R = A->numRows;
     for (int i = 0; i < 8; ++i) out[i] = 0;
     out[0] = R;
@@ -406,8 +411,8 @@ extern "C" spmv_status_t muxi_matrix_features(const muxi_csr_t* A, void* stream,
     const int nb = (int)grid_for(R, kFeatBlock);
     FeatPart* d_part = nullptr;
     MUXI_HIP(hipMallocAsync(&d_part, sizeof(FeatPart) * nb, s));
-    hipLaunchKernelGGL(features_kernel, dim3(nb), dim3(kFeatBlock), 0, s, R, A->numCols, A->d_rowPtr, A->d_colIdx,
-                       d_part);
+    hipLaunchKernelGGL(features_kernel, dim3(nb), dim3(kFeatBlock), 0, s, R, row_base,
+                       (double)A->numCols / (double)rows_full, A->d_rowPtr, A->d_colIdx, d_part);
     MUXI_LAUNCH_CHECK();
     std::vector<FeatPart> h(nb);
     MUXI_HIP(hipMemcpyAsync(h.data(), d_part, sizeof(FeatPart) * nb, hipMemcpyDeviceToHost, s));
@@ -431,6 +436,11 @@ extern "C" spmv_status_t muxi_matrix_features(const muxi_csr_t* A, void* stream,
     out[6] = A->nnz > 0 ? dc / A->nnz : 0.0;
     out[7] = ec / R;
     return SPMV_SUCCESS;
+}
+
+extern "C" spmv_status_t muxi_matrix_features(const muxi_csr_t* A, void* stream, double* out) {
+    if (!A) return SPMV_ERROR_INVALID_MATRIX;
+    return matrix_features_rows(A, stream, 0, A->numRows, out);
 }
 
 extern "C" spmv_status_t muxi_flush_caches(void* buf, size_t bytes, void* stream) {

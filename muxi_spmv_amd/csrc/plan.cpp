@@ -53,7 +53,8 @@ static void plan_free(muxi_plan* p) {
 static bool pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (v - 1)) == 0; }
 
 static spmv_status_t plan_create_impl(const muxi_csr_t* Ain, spmv_type_t vt, spmv_type_t xt, muxi_kernel_t kernel,
-                                          int param0, int compensated, void* stream, muxi_plan** out) {
+                                          int param0, int compensated, void* stream, muxi_plan** out,
+                                          long row_base = 0, long rows_full = -1) {
     if (!out) {
         set_error("plan output pointer is NULL");
         return SPMV_ERROR_INVALID_HANDLE;
@@ -87,12 +88,19 @@ static spmv_status_t plan_create_impl(const muxi_csr_t* Ain, spmv_type_t vt, spm
     p->vtype = vt;
     p->xtype = xt;
     p->comp = compensated != 0;
+    if (rows_full < 0) rows_full = Ain->numRows;
+    if (row_base < 0 || row_base + Ain->numRows > rows_full || rows_full > 2147483647L) {
+        set_error("invalid row block");
+        plan_free(p);
+        return SPMV_ERROR_INVALID_MATRIX;
+    }
+    p->row_base = (int)row_base;
     PLAN_HIP(hipGetDevice(&p->device));
     const int R = Ain->numRows, N = Ain->nnz;
 
     if (kernel == MUXI_KERNEL_AUTO) {
         int k = MUXI_KERNEL_MERGE, q = 4;
-        PLAN_TRY(muxi_selector_pick(Ain, xt, stream, &k, &q));
+        PLAN_TRY(selector_pick_rows(Ain, xt, stream, row_base, rows_full, &k, &q));
         if (p->comp && k == MUXI_KERNEL_XSLAB) {  // no compensated column-slab variant
             k = MUXI_KERNEL_MERGE;
             q = 4;
@@ -229,7 +237,7 @@ static spmv_status_t plan_create_impl(const muxi_csr_t* Ain, spmv_type_t vt, spm
                 {
                     int* d_bad = nullptr;
                     PLAN_HIP(hipMalloc(&d_bad, sizeof(int)));
-                    spmv_status_t st = launch_ell_offsets_fit(p->A, width, d_bad, s);
+                    spmv_status_t st = launch_ell_offsets_fit(p->A, width, p->row_base, d_bad, s);
                     if (st == SPMV_SUCCESS &&
                         (hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
                          hipStreamSynchronize(s) != hipSuccess))
@@ -244,7 +252,7 @@ static spmv_status_t plan_create_impl(const muxi_csr_t* Ain, spmv_type_t vt, spm
                 p->scratch_bytes = (csize + vsize(vt)) * slots;
                 PLAN_TRY(with_types(vt, xt, p->comp, [&](auto tag) {
                     using T = decltype(tag);
-                    return launch_csr_to_ellg<typename T::VT>(p->A, width, p->ecol16, p->d_ecol,
+                    return launch_csr_to_ellg<typename T::VT>(p->A, width, p->ecol16, p->row_base, p->d_ecol,
                                                               (typename T::VT*)p->d_eval, s);
                 }));
             }
@@ -391,7 +399,7 @@ static spmv_status_t plan_exec(const muxi_plan* p, const void* x, void* y, doubl
                 return launch_csr5<VT, XT, AT, C>(p->A, p->param0, p->d_tile_row, p->T, p->d_carry, xx, yy, a, b, s,
                                                   ev1);
             case MUXI_KERNEL_ELL:
-                return launch_ellg<VT, XT, AT, C>(p->A.R, p->A.C, p->width, p->ecol16, p->d_ecol,
+                return launch_ellg<VT, XT, AT, C>(p->A.R, p->A.C, p->width, p->ecol16, p->row_base, p->d_ecol,
                                                   (const VT*)p->d_eval, xx, yy, a, b, s);
             case MUXI_KERNEL_XSLAB:
                 if constexpr (C) {
@@ -454,6 +462,14 @@ extern "C" spmv_status_t muxi_event_elapsed_ms(void* start, void* end, float* ms
 extern "C" spmv_status_t muxi_plan_create(const muxi_csr_t* Ain, spmv_type_t vt, spmv_type_t xt, muxi_kernel_t kernel, int param0,
                                           int compensated, void* stream, muxi_plan** out) {
     return no_throw("muxi_plan_create", [&] { return plan_create_impl(Ain, vt, xt, kernel, param0, compensated, stream, out); });
+}
+
+spmv_status_t muxi::plan_create_rows(const muxi_csr_t* A, spmv_type_t vt, spmv_type_t xt, muxi_kernel_t kernel,
+                                     int param0, int compensated, void* stream, long row_base, long rows_full,
+                                     muxi_plan** out) {
+    return no_throw("plan_create_rows", [&] {
+        return plan_create_impl(A, vt, xt, kernel, param0, compensated, stream, out, row_base, rows_full);
+    });
 }
 
 extern "C" spmv_status_t muxi_plan_create_spmm(const muxi_csr_t* Ain, spmv_type_t vt, spmv_type_t xt, int nrhs, int ipt,

@@ -36,8 +36,9 @@ struct muxi_plan {
     void* d_carry = nullptr;
     // ELL
     int width = 0;
-    void* d_ecol = nullptr;  // int columns, or short offsets col - row when ecol16
+    void* d_ecol = nullptr;  // int columns, or short offsets col - (row + row_base) when ecol16
     bool ecol16 = false;
+    int row_base = 0;        // a row block's first row in the full matrix (plan_create_rows), else 0
     void* d_eval = nullptr;
     // adaptive row blocks: blk_row[nblocks+1], blk_chunk[nblocks], long_row[nlong], long_first[2*nlong]
     int nblocks = 0;
@@ -112,6 +113,13 @@ inline int xslab_default_slabs(long cols, int xbytes) {
     while (S < 8 && b > (long)S * (2L << 20)) S <<= 1;
     return S;
 }
+
+// A plan for a row block (rows [row_base, row_base + A->numRows) of a matrix of rows_full rows, numbered from 0,
+// full-matrix columns): AUTO picks from the block's features against the full diagonal, and the ELL plan takes
+// its 16-bit column offsets from the full-matrix row. muxi_plan_create is the case row_base = 0,
+// rows_full = A->numRows.
+spmv_status_t plan_create_rows(const muxi_csr_t* A, spmv_type_t vt, spmv_type_t xt, muxi_kernel_t kernel, int param0,
+                               int compensated, void* stream, long row_base, long rows_full, muxi_plan** out);
 
 // column-slab plan and launcher (kernels_xslab.hip); param0 = (slabs << 8) | nonzeros per lane (4 or 8)
 spmv_status_t xslab_build(muxi_plan* p, int S, int ipt, spmv_type_t vt, hipStream_t s);

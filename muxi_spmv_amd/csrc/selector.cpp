@@ -155,11 +155,11 @@ static void fallback_rule(const double* f, double x_bytes, int* kernel, int* par
     }
 }
 
-extern "C" spmv_status_t muxi_selector_pick(const muxi_csr_t* A, spmv_type_t xtype, void* stream, int* kernel,
-                                            int* param0) {
+spmv_status_t muxi::selector_pick_rows(const muxi_csr_t* A, spmv_type_t xtype, void* stream, long row_base,
+                                       long rows_full, int* kernel, int* param0) {
     if (!A || !kernel || !param0) return SPMV_ERROR_INVALID_MATRIX;
     double f[8];
-    spmv_status_t st = muxi_matrix_features(A, stream, f);
+    spmv_status_t st = matrix_features_rows(A, stream, row_base, rows_full, f);
     if (st != SPMV_SUCCESS) return st;
     std::lock_guard<std::mutex> lk(g_mu);
 #if MUXI_BUILTIN_SELECTOR
@@ -195,4 +195,10 @@ extern "C" spmv_status_t muxi_selector_pick(const muxi_csr_t* A, spmv_type_t xty
     *kernel = s.kernel[best];
     *param0 = s.param[best];
     return SPMV_SUCCESS;
+}
+
+extern "C" spmv_status_t muxi_selector_pick(const muxi_csr_t* A, spmv_type_t xtype, void* stream, int* kernel,
+                                            int* param0) {
+    if (!A) return SPMV_ERROR_INVALID_MATRIX;
+    return selector_pick_rows(A, xtype, stream, 0, A->numRows, kernel, param0);
 }

@@ -110,3 +110,27 @@ def test_full_size_powerlaw_end_to_end():
     y64 = api.oracle_f64(A, torch.from_numpy(np.array(x)).cuda()).cpu().numpy()
     rl2 = np.linalg.norm(y.astype(np.float64) - y64) / np.linalg.norm(y64)
     assert rl2 <= 1e-6, rl2
+
+
+@pytest.mark.parametrize("kernel", ["ell", "auto"])
+@pytest.mark.parametrize("chunks", [1, 3, 4])
+def test_row_blocks_of_a_banded_matrix(kernel, chunks):
+    """Chunk plans are row blocks of the whole matrix: the ELL plan takes its 16-bit column offsets from the
+    whole matrix's row (row + first row of the chunk), so later chunks of a banded matrix, whose columns lie
+    near their rows in the whole matrix but far from their 0-based chunk rows, decode the right columns."""
+    rng = np.random.default_rng(3)
+    R, half = 90_000, 300
+    rows = [np.unique(np.clip(r + rng.integers(-half, half + 1, size=5), 0, R - 1)) for r in range(R)]
+    rp = np.zeros(R + 1, np.int32)
+    rp[1:] = np.cumsum([len(c) for c in rows])
+    ci = np.concatenate(rows).astype(np.int32)
+    h = G.HostCSR(rp, ci, rng.uniform(-1, 1, ci.size).astype(np.float32), R, R)
+    A = dev_csr(h)
+    hp = host.HostPlan(A, chunks, kernel)
+    xs = G.uniform_vector(R, 12)
+    y = np.full(R, np.nan, np.float32)
+    hp.exec(xs, y)
+    assert_close_fp32(y, h, xs)
+    if kernel == "ell":
+        assert all(hp.chunk(k)[2] == "ell" for k in range(chunks))
+    hp.destroy()
