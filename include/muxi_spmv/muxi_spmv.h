@@ -247,6 +247,18 @@ spmv_status_t muxi_plan_create_spmm(const muxi_csr_t* A, spmv_type_t vtype, spmv
 spmv_status_t muxi_plan_create(const muxi_csr_t* A, spmv_type_t vtype, spmv_type_t xtype,
                                muxi_kernel_t kernel, int param0, int compensated,
                                void* stream, muxi_plan** plan);
+/* muxi_plan_create for a row block: A holds rows [row_base, row_base +
+ * A->numRows) of a matrix of rows_full rows, numbered from 0, with that
+ * matrix's columns (one GPU's row shard of a multi-GPU split, or a chunk of
+ * the host path). AUTO then measures the block's band against the whole
+ * matrix's diagonal, and an ELL plan takes its 16-bit column offsets from the
+ * whole-matrix row, as they would for the whole matrix. row_base = 0 and
+ * rows_full = A->numRows is muxi_plan_create. No counterpart in the
+ * reference, which has no row shards; for callers that split a matrix by
+ * rows. */
+spmv_status_t muxi_plan_create_rows(const muxi_csr_t* A, spmv_type_t vtype, spmv_type_t xtype,
+                                    muxi_kernel_t kernel, int param0, int compensated, void* stream,
+                                    long long row_base, long long rows_full, muxi_plan** plan);
 spmv_status_t muxi_plan_destroy(muxi_plan* plan);
 /* y = alpha*A*x + beta*y with nrhs right-hand sides: for nrhs > 1, X is
  * numCols x nrhs and Y numRows x nrhs, both row-major. */

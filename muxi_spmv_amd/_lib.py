@@ -123,6 +123,7 @@ SIGNATURES: dict[str, tuple] = {
     "muxi_spmv_ellpack_f64": (_I, [_P_ELL, _VP, _VP, _VP]),
     "muxi_ellpack_destroy": (_I, [_P_ELL]),
     "muxi_plan_create": (_I, [_P_CSR, _I, _I, _I, _I, _I, _VP, C.POINTER(_VP)]),
+    "muxi_plan_create_rows": (_I, [_P_CSR, _I, _I, _I, _I, _I, _VP, C.c_longlong, C.c_longlong, C.POINTER(_VP)]),
     "muxi_plan_destroy": (_I, [_VP]),
     "muxi_plan_create_spmm": (_I, [_P_CSR, _I, _I, _I, _I, _VP, C.POINTER(_VP)]),
     "muxi_plan_exec": (_I, [_VP, _VP, _VP, _D, _D, _I, _VP]),

@@ -295,13 +295,21 @@ class Plan:
     """A matrix bound to one kernel and its scratch; exec allocates nothing."""
 
     def __init__(self, A: CSRMatrix, kernel: str = "auto", param0: int = 0, xtype=torch.float32,
-                 compensated: bool = False, stream=None):
+                 compensated: bool = False, stream=None, row_base: int = 0, rows_full: int | None = None):
+        """With row_base / rows_full, A is rows [row_base, row_base + A.numRows) of a matrix of rows_full rows
+        (a row shard): muxi_plan_create_rows."""
         self.A = A
         self.xtype = xtype
         self.nrhs = 1
         self._p = C.c_void_p()
-        check(load().muxi_plan_create(C.byref(A.c), A.vtype, _TORCH2SPMV[xtype], _lib.KERNELS[kernel], param0,
-                                      int(compensated), _stream(stream), C.byref(self._p)), "plan_create")
+        if row_base == 0 and rows_full is None:
+            check(load().muxi_plan_create(C.byref(A.c), A.vtype, _TORCH2SPMV[xtype], _lib.KERNELS[kernel], param0,
+                                          int(compensated), _stream(stream), C.byref(self._p)), "plan_create")
+        else:
+            full = A.numRows + row_base if rows_full is None else rows_full
+            check(load().muxi_plan_create_rows(C.byref(A.c), A.vtype, _TORCH2SPMV[xtype], _lib.KERNELS[kernel],
+                                               param0, int(compensated), _stream(stream), int(row_base), int(full),
+                                               C.byref(self._p)), "plan_create_rows")
 
     @classmethod
     def spmm(cls, A: CSRMatrix, nrhs: int, ipt: int = 8, xtype=torch.float32, stream=None) -> "Plan":

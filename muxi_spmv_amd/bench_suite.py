@@ -227,13 +227,14 @@ def shard(args) -> list[dict]:
         n0, n1 = int(h.rowPtr[r0]), int(h.rowPtr[r1])
         rp = (h.rowPtr[r0:r1 + 1] - n0).astype(np.int32)
         x = torch.from_numpy(G.uniform_vector(h.numCols, 7, vdt)).cuda()
-        res = {"workload": name, "gpus": G_, "shard": g, "shard_rows": r1 - r0, "shard_nnz": n1 - n0,
+        res = {"workload": name, "gpus": G_, "shard_index": g, "shard_rows": r1 - r0, "shard_nnz": n1 - n0,
                "items_max_over_mean": round(float(items.max() / items.mean()), 4)}
-        for part, (A, R) in (("shard", (api.CSRMatrix(rp, h.colIdx[n0:n1], torch.from_numpy(h.values[n0:n1]).to(dt),
-                                                     h.numCols), r1 - r0)),
-                             ("whole", (api.CSRMatrix(h.rowPtr, h.colIdx, torch.from_numpy(h.values).to(dt),
-                                                      h.numCols), h.numRows))):
-            p = api.Plan(A, "auto", 0, xtype=dt)
+        for part, (A, R, rb) in (("shard", (api.CSRMatrix(rp, h.colIdx[n0:n1],
+                                                          torch.from_numpy(h.values[n0:n1]).to(dt), h.numCols),
+                                            r1 - r0, r0)),
+                                 ("whole", (api.CSRMatrix(h.rowPtr, h.colIdx, torch.from_numpy(h.values).to(dt),
+                                                          h.numCols), h.numRows, 0))):
+            p = api.Plan(A, "auto", 0, xtype=dt, row_base=rb, rows_full=h.numRows)  # the shard is a row block
             y = torch.empty(R, dtype=dt, device="cuda")
             ms = tm.time(lambda: p.exec(x, y))
             nb = algorithmic_bytes(R, h.numCols, A.nnz, esz, esz)

@@ -433,6 +433,15 @@ spmv_status_t muxi::matrix_features_rows(const muxi_csr_t* A, void* stream, long
     out[3] = lq / R - mean * mean;
     out[4] = mx;
     out[5] = A->nnz > 0 ? bs / A->nnz / (A->numCols > 0 ? A->numCols : 1) : 0.0;
+    if (row_base != 0 || R != rows_full) {
+        // A row block: scale the band by (1/3) / E, E = the band uniformly random columns would give for rows
+        // at t in [t0, t1) of the full diagonal (mean over t of (t^2 + (1 - t)^2) / 2), so that a random block
+        // reads 1/3 like a random square matrix wherever it sits, and a banded block stays near 0.
+        const double t0 = (double)row_base / rows_full, t1 = (double)(row_base + R) / rows_full;
+        const double u0 = 1.0 - t0, u1 = 1.0 - t1;
+        const double E = ((t1 * t1 * t1 - t0 * t0 * t0) - (u1 * u1 * u1 - u0 * u0 * u0)) / (6.0 * (t1 - t0));
+        if (E > 0) out[5] *= (1.0 / 3.0) / E;
+    }
     out[6] = A->nnz > 0 ? dc / A->nnz : 0.0;
     out[7] = ec / R;
     return SPMV_SUCCESS;

@@ -472,6 +472,12 @@ spmv_status_t muxi::plan_create_rows(const muxi_csr_t* A, spmv_type_t vt, spmv_t
     });
 }
 
+extern "C" spmv_status_t muxi_plan_create_rows(const muxi_csr_t* A, spmv_type_t vt, spmv_type_t xt,
+                                               muxi_kernel_t kernel, int param0, int compensated, void* stream,
+                                               long long row_base, long long rows_full, muxi_plan** out) {
+    return plan_create_rows(A, vt, xt, kernel, param0, compensated, stream, (long)row_base, (long)rows_full, out);
+}
+
 extern "C" spmv_status_t muxi_plan_create_spmm(const muxi_csr_t* Ain, spmv_type_t vt, spmv_type_t xt, int nrhs, int ipt,
                                                void* stream, muxi_plan** out) {
     return no_throw("muxi_plan_create_spmm", [&] { return plan_create_spmm_impl(Ain, vt, xt, nrhs, ipt, stream, out); });

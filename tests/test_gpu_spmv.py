@@ -388,3 +388,29 @@ def test_ell_column_offsets(far, off16, vdt):
         - 0.5 * y0.cpu().numpy().astype(np.float64)
     tol = 1e-5 if vdt == torch.float32 else 1e-12
     assert np.max(np.abs(y2.cpu().numpy() - ref)) <= tol * (1 + np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("kernel,param", [("ell", 0), ("auto", 0), ("merge", 4), ("xslab", 1028)])
+def test_row_block_plan(kernel, param):
+    """muxi_plan_create_rows: a plan for rows [r0, r1) of a larger matrix (0-based rows, the full columns)
+    gives that block of y; the ELL plan's 16-bit offsets, taken from the full-matrix rows, still decode to the
+    right columns when r0 is far beyond 32767."""
+    h = _offset_matrix(70_000, 30_000)
+    r0, r1 = 45_000, 70_000
+    n0, n1 = int(h.rowPtr[r0]), int(h.rowPtr[r1])
+    blk = G.HostCSR((h.rowPtr[r0:r1 + 1] - n0).astype(np.int32), h.colIdx[n0:n1], h.values[n0:n1], r1 - r0,
+                    h.numCols)
+    A = dev_csr(blk)
+    x = xvec(h.numCols)
+    xd = torch.from_numpy(x).cuda()
+    y = torch.full((r1 - r0,), float("nan"), device="cuda")
+    p = api.Plan(A, kernel, param, row_base=r0, rows_full=h.numRows)
+    p.exec(xd, y)
+    torch.cuda.synchronize()
+    assert_close_fp32(y, blk, x)
+    if kernel == "ell":  # every column is within 30,000 of its full-matrix row: 16-bit offsets
+        width = int(blk.row_lengths().max())
+        assert p.info().scratch_bytes == (2 + 4) * blk.numRows * width
+    p.destroy()
+    with pytest.raises(Exception):
+        api.Plan(A, kernel, param, row_base=r0 + 1, rows_full=h.numRows)  # the block would pass the last row
