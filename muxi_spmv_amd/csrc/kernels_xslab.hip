@@ -601,8 +601,41 @@ static XslabFn<VT, XT, AT> xslab_fn(int epl) {
     return epl == 16 ? xslab_kernel<VT, XT, AT, 16> : epl == 8 ? xslab_kernel<VT, XT, AT, 8> : xslab_kernel<VT, XT, AT, 4>;
 }
 
+// Plan-build temporaries on one stream: freed (stream-ordered) when the build returns, on every path, or earlier
+// by release().
+struct StreamTemps {
+    hipStream_t s;
+    std::vector<void*> held;
+    explicit StreamTemps(hipStream_t st) : s(st) {}
+    StreamTemps(const StreamTemps&) = delete;
+    StreamTemps& operator=(const StreamTemps&) = delete;
+    ~StreamTemps() {
+        for (void* q : held) (void)hipFreeAsync(q, s);
+    }
+    template <typename T>
+    hipError_t alloc(T** out, size_t bytes) {
+        *out = nullptr;
+        void* q = nullptr;
+        const hipError_t e = hipMallocAsync(&q, bytes, s);
+        if (e == hipSuccess) {
+            held.push_back(q);
+            *out = (T*)q;
+        }
+        return e;
+    }
+    void release(void* q) {
+        for (size_t i = 0; i < held.size(); ++i)
+            if (held[i] == q) {
+                (void)hipFreeAsync(q, s);
+                held.erase(held.begin() + (long)i);
+                return;
+            }
+    }
+};
+
 spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStream_t s) {
     const int R = p->A.R, C = p->A.C;
+    StreamTemps tmp(s);
     if (S < 1 || S > kMaxSlabs) {
         set_error("xslab: slabs must be in [1, 16]");
         return SPMV_ERROR_INTERNAL;
@@ -616,9 +649,9 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
     SlabCuts cut{};
     {
         int *hist, *pre, *d_cut;
-        MUXI_HIP(hipMallocAsync(&hist, sizeof(int) * ((size_t)C + 1), s));
-        MUXI_HIP(hipMallocAsync(&pre, sizeof(int) * ((size_t)C + 1), s));
-        MUXI_HIP(hipMallocAsync(&d_cut, sizeof(int) * (kMaxSlabs + 1), s));
+        MUXI_HIP(tmp.alloc(&hist, sizeof(int) * ((size_t)C + 1)));
+        MUXI_HIP(tmp.alloc(&pre, sizeof(int) * ((size_t)C + 1)));
+        MUXI_HIP(tmp.alloc(&d_cut, sizeof(int) * (kMaxSlabs + 1)));
         MUXI_HIP(hipMemsetAsync(hist, 0, sizeof(int) * ((size_t)C + 1), s));
         if (p->A.N > 0) {
             unsigned g = grid_for(p->A.N, 256);
@@ -632,14 +665,14 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
         MUXI_LAUNCH_CHECK();
         MUXI_HIP(hipMemcpyAsync(cut.c, d_cut, sizeof(int) * (S + 1), hipMemcpyDeviceToHost, s));
         MUXI_HIP(hipStreamSynchronize(s));
-        for (int* t : {hist, pre, d_cut}) (void)hipFreeAsync(t, s);
+        for (int* t : {hist, pre, d_cut}) tmp.release(t);
     }
     int *cnt, *flag, *pos, *lrow, *npair;
-    MUXI_HIP(hipMallocAsync(&cnt, sizeof(int) * (SR + 1), s));
-    MUXI_HIP(hipMallocAsync(&flag, sizeof(int) * (SR + 1), s));
-    MUXI_HIP(hipMallocAsync(&pos, sizeof(int) * (SR + 1), s));
-    MUXI_HIP(hipMallocAsync(&lrow, sizeof(int) * (SR + 1), s));
-    MUXI_HIP(hipMallocAsync(&npair, sizeof(int) * ((size_t)R + 1), s));
+    MUXI_HIP(tmp.alloc(&cnt, sizeof(int) * (SR + 1)));
+    MUXI_HIP(tmp.alloc(&flag, sizeof(int) * (SR + 1)));
+    MUXI_HIP(tmp.alloc(&pos, sizeof(int) * (SR + 1)));
+    MUXI_HIP(tmp.alloc(&lrow, sizeof(int) * (SR + 1)));
+    MUXI_HIP(tmp.alloc(&npair, sizeof(int) * ((size_t)R + 1)));
     MUXI_HIP(hipMemsetAsync(cnt + SR, 0, sizeof(int), s));
     MUXI_HIP(hipMemsetAsync(flag + SR, 0, sizeof(int), s));
     MUXI_HIP(hipMemsetAsync(npair + R, 0, sizeof(int), s));
@@ -680,7 +713,6 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
     }
     hpad[S] = (int)tp;
     if (tp >= INT_MAX) {
-        for (int* t : {cnt, flag, pos, lrow, npair}) (void)hipFreeAsync(t, s);
         set_error("xslab: padded nonzeros exceed INT32_MAX");
         return SPMV_ERROR_INVALID_MATRIX;
     }
@@ -688,7 +720,7 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
     p->P = maxt;
     const long ncarry = ntiles;  // one carry per tile
     int* d_nb = nullptr;
-    MUXI_HIP(hipMallocAsync(&d_nb, sizeof(int) * 2 * S, s));
+    MUXI_HIP(tmp.alloc(&d_nb, sizeof(int) * 2 * S));
     MUXI_HIP(hipMemcpyAsync(d_nb, hpos.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
     MUXI_HIP(hipMemcpyAsync(d_nb + S, hpad.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
     const size_t vs = vsize(vt);
@@ -718,7 +750,7 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
     p->slab_k = S <= 4 ? 4 : (S <= 8 ? 8 : 0);
     int* ploc4 = nullptr;
     if (p->slab_k) {
-        MUXI_HIP(hipMallocAsync(&ploc4, sizeof(int) * p->slab_k * (size_t)R, s));
+        MUXI_HIP(tmp.alloc(&ploc4, sizeof(int) * p->slab_k * (size_t)R));
         MUXI_HIP(hipMemsetAsync(ploc4, 0xff, sizeof(int) * p->slab_k * (size_t)R, s));
         MUXI_HIP(hipMalloc(&p->d_slab_rmask, sizeof(unsigned) * (size_t)R));
         const long nwb = ((long)R + 63) / 64 * S;
@@ -769,7 +801,7 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
         MUXI_HIP(hipMalloc(&p->d_slab_pc, acc * ((size_t)total_rows + 1)));
         MUXI_HIP(hipMemsetAsync(p->d_slab_pc, 0, acc * ((size_t)total_rows + 1), s));
         int* keys = nullptr;
-        MUXI_HIP(hipMallocAsync(&keys, sizeof(int) * (size_t)(ncarry + 1), s));
+        MUXI_HIP(tmp.alloc(&keys, sizeof(int) * (size_t)(ncarry + 1)));
         for (int q = 0; q < S; ++q) {
             if (sa.tiles[q] == 0) continue;
             hipLaunchKernelGGL(xslab_carry_keys_kernel, dim3(grid_for(sa.tiles[q], 256)), dim3(256), 0, s,
@@ -780,7 +812,7 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
         std::vector<int> hk((size_t)ncarry);
         MUXI_HIP(hipMemcpyAsync(hk.data(), keys, sizeof(int) * ncarry, hipMemcpyDeviceToHost, s));
         MUXI_HIP(hipStreamSynchronize(s));
-        (void)hipFreeAsync(keys, s);
+        tmp.release(keys);
         std::vector<int> single, ct;
         for (long t = 0; t < ncarry;) {
             if (hk[t] < 0) {
@@ -800,13 +832,13 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
         }
         if (!single.empty()) {
             int* d_single = nullptr;
-            MUXI_HIP(hipMallocAsync(&d_single, sizeof(int) * single.size(), s));
+            MUXI_HIP(tmp.alloc(&d_single, sizeof(int) * single.size()));
             MUXI_HIP(hipMemcpyAsync(d_single, single.data(), sizeof(int) * single.size(), hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(xslab_mark_single_kernel, dim3(grid_for((long)single.size(), 256)), dim3(256), 0, s,
                                d_single, (int)single.size(), p->d_tile_row);
             MUXI_LAUNCH_CHECK();
             MUXI_HIP(hipStreamSynchronize(s));
-            (void)hipFree(d_single);
+            tmp.release(d_single);
         }
         const long nct = (long)ct.size() / 3;
         p->nheavy = 0;
@@ -814,9 +846,9 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
             MUXI_HIP(hipMalloc(&p->d_slab_ct, sizeof(int) * ct.size()));
             MUXI_HIP(hipMemcpyAsync(p->d_slab_ct, ct.data(), sizeof(int) * ct.size(), hipMemcpyHostToDevice, s));
             int *cross_id = nullptr, *hflag = nullptr, *hrank = nullptr;
-            MUXI_HIP(hipMallocAsync(&cross_id, sizeof(int) * ((size_t)total_rows + 1), s));
-            MUXI_HIP(hipMallocAsync(&hflag, sizeof(int) * ((size_t)R + 1), s));
-            MUXI_HIP(hipMallocAsync(&hrank, sizeof(int) * ((size_t)R + 1), s));
+            MUXI_HIP(tmp.alloc(&cross_id, sizeof(int) * ((size_t)total_rows + 1)));
+            MUXI_HIP(tmp.alloc(&hflag, sizeof(int) * ((size_t)R + 1)));
+            MUXI_HIP(tmp.alloc(&hrank, sizeof(int) * ((size_t)R + 1)));
             MUXI_HIP(hipMemsetAsync(cross_id, 0xff, sizeof(int) * ((size_t)total_rows + 1), s));
             MUXI_HIP(hipMemsetAsync(hflag + R, 0, sizeof(int), s));
             hipLaunchKernelGGL(xslab_cross_id_kernel, dim3(grid_for(nct, 256)), dim3(256), 0, s, p->d_slab_ct, nct,
@@ -837,14 +869,14 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
                 MUXI_LAUNCH_CHECK();
             }
             p->nheavy = H;
-            for (int* t : {cross_id, hflag, hrank}) (void)hipFreeAsync(t, s);
+            for (int* t : {cross_id, hflag, hrank}) tmp.release(t);
         }
         if (!single.empty()) {  // the masks say which pairs have a carry slot (the others skip PC)
             std::vector<int> skey(single.size());
             for (size_t i = 0; i < single.size(); ++i) skey[i] = hk[(size_t)single[i]];
             int *d_sk = nullptr, *has_pc = nullptr;
-            MUXI_HIP(hipMallocAsync(&d_sk, sizeof(int) * skey.size(), s));
-            MUXI_HIP(hipMallocAsync(&has_pc, sizeof(int) * ((size_t)total_rows + 1), s));
+            MUXI_HIP(tmp.alloc(&d_sk, sizeof(int) * skey.size()));
+            MUXI_HIP(tmp.alloc(&has_pc, sizeof(int) * ((size_t)total_rows + 1)));
             MUXI_HIP(hipMemsetAsync(has_pc, 0, sizeof(int) * ((size_t)total_rows + 1), s));
             MUXI_HIP(hipMemcpyAsync(d_sk, skey.data(), sizeof(int) * skey.size(), hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(xslab_set_kernel, dim3(grid_for((long)skey.size(), 256)), dim3(256), 0, s, d_sk,
@@ -854,11 +886,9 @@ spmv_status_t xslab_build(muxi_plan* p, int S, int epl, spmv_type_t vt, hipStrea
                                has_pc, p->d_slab_rmask);
             MUXI_LAUNCH_CHECK();
             MUXI_HIP(hipStreamSynchronize(s));  // skey is host memory the copy reads
-            for (int* t : {d_sk, has_pc}) (void)hipFreeAsync(t, s);
+            for (int* t : {d_sk, has_pc}) tmp.release(t);
         }
     }
-    for (int* t : {cnt, flag, pos, lrow, npair, d_nb, ploc4})
-        if (t) (void)hipFreeAsync(t, s);
     MUXI_HIP(hipStreamSynchronize(s));
     p->scratch_bytes = sizeof(int) * ints + vs * (size_t)tp + 4 * nflag + 2 * sizeof(int) * (size_t)(ntiles + 1) +
                        acc * (size_t)(ncarry + total_rows) + sizeof(int) * ((size_t)ncarry + R + 1) +
