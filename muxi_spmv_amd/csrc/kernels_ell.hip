@@ -339,16 +339,100 @@ __global__ __launch_bounds__(256) void ellg_spmv_kernel(int R, int C, int rb, El
     store_y(y, row, a.value(), alpha, beta);
 }
 
+// Rows of 3 or 4 groups (width 12 to 19): every column and value load of the
+// row is issued at once, then every gather, then the products in slot order
+// (the same order as the round-based kernel, so the sums are the same): one
+// dependent load -> gather step instead of one per round, which is what a
+// small, latency-bound matrix waits on.
+template <typename VT, typename XT, typename AT, bool COMP, typename CT>
+__global__ __launch_bounds__(256) void ellg_spmv_short_kernel(int R, int C, int rb, EllGroups eg,
+                                                              const CT* __restrict__ ecol,
+                                                              const VT* __restrict__ eval, const XT* __restrict__ x,
+                                                              XT* __restrict__ y, AT alpha, AT beta) {
+    const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= R) return;
+    Acc<AT, COMP> a;
+    a.zero();
+    if (C > 0) {
+        Vec<int, 4> c[4];
+        Vec<VT, 4> v[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            if (g < eg.n4) {
+                const Vec<CT, 4> w = ldvec<CT, 4>(ecol + 4L * eg.R * g + 4 * row);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) c[g].v[j] = ell_col(w.v[j], row + rb);
+                v[g] = ldvec<VT, 4>(eval + 4L * eg.R * g + 4 * row);
+            }
+        }
+        Vec<int, 2> c2{{-1, -1}};
+        Vec<VT, 2> v2{};
+        int c1 = -1;
+        VT v1{};
+        if (eg.two) {
+            const Vec<CT, 2> w = ldvec<CT, 2>(ecol + eg.base2 + 2 * row);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) c2.v[j] = ell_col(w.v[j], row + rb);
+            v2 = ldvec<VT, 2>(eval + eg.base2 + 2 * row);
+        }
+        if (eg.one) {
+            c1 = ell_col(ntl(ecol + eg.base1 + row), row + rb);
+            v1 = ntl(eval + eg.base1 + row);
+        }
+        XT xs[4][4], x2[2] = {}, x1 = XT(0);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            if (g < eg.n4) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) xs[g][j] = x[c[g].v[j] > 0 ? c[g].v[j] : 0];
+            }
+        }
+        if (eg.two) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) x2[j] = x[c2.v[j] > 0 ? c2.v[j] : 0];
+        }
+        if (eg.one) x1 = x[c1 > 0 ? c1 : 0];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            if (g < eg.n4) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (c[g].v[j] >= 0) a.fma(v[g].v[j], xs[g][j]);
+            }
+        }
+        if (eg.two) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                if (c2.v[j] >= 0) a.fma(v2.v[j], x2[j]);
+        }
+        if (eg.one && c1 >= 0) a.fma(v1, x1);
+    }
+    store_y(y, row, a.value(), alpha, beta);
+}
+
 template <typename VT, typename XT, typename AT, bool COMP>
 spmv_status_t launch_ellg(int R, int C, int width, bool off16, int rb, const void* d_col, const VT* d_val,
                           const XT* x, XT* y, AT alpha, AT beta, hipStream_t s) {
     if (R <= 0) return SPMV_SUCCESS;
-    if (off16)
-        hipLaunchKernelGGL((ellg_spmv_kernel<VT, XT, AT, COMP, short>), dim3(grid_for(R, 256)), dim3(256), 0, s, R, C,
-                           rb, ell_groups(R, width), (const short*)d_col, d_val, x, y, alpha, beta);
-    else
-        hipLaunchKernelGGL((ellg_spmv_kernel<VT, XT, AT, COMP, int>), dim3(grid_for(R, 256)), dim3(256), 0, s, R, C,
-                           rb, ell_groups(R, width), (const int*)d_col, d_val, x, y, alpha, beta);
+    const EllGroups eg = ell_groups(R, width);
+    // every load of a row at once where the round-based kernel needs two rounds (3 or 4 groups); with fewer
+    // groups it has one round and fewer registers (the 5-point stencil in FP64 runs 9 % slower in this one)
+    const bool shortrow = eg.n4 == 3 || eg.n4 == 4;
+    if (off16) {
+        if (shortrow)
+            hipLaunchKernelGGL((ellg_spmv_short_kernel<VT, XT, AT, COMP, short>), dim3(grid_for(R, 256)), dim3(256), 0,
+                               s, R, C, rb, eg, (const short*)d_col, d_val, x, y, alpha, beta);
+        else
+            hipLaunchKernelGGL((ellg_spmv_kernel<VT, XT, AT, COMP, short>), dim3(grid_for(R, 256)), dim3(256), 0, s, R,
+                               C, rb, eg, (const short*)d_col, d_val, x, y, alpha, beta);
+    } else {
+        if (shortrow)
+            hipLaunchKernelGGL((ellg_spmv_short_kernel<VT, XT, AT, COMP, int>), dim3(grid_for(R, 256)), dim3(256), 0,
+                               s, R, C, rb, eg, (const int*)d_col, d_val, x, y, alpha, beta);
+        else
+            hipLaunchKernelGGL((ellg_spmv_kernel<VT, XT, AT, COMP, int>), dim3(grid_for(R, 256)), dim3(256), 0, s, R,
+                               C, rb, eg, (const int*)d_col, d_val, x, y, alpha, beta);
+    }
     MUXI_LAUNCH_CHECK();
     return SPMV_SUCCESS;
 }
