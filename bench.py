@@ -40,12 +40,14 @@ CANDIDATES = [("merge", 4), ("merge", 8), ("merge", 16), ("csr5", 256), ("csr5",
 
 # HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
 # `bench.py --kernel K:P` (KiB per dispatch, FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors a 1 GiB copy
-# kernel reads); the runs and commands are in profiles/xslab_spmv_powerlaw_2m_fp32.md and
-# profiles/ell_spmv_laplacian_4096_fp64.md. Configurations not measured report null.
+# kernel reads); the runs and commands are in profiles/xslab_spmv_powerlaw_2m_fp32.md,
+# profiles/ell_spmv_laplacian_4096_fp64.md and profiles/ell_spmv_uniform_100k_fp32.md. Configurations not
+# measured report null.
 MEASURED_TRAFFIC = {
     ("powerlaw_2m_fp32", "xslab", 1028): int((66336.6 * 2.0 + 15477.4 * 1.0) * 1024),
     ("powerlaw_2m_fp32", "xslab", 1032): int((68168.4 * 2.0 + 15791.2 * 1.0) * 1024),
     ("laplacian_4096_fp64", "ell", 5): int((483539.5 * 2.0 + 131072.0 * 1.0) * 1024),
+    ("uniform_100k_fp32", "ell", 15): int((7443.4 * 2.0 + 390.6 * 1.0) * 1024),
 }
 
 
@@ -170,12 +172,15 @@ def main(argv=None) -> int:
         plans = [api.Plan(m, "auto", 0, xtype=tdt) for m in mats]
         info = plans[0].info()
         pick = f"{info.kernel}:{info.param0}"
-        if pick not in sweep:  # time the pick the way the sweep timed its candidates
+        key = pick
+        if info.kernel == "ell" and info.param0 == maxlen and "ell:0" in sweep:
+            key = "ell:0"  # the same plan: an ELL width of 0 means the longest row
+        if key not in sweep:  # time the pick the way the sweep timed its candidates
             m = measure_plans(plans, xs, ys, a.sweep_steps, 3, record_kernel=False)
-            sweep[pick] = round(m.ms_per_step, 5)
+            sweep[key] = round(m.ms_per_step, 5)
         state = {0: "fixed rule", 1: "learned (built in)", 2: "learned (set by caller)"}[load().muxi_selector_state()]
         selector = {"state": state, "pick": pick, "sweep_best": best,
-                    "regret": round(sweep[pick] / sweep[best] - 1.0, 4)}
+                    "regret": round(sweep[key] / sweep[best] - 1.0, 4)}
         log(f"selector ({state}) picks {pick}; sweep best {best}; regret {selector['regret']:.3f}")
     else:
         if a.kernel != "sweep":
