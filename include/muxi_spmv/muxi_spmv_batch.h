@@ -23,7 +23,7 @@ typedef struct muxi_batch_plan muxi_batch_plan;
 
 /* mats[count] with device arrays; xs[count] / ys[count] device vectors of
  * numCols / numRows elements. vtype FLOAT (xtype FLOAT) or DOUBLE (xtype
- * DOUBLE). ipt: merge items per lane (4, 8 or 16; <= 0 picks 8). */
+ * DOUBLE). ipt: merge items per lane (4, 8 or 16; <= 0 picks 16). */
 spmv_status_t muxi_batch_plan_create(const muxi_csr_t* mats, int count, spmv_type_t vtype, spmv_type_t xtype,
                                      const void* const* xs, void* const* ys, int ipt, void* stream,
                                      muxi_batch_plan** plan);

@@ -13,7 +13,7 @@ from ._lib import check, load
 class BatchPlan:
     """Binds matrices (objects with .c and .dtype) and their x / y tensors."""
 
-    def __init__(self, mats, xs, ys, ipt: int = 8, stream=None):
+    def __init__(self, mats, xs, ys, ipt: int = 16, stream=None):
         n = len(mats)
         self.mats, self.xs, self.ys = mats, xs, ys  # keep alive
         arr = (_lib.muxi_csr_t * n)(*[m.c for m in mats])

@@ -136,10 +136,19 @@ def batched(args) -> list[dict]:
         tot_b += algorithmic_bytes(R, R, A.nnz, 4, 4)
     nnz = sum(m.nnz for m in mats)
     log(f"256 matrices, {sum(m.numRows for m in mats)} rows, {nnz} nnz, {tot_b / 1e6:.0f} MB")
-    bp = BatchPlan(mats, xs, ys, 8)
     tm = Timer()
     s = torch.cuda.Stream()
     out = []
+    ipt_ms = {}
+    for ipt in (4, 8, 16):  # items per lane of the merge tiles
+        bp = BatchPlan(mats, xs, ys, ipt)
+        with torch.cuda.stream(s):
+            ipt_ms[ipt] = tm.time(lambda: bp.exec())
+        torch.cuda.synchronize()
+        log(f"batched persistent, {ipt} items per lane: {ipt_ms[ipt]:.4f} ms")
+        bp.destroy()
+    best_ipt = min(ipt_ms, key=ipt_ms.get)
+    bp = BatchPlan(mats, xs, ys, best_ipt)
     with torch.cuda.stream(s):
         g_batch = bp.graph(stream=s)
         g_vec = bp.graph("vector", 0, stream=s)
@@ -159,7 +168,8 @@ def batched(args) -> list[dict]:
         out.append({"workload": "batched_256", "path": name, "ms": round(ms, 4), "GBps": round(tot_b / ms / 1e6, 1),
                     "nnz": nnz})
     out.append({"workload": "batched_256", "speedup_graph_vs_per_matrix_graph": round(t_vec_g / t_batch_g, 2),
-                "speedup_eager": round(t_vec / t_batch, 2), "worst_rel_l2": worst, "info": bp.info()})
+                "speedup_eager": round(t_vec / t_batch, 2), "worst_rel_l2": worst, "info": bp.info(),
+                "items_per_lane": best_ipt, "ms_by_items_per_lane": {str(k): round(v, 4) for k, v in ipt_ms.items()}})
     for r in out:
         log(json.dumps(r))
     return out
