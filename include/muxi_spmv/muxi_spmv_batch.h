@@ -5,7 +5,8 @@
  * src/spmv/csr/spmv_csr.cuh:545-869, batch rows of one matrix). A batch plan
  * binds `count` CSR matrices and their x / y vectors; every matrix is cut
  * into merge-path tiles of the same path length and one persistent kernel
- * (grid sized to the device) works through all tiles of all matrices, then
+ * (grid sized to the device, each XCD on its own contiguous eighth of the
+ * tiles) works through all tiles of all matrices, then
  * one fix-up kernel adds the partial sums of rows that span tiles. Exec takes
  * no pointers, allocates nothing, and can be captured in a hipGraph
  * (muxi_batch_graph_*).

@@ -3,7 +3,8 @@
 // Every matrix is cut into merge-path tiles of at most 256*IPT path items
 // (the same coordinates as the single-matrix merge kernel). A tile table
 // maps each global tile to its matrix; a persistent grid sized to the device
-// strides through the table, running the shared tile body. Carries are
+// (a multiple of 8) strides through it, the workgroups of each XCD through
+// their own contiguous eighth, running the shared tile body. Carries are
 // keyed by a virtual row (each matrix's rows plus one dummy row, laid end to
 // end) so that runs never cross matrices, and one fix-up kernel folds them.
 #include <climits>
@@ -32,7 +33,11 @@ __global__ __launch_bounds__(256) void batch_merge_kernel(const BatchDesc* __res
                                                           const int* __restrict__ gdiag, const int* __restrict__ grow,
                                                           int T, AT alpha, AT beta, Acc<AT, false>* __restrict__ carry) {
     __shared__ MergeSmem<AT, false, IPT> sm;
-    for (int t = blockIdx.x; t < T; t += gridDim.x) {
+    // Workgroup b runs on XCD b mod 8. Each XCD walks its own contiguous eighth of the tile table (tiles are in
+    // matrix order), so at any time an XCD's L2 holds the x of one or two matrices instead of a dozen.
+    const int xcd = blockIdx.x % 8, per = gridDim.x / 8;
+    const int t1 = (int)((long)T * (xcd + 1) / 8);
+    for (int t = (int)((long)T * xcd / 8) + (int)(blockIdx.x / 8); t < t1; t += per) {
         const int m = tile_mat[t];
         const BatchDesc d = desc[m];
         const int c = d.coord_base + (t - d.tile_base);
@@ -209,7 +214,8 @@ static spmv_status_t batch_plan_create_impl(const muxi_csr_t* mats, int count, s
     int dev = 0, cus = 256;
     BT_HIP(hipGetDevice(&dev));
     BT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    p->grid = (int)std::min<long>(T > 0 ? T : 1, (long)cus * 8);
+    // a multiple of 8 workgroups: an equal share for each XCD's part of the tile table
+    p->grid = (int)std::max<long>(8, std::min<long>(T, (long)cus * 8) / 8 * 8);
     const size_t acc = xt == SPMV_TYPE_DOUBLE ? 8 : 4;
     BT_HIP(hipMalloc(&p->d_desc, sizeof(BatchDesc) * count));
     BT_HIP(hipMalloc(&p->d_tile_mat, sizeof(int) * (size_t)(T > 0 ? T : 1)));
