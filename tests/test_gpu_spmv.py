@@ -347,6 +347,30 @@ def test_full_size_laplacian_exact():
     assert torch.equal(y, exp)
 
 
+def test_full_size_laplacian_columns():
+    """The same matrix with x[c] = c^2: every product and partial sum is an integer below 2^53, so y is exact
+    in FP64 and a wrong column anywhere (a mis-decoded 16-bit offset, a wrong slot) changes it. Checked against
+    the integer product on the host for ELL (16-bit offsets), merge and column slabs."""
+    n = 4096
+    h = G.cfg_laplacian_4096()
+    A = dev_csr(h, torch.float64)
+    c = np.arange(n * n, dtype=np.int64)
+    xi = c * c
+    prod = h.values.astype(np.int64) * xi[h.colIdx]
+    exp = np.add.reduceat(prod, h.rowPtr[:-1].astype(np.int64))
+    exp[np.diff(h.rowPtr) == 0] = 0
+    x = torch.from_numpy(xi.astype(np.float64)).cuda()
+    for kernel, param in [("ell", 0), ("merge", 8), ("xslab", 1028)]:
+        y = torch.full((n * n,), float("nan"), dtype=torch.float64, device="cuda")
+        p = api.Plan(A, kernel, param, xtype=torch.float64)
+        p.exec(x, y)
+        torch.cuda.synchronize()
+        assert np.array_equal(y.cpu().numpy(), exp.astype(np.float64)), kernel
+        if kernel == "ell":  # 5 slots per row, 16-bit columns
+            assert p.info().scratch_bytes == (2 + 8) * 5 * n * n
+        p.destroy()
+
+
 def _offset_matrix(R, far, dtype=np.float32, seed=5):
     """Rows r hold columns r - far, r, r + far (those inside [0, R)) plus up to two
     random columns within 100 of r, so row lengths (and the ELL padding) vary."""
