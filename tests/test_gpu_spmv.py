@@ -321,6 +321,30 @@ def test_full_size_powerlaw_properties(powerlaw_full, kernel, param):
     assert torch.all((ys.double() - (y.double() + ya.double())).abs() <= 3 * (n + 2) * 2.0 ** -24 * ab)
 
 
+@pytest.mark.parametrize("kernel,param", [("auto", 0), ("xslab", 1032), ("xslab", 1028), ("xslab", 2056),
+                                          ("xslab", (16 << 8) | 4), ("merge", 4)])
+def test_full_size_powerlaw_integers_exact(powerlaw_full, kernel, param):
+    """The power-law pattern at full size with integer values in [-3, 3] and x in [-4, 4]: every partial sum is
+    an integer below 2^24 in magnitude, so FP32 y is exact in any summation order and must equal the integer
+    product computed on the host, row for row: every slab, tile, carry, carry slot, long-row wavefront and
+    combine path of the column-slab plans is checked at the shape the benchmark runs."""
+    h, _ = powerlaw_full
+    i = np.arange(h.nnz, dtype=np.int64)
+    vi = ((i * 2654435761) >> 7) % 7 - 3
+    xi = (np.arange(h.numCols, dtype=np.int64) * 40503 >> 3) % 9 - 4
+    exp = np.add.reduceat(vi * xi[h.colIdx], h.rowPtr[:-1].astype(np.int64))
+    exp[np.diff(h.rowPtr) == 0] = 0
+    # every partial sum of a row is bounded by the row's sum of |products|: below 2^24, so exact in FP32
+    assert np.add.reduceat(np.abs(vi * xi[h.colIdx]), h.rowPtr[:-1].astype(np.int64)).max() < 2 ** 24
+    hi = G.HostCSR(h.rowPtr, h.colIdx, vi.astype(np.float32), h.numRows, h.numCols)
+    A = dev_csr(hi)
+    x = torch.from_numpy(xi.astype(np.float32)).cuda()
+    y = torch.full((h.numRows,), float("nan"), device="cuda")
+    api.Plan(A, kernel, param).exec(x, y)
+    torch.cuda.synchronize()
+    assert np.array_equal(y.cpu().numpy(), exp.astype(np.float32)), kernel
+
+
 def test_full_size_laplacian_exact():
     """4096^2 5-point Laplacian, FP64, x = 1: y is exactly 0 / 1 / 2 (interior /
     edge / corner) for CSR (merge, csr5, vwave) and ELL, at full size."""
