@@ -73,3 +73,25 @@ def test_banded_4m_precision(banded_4m_rcm, vt):
         if vt == torch.bfloat16:
             assert _rel_l2(y, y64) <= 2.0 ** -8  # the storage rounding, not the kernel
     assert errs[("merge", True)] <= errs[("merge", False)] * 1.01
+
+
+@pytest.mark.parametrize("vt", [torch.float32, torch.bfloat16])
+def test_banded_4m_integers_exact(banded_4m_rcm, vt):
+    """The RCM-ordered banded matrix at full size with integer values in [-3, 3] (exact in BF16 too) and x in
+    [-4, 4]: y equals the host's integer product row for row for ELL (16-bit offsets, four slot groups), the
+    AUTO pick and merge."""
+    A64 = banded_4m_rcm
+    rp = A64.d_rowPtr.cpu().numpy().astype(np.int64)
+    ci = A64.d_colIdx.cpu().numpy().astype(np.int64)
+    i = np.arange(ci.size, dtype=np.int64)
+    vi = ((i * 2654435761) >> 7) % 7 - 3
+    xi = (np.arange(A64.numCols, dtype=np.int64) * 40503 >> 3) % 9 - 4
+    exp = np.add.reduceat(vi * xi[ci], rp[:-1])
+    exp[np.diff(rp) == 0] = 0
+    A = A64.with_values(api.convert_values(torch.from_numpy(vi.astype(np.float64)).cuda(), vt))
+    x = torch.from_numpy(xi.astype(np.float32)).cuda()
+    for kernel in ("ell", "auto", "merge"):
+        y = torch.full((A64.numRows,), float("nan"), device="cuda")
+        api.Plan(A, kernel, 4 if kernel == "merge" else 0).exec(x, y)
+        torch.cuda.synchronize()
+        assert np.array_equal(y.cpu().numpy(), exp.astype(np.float32)), kernel
