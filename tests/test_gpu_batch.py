@@ -90,3 +90,26 @@ def test_graph_replays_match_eager():
         s.synchronize()
         for h, x, y in zip(hs, xs_h, ys):
             assert_close_fp32(y, h, x)
+
+
+def test_batch_integers_exact():
+    """Integer values in [-3, 3] and x in [-4, 4] on the mixed batch (Poisson matrices, empty rows, a long
+    row, tiny matrices): every matrix's y equals its integer product on the host, bit for bit."""
+    hs, mats, xs_h, xs, ys = make_batch(n=30, seed=8)
+    mats_i, xs_i, exps = [], [], []
+    for m, h in enumerate(hs):
+        i = np.arange(h.nnz, dtype=np.int64)
+        vi = ((i * 2654435761 + m) >> 7) % 7 - 3
+        xi = (np.arange(h.numCols, dtype=np.int64) * 40503 >> 3) % 9 - 4
+        e = np.zeros(h.numRows, np.int64)
+        if h.nnz:
+            rows = np.repeat(np.arange(h.numRows), np.diff(h.rowPtr))
+            np.add.at(e, rows, vi * xi[h.colIdx])
+        exps.append(e)
+        mats_i.append(dev_csr(G.HostCSR(h.rowPtr, h.colIdx, vi.astype(np.float32), h.numRows, h.numCols)))
+        xs_i.append(torch.from_numpy(xi.astype(np.float32)).cuda())
+    bp = batch.BatchPlan(mats_i, xs_i, ys)
+    bp.exec()
+    torch.cuda.synchronize()
+    for e, y in zip(exps, ys):
+        assert np.array_equal(y.cpu().numpy(), e.astype(np.float32))

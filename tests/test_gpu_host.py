@@ -134,3 +134,24 @@ def test_row_blocks_of_a_banded_matrix(kernel, chunks):
     if kernel == "ell":
         assert all(hp.chunk(k)[2] == "ell" for k in range(chunks))
     hp.destroy()
+
+
+@pytest.mark.parametrize("chunks", [1, 4, 8])
+def test_full_size_powerlaw_integers_exact(chunks):
+    """Host x -> host y on the 2M power-law pattern with integer values in [-3, 3] and x in [-4, 4]: every chunk
+    plan (a row block, AUTO) and the column pieces of x give y equal to the host's integer product."""
+    h = G.cfg_powerlaw_2m()
+    i = np.arange(h.nnz, dtype=np.int64)
+    vi = ((i * 2654435761) >> 7) % 7 - 3
+    xi = (np.arange(h.numCols, dtype=np.int64) * 40503 >> 3) % 9 - 4
+    exp = np.add.reduceat(vi * xi[h.colIdx], h.rowPtr[:-1].astype(np.int64))
+    exp[np.diff(h.rowPtr) == 0] = 0
+    A = dev_csr(G.HostCSR(h.rowPtr, h.colIdx, vi.astype(np.float32), h.numRows, h.numCols))
+    hp = host.HostPlan(A, chunks)
+    x = host.pinned_empty(h.numCols)
+    x[:] = xi.astype(np.float32)
+    y = host.pinned_empty(h.numRows)
+    y[:] = np.nan
+    hp.exec(x, y)
+    assert np.array_equal(np.array(y), exp.astype(np.float32))
+    hp.destroy()

@@ -106,3 +106,31 @@ def test_device_permutation_application_bit_exact():
     torch.cuda.synchronize()
     assert np.array_equal(yr.cpu().numpy(), oracle.restore_order(yp.cpu().numpy(), rm))
     assert torch.allclose(yr, y, rtol=0, atol=1e-4)
+
+
+@pytest.mark.parametrize("k,ipt", [(8, 8), (32, 16)])
+def test_spmm_full_size_integers_exact(k, ipt):
+    """SpMM on the 2M power-law pattern at full size with integer values in [-3, 3] and X in [-4, 4]: each
+    column of Y is an integer below 2^24 in magnitude at every partial sum, so Y equals the host's integer
+    product exactly, column for column."""
+    h = G.cfg_powerlaw_2m()
+    i = np.arange(h.nnz, dtype=np.int64)
+    vi = ((i * 2654435761) >> 7) % 7 - 3
+    rng = np.random.default_rng(k)
+    Xi = rng.integers(-4, 5, size=(h.numCols, k)).astype(np.int64)
+    hi = G.HostCSR(h.rowPtr, h.colIdx, vi.astype(np.float32), h.numRows, h.numCols)
+    A = dev_csr(hi)
+    X = torch.from_numpy(Xi.astype(np.float32)).cuda()
+    Y = torch.full((h.numRows, k), float("nan"), device="cuda")
+    p = api.Plan.spmm(A, k, ipt)
+    p.exec(X, Y)
+    torch.cuda.synchronize()
+    Yh = Y.cpu().numpy()
+    starts = h.rowPtr[:-1].astype(np.int64)
+    empty = np.diff(h.rowPtr) == 0
+    for c in range(k):
+        prod = vi * Xi[h.colIdx, c]
+        exp = np.add.reduceat(prod, starts)
+        exp[empty] = 0
+        assert np.add.reduceat(np.abs(prod), starts).max() < 2 ** 24
+        assert np.array_equal(Yh[:, c], exp.astype(np.float32)), c
