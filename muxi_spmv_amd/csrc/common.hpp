@@ -159,7 +159,7 @@ template <typename YT, typename AT>
 __device__ __forceinline__ void store_y(YT* y, long i, AT v, AT alpha, AT beta) {
     AT r = alpha * v;
     if (beta != AT(0)) r += beta * (AT)y[i];
-    y[i] = (YT)r;
+    __builtin_nontemporal_store((YT)r, y + i);  // y is written once per SpMV: stream it
 }
 
 }  // namespace muxi
