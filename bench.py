@@ -46,7 +46,7 @@ CANDIDATES = [("merge", 4), ("merge", 8), ("merge", 16), ("csr5", 256), ("csr5",
 MEASURED_TRAFFIC = {
     ("powerlaw_2m_fp32", "xslab", 1028): int((66336.6 * 2.0 + 15477.4 * 1.0) * 1024),
     ("powerlaw_2m_fp32", "xslab", 1032): int((68168.4 * 2.0 + 15791.2 * 1.0) * 1024),
-    ("laplacian_4096_fp64", "ell", 5): int((483539.5 * 2.0 + 131072.0 * 1.0) * 1024),
+    ("laplacian_4096_fp64", "ell", 5): int((483392.3 * 2.0 + 131072.0 * 1.0) * 1024),
     ("uniform_100k_fp32", "ell", 15): int((7443.4 * 2.0 + 390.6 * 1.0) * 1024),
 }
 
