@@ -66,13 +66,15 @@ spmv_status_t launch_ell(int R, int C, int width, const int* d_col, const VT* d_
 // the plan's layout: a row's slots grouped by four (kernels_ell.hip); with
 // off16 the columns are 16-bit offsets from the row (d_col holds shorts),
 // allowed when launch_ell_offsets_fit left *d_bad == 0
-// (offsets from row + rb: rb is a row block's first row in its full matrix, 0 for a whole matrix)
+// (offsets from row + rb: rb is a row block's first row in its full matrix, 0 for a whole matrix;
+// d_bad[0] = 1 when some offset needs more than 16 bits, d_bad[1] = the largest |offset|, capped at 32768)
 spmv_status_t launch_ell_offsets_fit(const DevCsr& A, int width, int rb, int* d_bad, hipStream_t s);
 template <typename VT>
 spmv_status_t launch_csr_to_ellg(const DevCsr& A, int width, bool off16, int rb, void* d_col, VT* d_val,
                                  hipStream_t s);
+// win >= 0 (with off16): every column lies within win of its row; the kernels stage that x window in LDS
 template <typename VT, typename XT, typename AT, bool COMP>
-spmv_status_t launch_ellg(int R, int C, int width, bool off16, int rb, const void* d_col, const VT* d_val,
+spmv_status_t launch_ellg(int R, int C, int width, bool off16, int rb, int win, const void* d_col, const VT* d_val,
                           const XT* x, XT* y, AT alpha, AT beta, hipStream_t s);
 
 // ---- SpMM over merge-path partitions (kernels_spmm.hip); carry holds P*k elements.

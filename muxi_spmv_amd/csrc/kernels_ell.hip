@@ -170,22 +170,32 @@ __device__ __forceinline__ void ell_put(short* p, int col, long row) {
 // *bad = 1 when a column among the first `width` of some row lies further
 // than 32767 from its row (the 16-bit offsets cannot hold it). Row r of a
 // row block is row r + rb of its full matrix; offsets are taken from that.
+// out[1] = the largest |column - row| (capped at 32768), which sizes the x window of the windowed kernels.
 __global__ void ell_offsets_fit_kernel(int R, int width, int rb, const int* __restrict__ rp,
                                        const int* __restrict__ ci, int* __restrict__ bad) {
     const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int b = 0;
+    int m = 0;
     if (row < R) {
         const int s0 = rp[row], n = rp[row + 1] - s0, len = n < width ? n : width;
-        for (int i = 0; i < len && !b; ++i) {
-            const long d = (long)ci[s0 + i] - (row + rb);
-            b = d < -32767 || d > 32767;
+        for (int i = 0; i < len && m <= 32767; ++i) {
+            long d = (long)ci[s0 + i] - (row + rb);
+            d = d < 0 ? -d : d;
+            m = d > m ? (d > 32768 ? 32768 : (int)d) : m;
         }
     }
-    if (__any(b) && (threadIdx.x & 63) == 0) atomicOr(bad, 1);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const int t = __shfl_xor(m, o);
+        m = t > m ? t : m;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (m > 32767) atomicOr(bad, 1);
+        atomicMax(bad + 1, m);
+    }
 }
 
 spmv_status_t launch_ell_offsets_fit(const DevCsr& A, int width, int rb, int* d_bad, hipStream_t s) {
-    MUXI_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), s));
+    MUXI_HIP(hipMemsetAsync(d_bad, 0, 2 * sizeof(int), s));
     if (A.R <= 0 || width <= 0) return SPMV_SUCCESS;
     hipLaunchKernelGGL(ell_offsets_fit_kernel, dim3(grid_for(A.R, 256)), dim3(256), 0, s, A.R, width, rb, A.rp,
                        A.ci, d_bad);
@@ -261,13 +271,28 @@ __device__ __forceinline__ Vec<T, N> ldvec(const T* p) {
 // slots (plus the pair and single in the last round), all column and value
 // loads first, then all gathers, then the products. Padding (col < 0) gathers
 // x[0] and adds nothing. CT is the stored column type (int or short offsets).
-template <typename VT, typename XT, typename AT, bool COMP, typename CT>
-__global__ __launch_bounds__(256) void ellg_spmv_kernel(int R, int C, int rb, EllGroups eg,
+template <typename VT, typename XT, typename AT, bool COMP, typename CT, bool W>
+__global__ __launch_bounds__(256) void ellg_spmv_kernel(int R, int C, int rb, int win, EllGroups eg,
                                                         const CT* __restrict__ ecol,
                                                         const VT* __restrict__ eval, const XT* __restrict__ x,
                                                         XT* __restrict__ y, AT alpha, AT beta) {
     const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    // W: the block's rows read x only within win of their whole-matrix rows; that window goes to LDS first
+    extern __shared__ unsigned char ell_lds[];
+    XT* sxw = reinterpret_cast<XT*>(ell_lds);
+    long w0 = 0;
+    if constexpr (W) {
+        const long first = (long)blockIdx.x * blockDim.x + rb;
+        w0 = first - win > 0 ? first - win : 0;
+        const long w1 = first + blockDim.x + win < C ? first + blockDim.x + win : C;
+        for (long i = w0 + threadIdx.x; i < w1; i += blockDim.x) sxw[i - w0] = x[i];
+        __syncthreads();
+    }
     if (row >= R) return;
+    auto gx = [&](int c) -> XT {
+        if constexpr (W) return sxw[c >= w0 ? c - w0 : 0];
+        else return x[c > 0 ? c : 0];
+    };
     Acc<AT, COMP> a;
     a.zero();
     if (C > 0) {
@@ -306,17 +331,17 @@ __global__ __launch_bounds__(256) void ellg_spmv_kernel(int R, int C, int rb, El
             XT xA[4] = {}, xB[4] = {}, x2[2] = {}, x1 = XT(0);
             if (hasA) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) xA[j] = x[cA.v[j] > 0 ? cA.v[j] : 0];
+                for (int j = 0; j < 4; ++j) xA[j] = gx(cA.v[j]);
             }
             if (hasB) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) xB[j] = x[cB.v[j] > 0 ? cB.v[j] : 0];
+                for (int j = 0; j < 4; ++j) xB[j] = gx(cB.v[j]);
             }
             if (has2) {
 #pragma unroll
-                for (int j = 0; j < 2; ++j) x2[j] = x[c2.v[j] > 0 ? c2.v[j] : 0];
+                for (int j = 0; j < 2; ++j) x2[j] = gx(c2.v[j]);
             }
-            if (has1) x1 = x[c1 > 0 ? c1 : 0];
+            if (has1) x1 = gx(c1);
             if (hasA) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
@@ -344,13 +369,28 @@ __global__ __launch_bounds__(256) void ellg_spmv_kernel(int R, int C, int rb, El
 // (the same order as the round-based kernel, so the sums are the same): one
 // dependent load -> gather step instead of one per round, which is what a
 // small, latency-bound matrix waits on.
-template <typename VT, typename XT, typename AT, bool COMP, typename CT>
-__global__ __launch_bounds__(256) void ellg_spmv_short_kernel(int R, int C, int rb, EllGroups eg,
+template <typename VT, typename XT, typename AT, bool COMP, typename CT, bool W>
+__global__ __launch_bounds__(256) void ellg_spmv_short_kernel(int R, int C, int rb, int win, EllGroups eg,
                                                               const CT* __restrict__ ecol,
                                                               const VT* __restrict__ eval, const XT* __restrict__ x,
                                                               XT* __restrict__ y, AT alpha, AT beta) {
     const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    // W: the block's rows read x only within win of their whole-matrix rows; that window goes to LDS first
+    extern __shared__ unsigned char ell_lds[];
+    XT* sxw = reinterpret_cast<XT*>(ell_lds);
+    long w0 = 0;
+    if constexpr (W) {
+        const long first = (long)blockIdx.x * blockDim.x + rb;
+        w0 = first - win > 0 ? first - win : 0;
+        const long w1 = first + blockDim.x + win < C ? first + blockDim.x + win : C;
+        for (long i = w0 + threadIdx.x; i < w1; i += blockDim.x) sxw[i - w0] = x[i];
+        __syncthreads();
+    }
     if (row >= R) return;
+    auto gx = [&](int c) -> XT {
+        if constexpr (W) return sxw[c >= w0 ? c - w0 : 0];
+        else return x[c > 0 ? c : 0];
+    };
     Acc<AT, COMP> a;
     a.zero();
     if (C > 0) {
@@ -384,14 +424,14 @@ __global__ __launch_bounds__(256) void ellg_spmv_short_kernel(int R, int C, int 
         for (int g = 0; g < 4; ++g) {
             if (g < eg.n4) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) xs[g][j] = x[c[g].v[j] > 0 ? c[g].v[j] : 0];
+                for (int j = 0; j < 4; ++j) xs[g][j] = gx(c[g].v[j]);
             }
         }
         if (eg.two) {
 #pragma unroll
-            for (int j = 0; j < 2; ++j) x2[j] = x[c2.v[j] > 0 ? c2.v[j] : 0];
+            for (int j = 0; j < 2; ++j) x2[j] = gx(c2.v[j]);
         }
-        if (eg.one) x1 = x[c1 > 0 ? c1 : 0];
+        if (eg.one) x1 = gx(c1);
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             if (g < eg.n4) {
@@ -410,29 +450,35 @@ __global__ __launch_bounds__(256) void ellg_spmv_short_kernel(int R, int C, int 
     store_y(y, row, a.value(), alpha, beta);
 }
 
-template <typename VT, typename XT, typename AT, bool COMP>
-spmv_status_t launch_ellg(int R, int C, int width, bool off16, int rb, const void* d_col, const VT* d_val,
+template <typename VT, typename XT, typename AT, bool COMP, typename CT, bool W>
+static void launch_ellg_t(int R, int C, int rb, int win, const EllGroups& eg, const CT* col, const VT* val,
                           const XT* x, XT* y, AT alpha, AT beta, hipStream_t s) {
-    if (R <= 0) return SPMV_SUCCESS;
-    const EllGroups eg = ell_groups(R, width);
     // every load of a row at once where the round-based kernel needs two rounds (3 or 4 groups); with fewer
     // groups it has one round and fewer registers (the 5-point stencil in FP64 runs 9 % slower in this one)
     const bool shortrow = eg.n4 == 3 || eg.n4 == 4;
-    if (off16) {
-        if (shortrow)
-            hipLaunchKernelGGL((ellg_spmv_short_kernel<VT, XT, AT, COMP, short>), dim3(grid_for(R, 256)), dim3(256), 0,
-                               s, R, C, rb, eg, (const short*)d_col, d_val, x, y, alpha, beta);
-        else
-            hipLaunchKernelGGL((ellg_spmv_kernel<VT, XT, AT, COMP, short>), dim3(grid_for(R, 256)), dim3(256), 0, s, R,
-                               C, rb, eg, (const short*)d_col, d_val, x, y, alpha, beta);
-    } else {
-        if (shortrow)
-            hipLaunchKernelGGL((ellg_spmv_short_kernel<VT, XT, AT, COMP, int>), dim3(grid_for(R, 256)), dim3(256), 0,
-                               s, R, C, rb, eg, (const int*)d_col, d_val, x, y, alpha, beta);
-        else
-            hipLaunchKernelGGL((ellg_spmv_kernel<VT, XT, AT, COMP, int>), dim3(grid_for(R, 256)), dim3(256), 0, s, R,
-                               C, rb, eg, (const int*)d_col, d_val, x, y, alpha, beta);
-    }
+    const size_t lds = W ? sizeof(XT) * (256 + 2 * (size_t)win) : 0;
+    if (shortrow)
+        hipLaunchKernelGGL((ellg_spmv_short_kernel<VT, XT, AT, COMP, CT, W>), dim3(grid_for(R, 256)), dim3(256), lds,
+                           s, R, C, rb, win, eg, col, val, x, y, alpha, beta);
+    else
+        hipLaunchKernelGGL((ellg_spmv_kernel<VT, XT, AT, COMP, CT, W>), dim3(grid_for(R, 256)), dim3(256), lds, s, R,
+                           C, rb, win, eg, col, val, x, y, alpha, beta);
+}
+
+template <typename VT, typename XT, typename AT, bool COMP>
+spmv_status_t launch_ellg(int R, int C, int width, bool off16, int rb, int win, const void* d_col, const VT* d_val,
+                          const XT* x, XT* y, AT alpha, AT beta, hipStream_t s) {
+    if (R <= 0) return SPMV_SUCCESS;
+    const EllGroups eg = ell_groups(R, width);
+    if (off16 && win >= 0)
+        launch_ellg_t<VT, XT, AT, COMP, short, true>(R, C, rb, win, eg, (const short*)d_col, d_val, x, y, alpha,
+                                                     beta, s);
+    else if (off16)
+        launch_ellg_t<VT, XT, AT, COMP, short, false>(R, C, rb, 0, eg, (const short*)d_col, d_val, x, y, alpha,
+                                                      beta, s);
+    else
+        launch_ellg_t<VT, XT, AT, COMP, int, false>(R, C, rb, 0, eg, (const int*)d_col, d_val, x, y, alpha, beta,
+                                                    s);
     MUXI_LAUNCH_CHECK();
     return SPMV_SUCCESS;
 }
@@ -448,8 +494,8 @@ template spmv_status_t launch_csr_to_ell<bf16_t>(const DevCsr&, int, int*, bf16_
 #define INST(VT, XT, AT, COMP)                                                                                       \
     template spmv_status_t launch_ell<VT, XT, AT, COMP>(int, int, int, const int*, const VT*, const XT*, XT*, AT, AT, \
                                                         hipStream_t);                                                    \
-    template spmv_status_t launch_ellg<VT, XT, AT, COMP>(int, int, int, bool, int, const void*, const VT*, const XT*, \
-                                                         XT*, AT, AT, hipStream_t);
+    template spmv_status_t launch_ellg<VT, XT, AT, COMP>(int, int, int, bool, int, int, const void*, const VT*,     \
+                                                         const XT*, XT*, AT, AT, hipStream_t);
 MUXI_FOR_EACH_COMBO(INST)
 #undef INST
 

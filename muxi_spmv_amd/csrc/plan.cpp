@@ -233,19 +233,21 @@ static spmv_status_t plan_create_impl(const muxi_csr_t* Ain, spmv_type_t vt, spm
             const size_t slots = (size_t)R * (size_t)width;
             if (slots > 0) {
                 // 16-bit column offsets when every column is within 32767 of its row
-                int bad = 1;
+                int fit[2] = {1, 32768};  // needs more than 16 bits, largest |column - row|
                 {
                     int* d_bad = nullptr;
-                    PLAN_HIP(hipMalloc(&d_bad, sizeof(int)));
+                    PLAN_HIP(hipMalloc(&d_bad, 2 * sizeof(int)));
                     spmv_status_t st = launch_ell_offsets_fit(p->A, width, p->row_base, d_bad, s);
                     if (st == SPMV_SUCCESS &&
-                        (hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                        (hipMemcpyAsync(fit, d_bad, 2 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
                          hipStreamSynchronize(s) != hipSuccess))
                         st = SPMV_ERROR_MEMORY_COPY;
                     (void)hipFree(d_bad);
                     PLAN_TRY(st);
                 }
-                p->ecol16 = bad == 0;
+                p->ecol16 = fit[0] == 0;
+                // narrow bands (every column within 512 of its row) stage each block's x window in LDS
+                p->ell_win = p->ecol16 && fit[1] <= 512 ? fit[1] : -1;
                 const size_t csize = p->ecol16 ? sizeof(short) : sizeof(int);
                 PLAN_HIP(hipMalloc(&p->d_ecol, csize * slots));
                 PLAN_HIP(hipMalloc(&p->d_eval, vsize(vt) * slots));
@@ -399,8 +401,8 @@ static spmv_status_t plan_exec(const muxi_plan* p, const void* x, void* y, doubl
                 return launch_csr5<VT, XT, AT, C>(p->A, p->param0, p->d_tile_row, p->T, p->d_carry, xx, yy, a, b, s,
                                                   ev1);
             case MUXI_KERNEL_ELL:
-                return launch_ellg<VT, XT, AT, C>(p->A.R, p->A.C, p->width, p->ecol16, p->row_base, p->d_ecol,
-                                                  (const VT*)p->d_eval, xx, yy, a, b, s);
+                return launch_ellg<VT, XT, AT, C>(p->A.R, p->A.C, p->width, p->ecol16, p->row_base, p->ell_win,
+                                                  p->d_ecol, (const VT*)p->d_eval, xx, yy, a, b, s);
             case MUXI_KERNEL_XSLAB:
                 if constexpr (C) {
                     set_error("xslab: no compensated variant");

@@ -39,6 +39,7 @@ struct muxi_plan {
     void* d_ecol = nullptr;  // int columns, or short offsets col - (row + row_base) when ecol16
     bool ecol16 = false;
     int row_base = 0;        // a row block's first row in the full matrix (plan_create_rows), else 0
+    int ell_win = -1;        // ELL: every column within ell_win of its row (x window in LDS), -1 = no window
     void* d_eval = nullptr;
     // adaptive row blocks: blk_row[nblocks+1], blk_chunk[nblocks], long_row[nlong], long_first[2*nlong]
     int nblocks = 0;
