@@ -48,7 +48,7 @@ def test_plan_fp32(mname, kernel, param):
     assert_close_fp32(y, h, x)
 
 
-@pytest.mark.parametrize("mname", ["uniform", "powerlaw", "empty_rows", "long_row", "tiny"])
+@pytest.mark.parametrize("mname", ["uniform", "powerlaw", "empty_rows", "long_row", "tiny", "banded", "blockdiag"])
 @pytest.mark.parametrize("kernel,param", [("scalar", 0), ("vwave", 8), ("merge", 8), ("csr5", 256), ("ell", 0),
                                           ("adaptive", 1024), ("xslab", 1028)])
 def test_plan_fp64(mname, kernel, param):
