@@ -36,15 +36,17 @@ spmv_status_t muxi_batch_exec_ev(const muxi_batch_plan* plan, double alpha, doub
                                  void* ev_before, void* ev_after);
 /* The plain per-matrix path for comparison: one launch of the given row
  * kernel (MUXI_KERNEL_SCALAR, MUXI_KERNEL_VECTOR or MUXI_KERNEL_VWAVE with
- * param0 lanes) per matrix, y = A*x. */
+ * param0 lanes; any other id is SPMV_ERROR_INTERNAL) per matrix, y = A*x. */
 spmv_status_t muxi_batch_exec_per_matrix(const muxi_batch_plan* plan, muxi_kernel_t kernel, int param0,
                                          void* stream);
 /* Tiles, grid size and scratch bytes of the plan. */
 spmv_status_t muxi_batch_info(const muxi_batch_plan* plan, int* tiles, int* grid, size_t* scratch_bytes);
 
-/* hipGraph of one muxi_batch_exec (per_matrix_kernel = 0) or of one
- * muxi_batch_exec_per_matrix (its kernel id), captured on `stream` (must not
- * be the null stream), then replayed by muxi_graph_launch. */
+/* hipGraph of one muxi_batch_exec (per_matrix_kernel = 0) or of one launch
+ * per matrix with that kernel id (SCALAR, VECTOR or VWAVE with param0 lanes;
+ * any other id is SPMV_ERROR_INTERNAL), both computing
+ * ys[m] = alpha * mats[m] * xs[m] + beta * ys[m], captured on `stream` (must
+ * not be the null stream), then replayed by muxi_graph_launch. */
 typedef struct muxi_graph muxi_graph;
 spmv_status_t muxi_batch_graph_create(const muxi_batch_plan* plan, double alpha, double beta,
                                       int per_matrix_kernel, int param0, void* stream, muxi_graph** graph);

@@ -99,9 +99,39 @@ def _fn(base: str, dtype: torch.dtype):
     return getattr(load(), f"{base}_{suffix}")
 
 
-def _check_vec(t: torch.Tensor, dtype, n: int, name: str):
-    if t.dtype != dtype or t.numel() < n or not t.is_contiguous():
-        raise SpmvError(-3, name, f"expected contiguous {dtype} with >= {n} elements")
+def _check_vec(t, dtype, n: int, name: str, what: str = "vector") -> None:
+    """Raises SpmvError(SPMV_ERROR_INVALID_VECTOR) unless t is a contiguous tensor of `dtype` with at least n
+    elements: the kernels read and write through its raw address, so a short, strided or mistyped tensor would
+    otherwise be read or written out of bounds."""
+    if not torch.is_tensor(t):
+        raise SpmvError(-3, name, f"{what} must be a torch tensor")
+    if t.dtype != dtype:
+        raise SpmvError(-3, name, f"{what} must be {dtype} (got {t.dtype})")
+    if not t.is_contiguous():
+        raise SpmvError(-3, name, f"{what} must be contiguous")
+    if t.numel() < n:
+        raise SpmvError(-3, name, f"{what} needs >= {n} elements (got {t.numel()})")
+
+
+def _check_xy(x, y, dtype, n_x: int, n_y: int, name: str) -> None:
+    """x and y as _check_vec requires, both on the same GPU (a host tensor's address is not a device address)."""
+    _check_vec(x, dtype, n_x, name, "x")
+    _check_vec(y, dtype, n_y, name, "y")
+    for t, what in ((x, "x"), (y, "y")):
+        if not t.is_cuda:
+            raise SpmvError(-3, name, f"{what} must be on the GPU (got {t.device})")
+    if x.device != y.device:
+        raise SpmvError(-3, name, f"x and y on different devices ({x.device}, {y.device})")
+
+
+def _dims(matrix) -> tuple[int, int]:
+    """(numRows, numCols) of a CSRMatrix, CSR5Matrix or ELLPACKMatrix."""
+    if isinstance(matrix, CSR5Matrix):
+        return matrix.csr.numRows, matrix.csr.numCols
+    if isinstance(matrix, CSRMatrix):
+        return matrix.numRows, matrix.numCols
+    c = matrix.c
+    return int(c.numRows), int(c.numCols)
 
 
 # ---------------------------------------------------------------- handle / API
@@ -121,6 +151,11 @@ class Handle:
 
     def exec(self, fmt: int, matrix, x, y, alpha=1.0, beta=0.0, transpose=False) -> None:
         name = "muxi_spmv_exec_transpose" if transpose else "muxi_spmv_exec"
+        rows, cols = _dims(matrix)
+        if transpose:
+            rows, cols = cols, rows
+        dtype = matrix.csr.dtype if isinstance(matrix, CSR5Matrix) else matrix.dtype
+        _check_xy(x, y, dtype, cols, rows, name)
         f = _fn(name, x.dtype)
         check(f(self._h, fmt, C.cast(C.pointer(matrix.c), C.c_void_p), _ptr(x), _ptr(y), alpha, beta), name)
 
@@ -139,6 +174,10 @@ class Handle:
 
 def spmv_csr(A: CSRMatrix, x: torch.Tensor, y: torch.Tensor, alpha=1.0, beta=0.0, opts=None) -> None:
     """spmv_csr<T> (src/spmv/csr/spmv_csr.cu:52-249): y = alpha*A*x + beta*y."""
+    if opts is not None and opts.operation == _lib.SPMV_OP_TRANSPOSE:
+        _check_xy(x, y, A.dtype, A.numRows, A.numCols, "spmv_csr")
+    else:
+        _check_xy(x, y, A.dtype, A.numCols, A.numRows, "spmv_csr")
     o = opts if opts is not None else default_opts()
     f = _fn("muxi_spmv_csr", A.dtype)
     check(f(C.byref(A.c), _ptr(x), _ptr(y), alpha, beta, C.byref(o)), "spmv_csr")
@@ -151,18 +190,21 @@ def spmv_csr_exec(A, x, y, alpha=1.0, beta=0.0, opts=None) -> None:
 
 def spmv_merge_based(A: CSRMatrix, x, y, stream=None) -> None:
     """spmv_merge_based<T> (src/spmv/csr5/spmv_csr5.cu:930-1014): y = A*x."""
+    _check_xy(x, y, A.dtype, A.numCols, A.numRows, "spmv_merge_based")
     check(_fn("muxi_spmv_merge_based", A.dtype)(C.byref(A.c), _ptr(x), _ptr(y), _stream(stream)),
           "spmv_merge_based")
 
 
 def spmv_merge_based_v2(A: CSRMatrix, x, y, stream=None, useOptimized: bool = False) -> None:
     """spmv_merge_based_v2<T> (src/spmv/csr5/spmv_csr5.cu:1019-1072)."""
+    _check_xy(x, y, A.dtype, A.numCols, A.numRows, "spmv_merge_based_v2")
     check(_fn("muxi_spmv_merge_based_v2", A.dtype)(C.byref(A.c), _ptr(x), _ptr(y), _stream(stream),
                                                     bool(useOptimized)), "spmv_merge_based_v2")
 
 
 def spmv_sparse_optimized(A: CSRMatrix, x, y, stream=None) -> None:
     """spmv_sparse_optimized<T> (src/spmv/csr/spmv_sparse_optimized.cu:80-121)."""
+    _check_xy(x, y, A.dtype, A.numCols, A.numRows, "spmv_sparse_optimized")
     check(_fn("muxi_spmv_sparse_optimized", A.dtype)(C.byref(A.c), _ptr(x), _ptr(y), _stream(stream)),
           "spmv_sparse_optimized")
 
@@ -237,6 +279,7 @@ def csr5_preprocess(csr: CSRMatrix, sigma: int = 0, stream=None) -> CSR5Matrix:
 
 def spmv_csr5(A: CSR5Matrix, x, y, alpha=1.0, beta=0.0, opts=None) -> None:
     """spmv_csr5<T> (src/spmv/csr5/spmv_csr5.cu:277-362)."""
+    _check_xy(x, y, A.csr.dtype, A.csr.numCols, A.csr.numRows, "spmv_csr5")
     o = opts if opts is not None else default_opts()
     check(_fn("muxi_spmv_csr5", A.csr.dtype)(C.byref(A.c), _ptr(x), _ptr(y), alpha, beta, C.byref(o)), "spmv_csr5")
 
@@ -280,6 +323,7 @@ def csr_to_ellpack(csr: CSRMatrix, maxElements: int = 0) -> ELLPACKMatrix:
 
 def spmv_ellpack(E: ELLPACKMatrix, x, y, stream=None) -> None:
     """spmv_ellpack<T> (src/spmv/ellpack/spmv_ellpack.cu:72-91)."""
+    _check_xy(x, y, E.dtype, int(E.c.numCols), int(E.c.numRows), "spmv_ellpack")
     check(_fn("muxi_spmv_ellpack", E.dtype)(C.byref(E.c), _ptr(x), _ptr(y), _stream(stream)), "spmv_ellpack")
 
 
@@ -331,6 +375,7 @@ class Plan:
     def exec(self, x: torch.Tensor, y: torch.Tensor, alpha=1.0, beta=0.0, nrhs: int | None = None,
              stream=None) -> None:
         n = self.nrhs if nrhs is None else nrhs
+        _check_xy(x, y, self.xtype, self.A.numCols * n, self.A.numRows * n, "plan_exec")
         check(load().muxi_plan_exec(self._p, _ptr(x), _ptr(y), float(alpha), float(beta), n, _stream(stream)),
               "plan_exec")
 

@@ -295,30 +295,43 @@ extern "C" spmv_status_t muxi_batch_exec(const muxi_batch_plan* p, double alpha,
     return muxi_batch_exec_ev(p, alpha, beta, stream, nullptr, nullptr);
 }
 
-extern "C" spmv_status_t muxi_batch_exec_per_matrix(const muxi_batch_plan* p, muxi_kernel_t kernel, int param0,
-                                                    void* stream) {
-    if (!p) return SPMV_ERROR_INVALID_HANDLE;
-    hipStream_t s = (hipStream_t)stream;
+// One launch per member matrix (the baseline the batched launch is measured against): y_m = alpha A_m x_m +
+// beta y_m with the scalar, vector or virtual-wavefront kernel.
+template <typename T>
+static spmv_status_t batch_per_matrix(const muxi_batch_plan* p, muxi_kernel_t kernel, int param0, T alpha, T beta,
+                                      hipStream_t s) {
     for (const BatchDesc& d : p->host) {
         if (d.R == 0) continue;
         DevCsr A{d.R, d.C, d.N, d.rp, d.ci, d.v};
+        const T* x = (const T*)d.x;
+        T* y = (T*)d.y;
         spmv_status_t st;
-        if (p->xtype == SPMV_TYPE_DOUBLE) {
-            const double* x = (const double*)d.x;
-            double* y = (double*)d.y;
-            if (kernel == MUXI_KERNEL_SCALAR) st = launch_scalar<double, double, double, false>(A, x, y, 1.0, 0.0, s);
-            else if (kernel == MUXI_KERNEL_VWAVE) st = launch_vwave<double, double, double, false>(A, param0, x, y, 1.0, 0.0, s);
-            else st = launch_vector<double, double, double, false>(A, x, y, 1.0, 0.0, s);
-        } else {
-            const float* x = (const float*)d.x;
-            float* y = (float*)d.y;
-            if (kernel == MUXI_KERNEL_SCALAR) st = launch_scalar<float, float, float, false>(A, x, y, 1.f, 0.f, s);
-            else if (kernel == MUXI_KERNEL_VWAVE) st = launch_vwave<float, float, float, false>(A, param0, x, y, 1.f, 0.f, s);
-            else st = launch_vector<float, float, float, false>(A, x, y, 1.f, 0.f, s);
-        }
+        if (kernel == MUXI_KERNEL_SCALAR) st = launch_scalar<T, T, T, false>(A, x, y, alpha, beta, s);
+        else if (kernel == MUXI_KERNEL_VWAVE) st = launch_vwave<T, T, T, false>(A, param0, x, y, alpha, beta, s);
+        else st = launch_vector<T, T, T, false>(A, x, y, alpha, beta, s);
         if (st != SPMV_SUCCESS) return st;
     }
     return SPMV_SUCCESS;
+}
+
+static spmv_status_t batch_per_matrix_ab(const muxi_batch_plan* p, muxi_kernel_t kernel, int param0, double alpha,
+                                         double beta, hipStream_t s) {
+    if (!p) return SPMV_ERROR_INVALID_HANDLE;
+    if (!(kernel == MUXI_KERNEL_SCALAR || kernel == MUXI_KERNEL_VECTOR || kernel == MUXI_KERNEL_VWAVE)) {
+        set_error("batch per-matrix launches: kernel must be SCALAR, VECTOR or VWAVE");
+        return SPMV_ERROR_INTERNAL;
+    }
+    if (kernel == MUXI_KERNEL_VWAVE && !(param0 >= 1 && param0 <= 64 && (param0 & (param0 - 1)) == 0)) {
+        set_error("batch per-matrix launches: VWAVE lanes per row must be a power of two in [1, 64]");
+        return SPMV_ERROR_INTERNAL;
+    }
+    if (p->xtype == SPMV_TYPE_DOUBLE) return batch_per_matrix<double>(p, kernel, param0, alpha, beta, s);
+    return batch_per_matrix<float>(p, kernel, param0, (float)alpha, (float)beta, s);
+}
+
+extern "C" spmv_status_t muxi_batch_exec_per_matrix(const muxi_batch_plan* p, muxi_kernel_t kernel, int param0,
+                                                    void* stream) {
+    return batch_per_matrix_ab(p, kernel, param0, 1.0, 0.0, (hipStream_t)stream);
 }
 
 extern "C" spmv_status_t muxi_batch_info(const muxi_batch_plan* p, int* tiles, int* grid, size_t* scratch) {
@@ -337,10 +350,11 @@ extern "C" spmv_status_t muxi_batch_graph_create(const muxi_batch_plan* p, doubl
         set_error("graph capture needs a non-null stream");
         return SPMV_ERROR_INVALID_HANDLE;
     }
-    muxi_graph* g = new muxi_graph();
     MUXI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    spmv_status_t st = per_matrix_kernel ? muxi_batch_exec_per_matrix(p, (muxi_kernel_t)per_matrix_kernel, param0, s)
-                                         : muxi_batch_exec(p, alpha, beta, s);
+    muxi_graph* g = new muxi_graph();
+    spmv_status_t st = per_matrix_kernel
+                           ? batch_per_matrix_ab(p, (muxi_kernel_t)per_matrix_kernel, param0, alpha, beta, s)
+                           : muxi_batch_exec(p, alpha, beta, s);
     hipError_t e = hipStreamEndCapture(s, &g->g);
     if (st != SPMV_SUCCESS || e != hipSuccess) {
         if (g->g) (void)hipGraphDestroy(g->g);

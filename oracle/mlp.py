@@ -3,8 +3,10 @@
 FP64 numpy restatement of the network in include/muxi_spmv/muxi_spmv_tune.h
 (in -> h -> h -> out, ReLU on the hidden layers, parameters W1,b1,W2,b2,W3,b3
 row-major in one flat array), its masked mean-squared-error loss and the
-gradient by backpropagation, plus Adam. Pinned in tests/test_mlp_oracle.py
-by central finite differences of the loss.
+gradient by backpropagation, plus Adam. Pinned in tests/test_mlp_oracle.py:
+the gradient by central finite differences of the loss, Adam by the
+constant-gradient invariant (each bias-corrected step is lr * sign(g) at every
+t) and a hand-worked three-step case.
 """
 from __future__ import annotations
 

@@ -14,6 +14,9 @@
 //                      within the recursive-summation bound.
 //   spmv_f64           closed forms; exact on small-integer data; Dot2 vs
 //                      Python's exactly-rounded math.fsum per row.
+//   spmv_csr_f64_plain exact integer product (numpy int64) and bit-equality
+//                      with spmv_f64 on small-integer data; Laplacian and
+//                      diagonal closed forms; recursive-summation bound vs fsum.
 //   merge_*            brute force over all path diagonals on tiny matrices
 //                      (the coordinate is the unique r with
 //                      rowPtr[r]+r <= k <= rowPtr[r+1]+r) and the reference
@@ -21,6 +24,9 @@
 //   csr5_tiles         numpy searchsorted(side="right")-1 on random rowPtr,
 //                      incl. runs of empty rows.
 //   csr_to_ell         dense reconstruction equals the CSR matrix.
+//   spmv_ell_f64       exact integer product of the CSR matrix (padded widths)
+//                      and of the matrix cut to the first w entries per row
+//                      (truncated widths).
 //   permutations       bijection checks, key order, golden fixtures; the
 //                      unstable std::sort ties are libstdc++'s and are pinned
 //                      only by the committed fixtures (see header of

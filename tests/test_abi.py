@@ -5,6 +5,7 @@ import ctypes as C
 import re
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 from muxi_spmv_amd import _lib
@@ -151,3 +152,46 @@ def test_host_plan_host_side_errors():
     assert lib.muxi_host_plan_destroy(None) == -1
     assert lib.muxi_host_alloc(None, 16) == -3
     assert lib.muxi_host_free(None) == 0
+
+
+def test_python_wrappers_reject_bad_vectors_before_any_launch():
+    # Every wrapper checks x and y (dtype, contiguity, length, device) before it hands raw addresses to the
+    # library: a short y would be written out of bounds, a host tensor read as a device address.
+    import torch
+
+    from muxi_spmv_amd import api
+    from muxi_spmv_amd._lib import SpmvError
+
+    rp = np.array([0, 2, 3, 5], np.int32)
+    ci = np.array([0, 2, 1, 0, 3], np.int32)
+    A = api.CSRMatrix(rp, ci, np.ones(5, np.float32), numCols=4, device="cpu")
+    good_x, good_y = torch.zeros(4), torch.zeros(3)
+    cases = [
+        (torch.zeros(3), good_y, "x needs >= 4"),
+        (good_x, torch.zeros(2), "y needs >= 3"),
+        (torch.zeros(4, dtype=torch.float64), good_y, "x must be torch.float32"),
+        (torch.zeros(8)[::2], good_y, "x must be contiguous"),
+        (good_x, good_y, "x must be on the GPU"),
+        ([0.0] * 4, good_y, "x must be a torch tensor"),
+    ]
+    for fn in (api.spmv_csr, api.spmv_merge_based, api.spmv_sparse_optimized):
+        for x, y, msg in cases:
+            with pytest.raises(SpmvError, match=msg) as ei:
+                fn(A, x, y)
+            assert ei.value.status == -3
+    # transpose: x has numRows entries, y numCols
+    with pytest.raises(SpmvError, match="y needs >= 4"):
+        api.spmv_csr(A, torch.zeros(3), torch.zeros(3), opts=_opts_transpose())
+    # a plan checks against its x type and nrhs without touching the device
+    p = api.Plan.__new__(api.Plan)
+    p.A, p.xtype, p.nrhs, p._p = A, torch.float32, 2, None
+    with pytest.raises(SpmvError, match="x needs >= 8"):
+        p.exec(torch.zeros(4), torch.zeros(6))
+    with pytest.raises(SpmvError, match="y needs >= 6"):
+        p.exec(torch.zeros(8), torch.zeros(3))
+
+
+def _opts_transpose():
+    o = _lib.spmv_opts_t()
+    o.operation = _lib.SPMV_OP_TRANSPOSE
+    return o

@@ -94,6 +94,60 @@ def test_f32_loop_within_summation_bound():
     assert np.all(np.abs(y32 - y64) <= bound)
 
 
+def int_csr(R, C, max_len, seed, empty_frac=0.2):
+    """Random CSR with small-integer values: every product and partial sum is an integer far below 2^53, so any
+    order of FP64 summation gives the exact integer answer."""
+    rp, ci, _ = rand_csr(R, C, max_len, seed, empty_frac)
+    rng = np.random.default_rng(seed + 1000)
+    v = rng.integers(-50, 51, size=ci.size).astype(np.float64)
+    x = rng.integers(-50, 51, size=C).astype(np.float64)
+    return rp, ci, v, x
+
+
+def dense_of(rp, ci, v, C):
+    R = rp.size - 1
+    D = np.zeros((R, C), np.int64)
+    for r in range(R):
+        D[r, ci[rp[r]:rp[r + 1]]] = v[rp[r]:rp[r + 1]].astype(np.int64)
+    return D
+
+
+def test_f64_plain_exact_on_integers_and_closed_forms():
+    # integer data: the plain FP64 loop must equal the exact integer product (numpy int64 on the dense matrix)
+    # and the Dot2 oracle bit for bit
+    rp, ci, v, x = int_csr(400, 300, 40, seed=31)
+    y = oracle.spmv_csr_f64_plain(rp, ci, v, x)
+    exact = dense_of(rp, ci, v, 300) @ x.astype(np.int64)
+    assert np.array_equal(y, exact.astype(np.float64))
+    assert np.array_equal(y, oracle.spmv_f64(rp, ci, v, x)[0])
+    # Laplacian with x = 1 (interior 0, edges 1, corners 2) and the diagonal known answer
+    n = 29
+    A = G.laplacian_2d(n)
+    y = oracle.spmv_csr_f64_plain(A.rowPtr, A.colIdx, A.values, np.ones(n * n))
+    exp = np.zeros((n, n))
+    exp[0, :] += 1
+    exp[-1, :] += 1
+    exp[:, 0] += 1
+    exp[:, -1] += 1
+    assert np.array_equal(y, exp.ravel())
+    d = np.arange(1, 65, dtype=np.float64)
+    y = oracle.spmv_csr_f64_plain(np.arange(65, dtype=np.int32), np.arange(64, dtype=np.int32), np.ones(64), d)
+    assert np.array_equal(y, d)
+
+
+def test_f64_plain_within_summation_bound():
+    # real data: |plain - exact| <= gamma_n * sum |a x| (recursive summation), gamma_n ~ n u with u = 2^-53;
+    # the row order of the loop is what a dropped or doubled term would break by far more than this
+    rp, ci, v = rand_csr(500, 800, 60, seed=33, dtype=np.float64)
+    x = np.random.default_rng(34).standard_normal(800)
+    y = oracle.spmv_csr_f64_plain(rp, ci, v, x)
+    for r in range(500):
+        terms = [float(v[i]) * float(x[ci[i]]) for i in range(rp[r], rp[r + 1])]
+        n = len(terms)
+        bound = (n + 1) * 2.0 ** -53 * math.fsum(abs(t) for t in terms) * 1.0001 + 1e-300
+        assert abs(y[r] - math.fsum(terms)) <= bound
+
+
 def test_f32_mt_equals_single_thread():
     rp, ci, v = rand_csr(5000, 5000, 30, seed=7)
     x = np.random.default_rng(8).uniform(-1, 1, 5000).astype(np.float32)
@@ -205,6 +259,36 @@ def test_ell_truncation():
     ec, ev = oracle.csr_to_ell(rp, ci, v, 3)
     assert ec.tolist() == [0, 5, 1, -1, 2, -1]
     assert ev.tolist() == [1, 6, 2, 0, 3, 0]
+
+
+@pytest.mark.parametrize("extra", [0, 3])
+def test_ell_spmv_exact_on_integers(extra):
+    # the ELL product over the packed slots (padding skipped) equals the exact integer product of the CSR
+    # matrix: width = max row length, and with `extra` more padded slots per row
+    rp, ci, v, x = int_csr(300, 250, 25, seed=41)
+    w = oracle.ell_width(rp) + extra
+    ec, ev = oracle.csr_to_ell(rp, ci, v, w)
+    y = oracle.spmv_ell_f64(ec, ev, w, x)
+    exact = dense_of(rp, ci, v, 250) @ x.astype(np.int64)
+    assert np.array_equal(y, exact.astype(np.float64))
+    assert np.array_equal(y, oracle.spmv_csr_f64_plain(rp, ci, v, x))
+
+
+def test_ell_spmv_truncated_width_keeps_first_slots():
+    # a width below the longest row keeps each row's first `w` nonzeros (spmv_ellpack.cu:49): the product equals
+    # the exact product of the CSR matrix cut to those entries
+    rp, ci, v, x = int_csr(200, 180, 30, seed=43, empty_frac=0.0)
+    w = 7
+    ec, ev = oracle.csr_to_ell(rp, ci, v, w)
+    y = oracle.spmv_ell_f64(ec, ev, w, x)
+    R = rp.size - 1
+    cut = np.array([min(rp[r + 1] - rp[r], w) for r in range(R)])
+    crp = np.zeros(R + 1, np.int32)
+    np.cumsum(cut, out=crp[1:])
+    keep = np.concatenate([np.arange(rp[r], rp[r] + cut[r]) for r in range(R)])
+    exact = dense_of(crp, ci[keep], v[keep], 180) @ x.astype(np.int64)
+    assert np.array_equal(y, exact.astype(np.float64))
+    assert (np.diff(rp) > w).any()  # the case really truncates
 
 
 # ---------------------------------------------------------------- permutations
