@@ -227,7 +227,11 @@ typedef enum {
     MUXI_KERNEL_XSLAB = 8,     /* column slabs, slab = workgroup mod slabs (one x slice per XCD L2);
                                   param0 = (slabs << 8) | items per lane; slabs 0 = the fewest of
                                   1, 2, 4, 8 that keep a slab of x within 2 MiB; items 0 = 4 */
-    MUXI_KERNEL_COUNT = 9
+    MUXI_KERNEL_LSLAB = 9,     /* column slabs of 32768 columns with the slab of x in LDS, pair sums combined
+                                  per row block in LDS; param0 = nonzeros per lane (4 or 8; 0 = 4)
+                                  | 0x100 (slabs of 16384 columns) | 0x200 (persistent combine);
+                                  FP32 x and y, at most 8,388,608 columns */
+    MUXI_KERNEL_COUNT = 10
 } muxi_kernel_t;
 
 /* Accumulation: 0 = plain, 1 = compensated (two-sum) accumulation. */

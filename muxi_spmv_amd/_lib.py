@@ -24,7 +24,7 @@ SPMV_TYPE_FLOAT, SPMV_TYPE_DOUBLE, SPMV_TYPE_HALF, SPMV_TYPE_BF16 = 0, 1, 2, 3
 SPMV_OP_NORMAL, SPMV_OP_TRANSPOSE = 0, 1
 
 KERNELS = {"auto": 0, "scalar": 1, "vector": 2, "vwave": 3, "adaptive": 4, "merge": 5, "csr5": 6, "ell": 7,
-           "xslab": 8}
+           "xslab": 8, "lslab": 9}
 KERNEL_NAMES = {v: k for k, v in KERNELS.items()}
 
 

@@ -44,7 +44,7 @@ FLAGS = [
 ]
 
 WEIGHTS = PKG / "selector_weights.json"
-_KERNEL_IDS = {"scalar": 1, "vector": 2, "vwave": 3, "adaptive": 4, "merge": 5, "csr5": 6, "ell": 7, "xslab": 8}
+_KERNEL_IDS = {"scalar": 1, "vector": 2, "vwave": 3, "adaptive": 4, "merge": 5, "csr5": 6, "ell": 7, "xslab": 8, "lslab": 9}
 
 
 def write_selector_header() -> Path | None:

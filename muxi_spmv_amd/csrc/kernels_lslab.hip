@@ -1,0 +1,887 @@
+// LDS column-slab SpMV ("lslab"): x gathers from LDS instead of L2.
+//
+// With uniformly scattered columns every nonzero of a plain CSR kernel costs
+// one random 4-byte gather, and L2 serves those at one request per gather:
+// on MI355X about 220-270 G gathers/s, a quarter of what the HBM stream of
+// the matrix itself could sustain. This plan cuts the columns into slabs of
+// 32,768 (the slab of FP32 x, 128 KiB, fits one CU's LDS) and runs two
+// launches:
+//
+// 1. Slab kernel. The nonzeros of each slab, in row order, form one stream
+//    of 16-bit column offsets (bit 15 set on the last nonzero of the row
+//    inside the slab) and values. A stream is cut at row ends into chunks
+//    of about equal size, padded to whole tiles of 64*EPL nonzeros; one
+//    1024-lane workgroup per chunk copies its slab of x into LDS once and
+//    then walks its tiles, one wavefront per tile: a lane loads its EPL
+//    offsets and values with one vector load each, gathers x from LDS,
+//    sums its products cutting at its row ends, and a segmented wave scan
+//    in DPP carries open sums between lanes. The sum of a (row, slab) pair
+//    goes to P[pair], pairs numbered in stream order. A pair that starts in
+//    an earlier tile of the chunk is finished after a barrier from the
+//    tiles' head and tail sums kept in LDS (added in a fixed order).
+// 2. Combine kernel. Rows are cut into blocks (at most 4,096 rows and a
+//    fixed number of pairs). A workgroup copies its block's pair sums from
+//    every slab (one contiguous run per slab) into LDS, then reads the
+//    block's pairs in row-major order through a static map (LDS index,
+//    row, last-of-row bit), adds each row's pairs in slab order with a
+//    segmented wave scan, and writes y for its rows in one coalesced pass.
+//
+// The matrix is read once as 6 bytes a nonzero (FP32 values), the pairs
+// (about 0.44 per nonzero on the 2M power-law matrix) cost 4 bytes written
+// and 8 read, and no access goes to L2 at random. Every sum has a fixed
+// order, so results are reproducible bit for bit.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace muxi {
+
+// ---------------------------------------------------------------- pair sums
+template <bool COMP> struct LsPair;
+template <> struct LsPair<false> {
+    float s;
+    __device__ __forceinline__ static LsPair zero() { LsPair r; r.s = 0.f; return r; }
+    __device__ __forceinline__ void add(const LsPair& o) { s += o.s; }
+    template <typename VT>
+    __device__ __forceinline__ static LsPair prod(VT v, float xv) { LsPair r; r.s = to_f32(v) * xv; return r; }
+    __device__ __forceinline__ float value() const { return s; }
+};
+// Dot2 (Ogita, Rump, Oishi 2005): TwoProduct by FMA, TwoSum, errors summed in c.
+template <> struct LsPair<true> {
+    float s, c;
+    __device__ __forceinline__ static LsPair zero() { LsPair r; r.s = 0.f; r.c = 0.f; return r; }
+    __device__ __forceinline__ void add(const LsPair& o) {
+        const float t = __fadd_rn(s, o.s);
+        const float bp = __fsub_rn(t, s);
+        const float e = __fadd_rn(__fsub_rn(s, __fsub_rn(t, bp)), __fsub_rn(o.s, bp));
+        s = t;
+        c = __fadd_rn(c, __fadd_rn(o.c, e));
+    }
+    template <typename VT>
+    __device__ __forceinline__ static LsPair prod(VT v, float xv) {
+        const float a = to_f32(v);
+        LsPair r;
+        r.s = __fmul_rn(a, xv);
+        r.c = __fmaf_rn(a, xv, -r.s);  // the product's exact rounding error
+        return r;
+    }
+    __device__ __forceinline__ float value() const { return __fadd_rn(s, c); }
+};
+
+// DPP cross-lane moves (GFX9 encodings: row_shr:n = 0x110 | n within a row of
+// 16 lanes, row_bcast:15 = 0x142, row_bcast:31 = 0x143, wave_shr:1 = 0x138);
+// lanes without a source read 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int ls_dpp(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ LsPair<false> ls_dpp(LsPair<false> v) {
+    LsPair<false> r;
+    r.s = __int_as_float(ls_dpp<CTRL, ROWS>(__float_as_int(v.s)));
+    return r;
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ LsPair<true> ls_dpp(LsPair<true> v) {
+    LsPair<true> r;
+    r.s = __int_as_float(ls_dpp<CTRL, ROWS>(__float_as_int(v.s)));
+    r.c = __int_as_float(ls_dpp<CTRL, ROWS>(__float_as_int(v.c)));
+    return r;
+}
+
+// One step of an inclusive segmented scan: a lane whose flag is set starts a
+// segment and takes nothing from below.
+template <int CTRL, int ROWS, bool COMP>
+__device__ __forceinline__ void ls_seg_step(LsPair<COMP>& v, int& f) {
+    LsPair<COMP> u = ls_dpp<CTRL, ROWS>(v);
+    const int fu = ls_dpp<CTRL, ROWS>(f);
+    if (!f) {
+        u.add(v);
+        v = u;
+    }
+    f |= fu;
+}
+// Shifts of 1, 2, 4, 8 inside each row of 16 lanes, then row 0's total into
+// rows 1 and 3 and rows 0-1's total into rows 2 and 3.
+template <bool COMP>
+__device__ __forceinline__ void ls_seg_scan(LsPair<COMP>& v, int& f) {
+    ls_seg_step<0x111, 0xf>(v, f);
+    ls_seg_step<0x112, 0xf>(v, f);
+    ls_seg_step<0x114, 0xf>(v, f);
+    ls_seg_step<0x118, 0xf>(v, f);
+    ls_seg_step<0x142, 0xa>(v, f);
+    ls_seg_step<0x143, 0xc>(v, f);
+}
+template <bool COMP>
+__device__ __forceinline__ LsPair<COMP> ls_readlane63(LsPair<COMP> v);
+template <>
+__device__ __forceinline__ LsPair<false> ls_readlane63(LsPair<false> v) {
+    LsPair<false> r;
+    r.s = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.s), 63));
+    return r;
+}
+template <>
+__device__ __forceinline__ LsPair<true> ls_readlane63(LsPair<true> v) {
+    LsPair<true> r;
+    r.s = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.s), 63));
+    r.c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.c), 63));
+    return r;
+}
+
+// Exclusive prefix over the wave of a per-lane count in [0, 15], and the total.
+__device__ __forceinline__ int ls_count_prefix(int n, int& total) {
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const unsigned long long m = __ballot((n >> b) & 1);
+        pre += __popcll(m & __lanemask_lt()) << b;
+        tot += __popcll(m) << b;
+    }
+    total = tot;
+    return pre;
+}
+
+// N consecutive elements with one non-temporal vector load (aligned: tiles start on 64*EPL elements).
+template <typename T, int N>
+struct LsLoad {
+    __device__ __forceinline__ static void run(const T* p, T* out) {
+        typedef T V __attribute__((ext_vector_type(N)));
+        const V v = __builtin_nontemporal_load((const V*)p);
+#pragma unroll
+        for (int j = 0; j < N; ++j) out[j] = v[j];
+    }
+};
+template <int N>
+struct LsLoad<bf16_t, N> {
+    __device__ __forceinline__ static void run(const bf16_t* p, bf16_t* out) {
+        typedef uint16_t V __attribute__((ext_vector_type(N)));
+        const V v = __builtin_nontemporal_load((const V*)p);
+#pragma unroll
+        for (int j = 0; j < N; ++j) out[j].bits = v[j];
+    }
+};
+
+// ---------------------------------------------------------------- the slab kernel
+// chunks[4b ..] = (tile0, ntiles, first column of the slab, 0); ntiles = 0 is an empty launch slot.
+// tile_pair[t] = index of the pair holding tile t's first element, bit 31 set when that pair started in an
+// earlier tile. The sum of pair j goes to P[j] (pairs in stream order: slab, then row).
+template <typename VT, bool COMP, int EPL, int W>
+__global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __restrict__ chunks,
+                                                                const uint16_t* __restrict__ lc,
+                                                                const VT* __restrict__ lv,
+                                                                const int* __restrict__ tile_pair,
+                                                                const float* __restrict__ x, int C,
+                                                                LsPair<COMP>* __restrict__ P) {
+    using PT = LsPair<COMP>;
+    constexpr int TILE = 64 * EPL;
+    constexpr int NWAVE = kLsThreads / 64;
+    __shared__ float xs[W];
+    // per wavefront: one tile's pair sums, where the LDS holds them beside the slab of x
+    constexpr bool STAGE = !COMP && (W + kLsThreads * EPL) * 4 + kLsMaxTiles * 9 <= kLsLdsBytes;
+    __shared__ float stage[STAGE ? kLsThreads * EPL : 1];
+    __shared__ PT head[kLsMaxTiles];
+    __shared__ PT tail[kLsMaxTiles];
+    __shared__ unsigned char tflag[kLsMaxTiles];  // bit 0: the tile has a row end; bit 1: starts mid-pair
+    const int4 ch = chunks[blockIdx.x];
+    const int ntiles = ch.y;
+    if (ntiles == 0) return;  // whole workgroup, before any barrier
+    const int x0 = ch.z;
+    const int width = min(W, C - x0);
+    const int tid = threadIdx.x;
+
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // a lane's EPL offsets and values of tile lt (one vector load each)
+    auto load_tile = [&](int lt, uint16_t* c, VT* v) {
+        const long e0 = ((long)ch.x + lt) * TILE + lane * EPL;
+        LsLoad<uint16_t, EPL>::run(lc + e0, c);
+        LsLoad<VT, EPL>::run(lv + e0, v);
+    };
+    uint16_t cn[EPL];
+    VT vn[EPL];
+    // the slab of x into LDS: a whole slab with eight 16-byte loads per lane in flight together (then the first
+    // tile's loads join them), the last, partial slab one element at a time
+    const float* xg = x + x0;
+    if (width == W && (((uintptr_t)xg) & 15) == 0) {
+        constexpr int K = W / (4 * kLsThreads);
+        float4 v[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = *(const float4*)(xg + (tid + k * kLsThreads) * 4);
+        if (wave < ntiles) load_tile(wave, cn, vn);
+#pragma unroll
+        for (int k = 0; k < K; ++k) *(float4*)(xs + (tid + k * kLsThreads) * 4) = v[k];
+    } else {
+        if (wave < ntiles) load_tile(wave, cn, vn);
+        for (int i = tid; i < width; i += kLsThreads) xs[i] = xg[i];
+    }
+    __syncthreads();
+
+    // one tile: products from LDS x, the lane's segmented sums, a DPP scan across lanes, pair sums to P
+    auto process = [&](int lt, const uint16_t* c, const VT* v) {
+        const long t = (long)ch.x + lt;
+        const int tp = tile_pair[t];  // t is uniform: a scalar load
+        const bool mid = tp < 0;
+        const int pair0 = tp & 0x7fffffff;
+        PT p[EPL];
+        unsigned f = 0;
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) {
+            p[j] = PT::prod(v[j], xs[c[j] & 0x7fff]);
+            f |= (unsigned)(c[j] >> 15) << j;
+        }
+        // the sum left open after the lane's last row end (all of it if none)
+        PT seg = PT::zero();
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) {
+            seg.add(p[j]);
+            if ((f >> j) & 1u) seg = PT::zero();
+        }
+        int total;
+        const int before = ls_count_prefix(__popc(f), total);
+        PT sc = seg;
+        int fl = f != 0u;
+        ls_seg_scan<COMP>(sc, fl);
+        PT run = ls_dpp<0x138, 0xf>(sc);  // open sum entering the lane (0 for lane 0)
+        // the lane's k-th row end closes pair pair0 + before + k; the tile's first end, when its pair started in
+        // an earlier tile, is finished after the barrier
+        if constexpr (STAGE) {
+            // the tile's pair sums go to the wavefront's LDS stage in pair order, then to P with contiguous,
+            // whole-wavefront stores (scattered per-lane stores cost more than everything else here)
+            float* st = stage + wave * (64 * EPL);
+#pragma unroll
+            for (int j = 0; j < EPL; ++j) {
+                run.add(p[j]);
+                if ((f >> j) & 1u) {
+                    st[before + __popc(f & ((1u << j) - 1u))] = run.value();
+                    run = PT::zero();
+                }
+            }
+            const int first = mid ? 1 : 0;
+            for (int i = first + lane; i < total; i += 64) P[pair0 + i].s = st[i];
+            if (mid && lane == 0 && total > 0) head[lt].s = st[0];
+        } else {
+            PT* out = P + pair0 + before;
+            const bool skip_first = mid && before == 0;
+#pragma unroll
+            for (int j = 0; j < EPL; ++j) {
+                run.add(p[j]);
+                if ((f >> j) & 1u) {
+                    const int k = __popc(f & ((1u << j) - 1u));
+                    if (k == 0 && skip_first) head[lt] = run;
+                    else out[k] = run;
+                    run = PT::zero();
+                }
+            }
+        }
+        const PT open = ls_readlane63<COMP>(sc);  // the sum open at the tile end (0 after a final row end)
+        if (lane == 0) {
+            tail[lt] = open;
+            tflag[lt] = (unsigned char)((total > 0 ? 1 : 0) | (mid ? 2 : 0));
+        }
+    };
+    for (int lt = wave; lt < ntiles; lt += NWAVE) {
+        uint16_t c[EPL];
+        VT v[EPL];
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) {
+            c[j] = cn[j];
+            v[j] = vn[j];
+        }
+        if (lt + NWAVE < ntiles) load_tile(lt + NWAVE, cn, vn);  // the next tile's loads fly meanwhile
+        process(lt, c, v);
+    }
+    __syncthreads();
+    // pairs that started in an earlier tile of the chunk: head + tails of the tiles before, newest first (a
+    // chunk starts on a pair boundary, so the walk ends at its first tile at the latest)
+    for (int lt = tid; lt < ntiles; lt += kLsThreads) {
+        if (tflag[lt] != 3) continue;
+        PT s = head[lt];
+        for (int q = lt - 1; q >= 0; --q) {
+            s.add(tail[q]);
+            if ((tflag[q] & 1) || q == 0) break;
+        }
+        P[tile_pair[ch.x + lt] & 0x7fffffff] = s;
+    }
+}
+
+// ---------------------------------------------------------------- the combine kernel
+// blocks[8b ..] = (first row, rows, first map entry, pairs, wave split 1, 2, 3, run table offset); wave split w =
+// (first map entry | first row << 16) of wavefront w, at a row boundary (split 0 = (0, 0), split 4 = (pairs,
+// rows)). The block's
+// pairs, slab by slab, form S runs of P (pairs in stream order): runs[off + s] = (first pair of the run in P,
+// its offset in the block's LDS copy), runs[off + S] = (0, pairs). map[first .. first + pairs) lists the block's
+// pairs in row-major order: LDS index (bits 0-15), row offset in the block (bits 16-30), bit 31 on the row's
+// last pair. Wavefront w sums the rows of its map entries and writes their y.
+
+// A wavefront's share of one block's run table, as loaded: lane l holds runs[off + wave + 4 l] and the entry
+// after it (its LDS end). Nothing reads these until the next block starts, so the loads stay in flight.
+struct LsRunsRaw {
+    int2 d0, d1;
+};
+
+__device__ __forceinline__ void ls_block_info(const int* __restrict__ blocks, int b, int* bi) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) bi[k] = blocks[8 * (long)b + k];  // uniform: scalar loads
+}
+
+__device__ __forceinline__ LsRunsRaw ls_runs(const int2* __restrict__ runs, int roff, int S, int wave, int lane) {
+    LsRunsRaw r{{0, 0}, {0, 0}};
+    if (lane < (S - wave + 3) / 4) {
+        const int q = wave + 4 * lane;
+        r.d0 = runs[roff + q];
+        r.d1 = runs[roff + q + 1];
+    }
+    return r;
+}
+
+// Starts the direct global->LDS loads of one block's map entries and pair sums (nothing waits here).
+template <int PD>
+__device__ __forceinline__ void ls_issue(const int* bi, int S, const LsRunsRaw& r, const unsigned* __restrict__ map,
+                                         const unsigned* __restrict__ P, unsigned* __restrict__ pb,
+                                         unsigned* __restrict__ lmap, int wave, int lane) {
+    const int mbase = bi[2], np = bi[3];
+    for (int e0 = wave * 64; e0 < np; e0 += 256)
+        if (e0 + lane < np)
+            __builtin_amdgcn_global_load_lds((const void*)(map + mbase + e0 + lane),
+                                             (__attribute__((address_space(3))) void*)(lmap + e0), 4, 0, 0);
+    const int nr = (S - wave + 3) / 4;
+    const int len = r.d1.y - r.d0.y;
+    for (int l = 0; l < nr; ++l) {
+        const int src = __builtin_amdgcn_readlane(r.d0.x, l), dst = __builtin_amdgcn_readlane(r.d0.y, l);
+        const int n = __builtin_amdgcn_readlane(len, l) * PD;
+        const unsigned* g = P + (long)src * PD;
+        unsigned* d = pb + (long)dst * PD;
+        for (int e0 = 0; e0 < n; e0 += 64)
+            if (e0 + lane < n)
+                __builtin_amdgcn_global_load_lds((const void*)(g + e0 + lane),
+                                                 (__attribute__((address_space(3))) void*)(d + e0), 4, 0, 0);
+    }
+}
+
+// A wavefront sums its rows of one block from LDS (its map range starts and ends on row boundaries): rounds of
+// 64 * E entries start on a multiple of E, a lane takes E consecutive entries (two 16-byte LDS reads), and
+// entries outside [a, b) count as zero and end nothing. Each row's last pair writes the row's sum to yl.
+template <bool COMP>
+__device__ __forceinline__ void ls_rows(const int* bi, const LsPair<COMP>* __restrict__ pb,
+                                        const unsigned* __restrict__ lmap, float* __restrict__ yl, int wave,
+                                        int lane) {
+    using PT = LsPair<COMP>;
+    constexpr int E = 8;
+    const int np = bi[3];
+    const int a = wave == 0 ? 0 : bi[3 + wave] & 0xffff, b = wave == 3 ? np : bi[4 + wave] & 0xffff;
+    PT carry = PT::zero();
+    for (int i0 = a & ~(E - 1); i0 < b; i0 += 64 * E) {
+        const int ib = i0 + lane * E;
+        unsigned m[E];
+        {
+            const uint4 u0 = *(const uint4*)(lmap + ib), u1 = *(const uint4*)(lmap + ib + 4);
+            m[0] = u0.x; m[1] = u0.y; m[2] = u0.z; m[3] = u0.w;
+            m[4] = u1.x; m[5] = u1.y; m[6] = u1.z; m[7] = u1.w;
+        }
+        PT p[E];
+        unsigned f = 0;
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            const int i = ib + j;
+            const bool in = i >= a && i < b;
+            p[j] = in ? pb[m[j] & 0xffffu] : PT::zero();
+            f |= (in ? (m[j] >> 31) : 0u) << j;
+        }
+        PT seg = PT::zero();
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            seg.add(p[j]);
+            if ((f >> j) & 1u) seg = PT::zero();
+        }
+        // the row open at the end of the previous round continues in lane 0 (carry is 0 in the first round, and
+        // 0 after a round whose last entry ended a row)
+        PT sc = seg;
+        if (lane == 0 && f == 0u) {
+            PT c = carry;
+            c.add(sc);
+            sc = c;
+        }
+        int fl = f != 0u;
+        ls_seg_scan<COMP>(sc, fl);
+        PT run = ls_dpp<0x138, 0xf>(sc);
+        if (lane == 0) run = carry;
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            run.add(p[j]);
+            if ((f >> j) & 1u) {
+                yl[(m[j] >> 16) & 0x7fffu] = run.value();
+                run = PT::zero();
+            }
+        }
+        carry = ls_readlane63<COMP>(sc);
+    }
+}
+
+// One step of the combine: start the loads of the next block into one pair of LDS buffers, the run tables of
+// the block after it into registers, then sum the current block from the other pair. All buffers are
+// distinct (restrict), so the sums read LDS without waiting for the loads in flight.
+template <bool COMP>
+__device__ __forceinline__ void ls_step(const int* bc, const int* bn, bool more, const int* bnn, bool more2, int S,
+                                        const LsRunsRaw& rn, LsRunsRaw& rnn, const int2* __restrict__ runs,
+                                        const unsigned* __restrict__ map, const unsigned* __restrict__ P,
+                                        const LsPair<COMP>* __restrict__ cur_pb, const unsigned* __restrict__ cur_lm,
+                                        unsigned* __restrict__ nxt_pb, unsigned* __restrict__ nxt_lm,
+                                        float* __restrict__ yl, int wave, int lane) {
+    constexpr int PD = (int)sizeof(LsPair<COMP>) / 4;
+    if (more) ls_issue<PD>(bn, S, rn, map, P, nxt_pb, nxt_lm, wave, lane);
+    if (more2) rnn = ls_runs(runs, bnn[7], S, wave, lane);
+    ls_rows<COMP>(bc, cur_pb, cur_lm, yl, wave, lane);
+}
+
+// Persistent: workgroup g takes blocks g, g + grid, ... with two LDS buffers. While it sums one block from LDS,
+// the direct loads of its next block are in flight, and the run table of the block after that is loading.
+template <bool COMP, int PAIR_BYTES, int ROWS>
+__global__ __launch_bounds__(256) void lslab_combine_kernel(const int* __restrict__ blocks, int B, int S,
+                                                            const int2* __restrict__ runs,
+                                                            const unsigned* __restrict__ map,
+                                                            const LsPair<COMP>* __restrict__ P, float* __restrict__ y,
+                                                            float alpha, float beta) {
+    using PT = LsPair<COMP>;
+    constexpr int PD = (int)sizeof(PT) / 4;  // dwords per pair sum
+    constexpr int CAP = PAIR_BYTES / (int)sizeof(PT);
+    constexpr int MC = PAIR_BYTES / 4 + 8;  // map entries (+ the tail of a lane's last 16-byte reads)
+    __shared__ __attribute__((aligned(16))) PT pb0[CAP];
+    __shared__ __attribute__((aligned(16))) PT pb1[CAP];
+    __shared__ __attribute__((aligned(16))) unsigned lm0[MC];
+    __shared__ __attribute__((aligned(16))) unsigned lm1[MC];
+    __shared__ float yl[ROWS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int b = blockIdx.x;
+    if (b >= B) return;  // whole workgroup
+    const unsigned* P32 = (const unsigned*)P;
+    for (int i = tid; i < ROWS; i += 256) yl[i] = 0.f;
+    int bc[8], bn[8], bnn[8];
+    ls_block_info(blocks, b, bc);
+    {
+        const LsRunsRaw rc = ls_runs(runs, bc[7], S, wave, lane);
+        ls_issue<PD>(bc, S, rc, map, P32, (unsigned*)pb0, lm0, wave, lane);
+    }
+    int b1 = b + (int)gridDim.x;
+    LsRunsRaw rn{{0, 0}, {0, 0}};
+    if (b1 < B) {
+        ls_block_info(blocks, b1, bn);
+        rn = ls_runs(runs, bn[7], S, wave, lane);
+    }
+    for (int it = 0;; ++it) {
+        __builtin_amdgcn_s_waitcnt(0);  // this wavefront's loads of the current block (and the next run table) landed
+        __syncthreads();
+        const bool more = b1 < B;
+        const int b2 = b1 + (int)gridDim.x;
+        const bool more2 = b2 < B;
+        if (more2) ls_block_info(blocks, b2, bnn);
+        LsRunsRaw rnn{{0, 0}, {0, 0}};
+        const bool odd = it & 1;
+        ls_step<COMP>(bc, bn, more, bnn, more2, S, rn, rnn, runs, map, P32, odd ? pb1 : pb0, odd ? lm1 : lm0,
+                      (unsigned*)(odd ? pb0 : pb1), odd ? lm0 : lm1, yl, wave, lane);
+        // each wavefront writes y for its own rows (only it wrote their sums: no barrier, so the next block's
+        // direct loads stay in flight meanwhile)
+        {
+            const int r0 = wave == 0 ? 0 : bc[3 + wave] >> 16, r1 = wave == 3 ? bc[1] : bc[4 + wave] >> 16;
+            for (int i = r0 + lane; i < r1; i += 64) {
+                store_y(y, (long)bc[0] + i, yl[i], alpha, beta);
+                yl[i] = 0.f;  // for the next block (the barrier at the top orders this before its sums)
+            }
+        }
+        if (!more) break;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            bc[k] = bn[k];
+            bn[k] = bnn[k];
+        }
+        rn = rnn;
+        b1 = b2;
+    }
+}
+
+// One block per workgroup, one LDS buffer: small blocks, so that many workgroups share a CU and hide each
+// other's load latency (run table, then the map entries and pair sums).
+template <bool COMP, int PAIR_BYTES, int ROWS>
+__global__ __launch_bounds__(256) void lslab_combine1_kernel(const int* __restrict__ blocks, int S,
+                                                             const int2* __restrict__ runs,
+                                                             const unsigned* __restrict__ map,
+                                                             const LsPair<COMP>* __restrict__ P, float* __restrict__ y,
+                                                             float alpha, float beta) {
+    using PT = LsPair<COMP>;
+    constexpr int PD = (int)sizeof(PT) / 4;
+    constexpr int CAP = PAIR_BYTES / (int)sizeof(PT);
+    constexpr int MC = PAIR_BYTES / 4 + 8;
+    __shared__ __attribute__((aligned(16))) PT pb[CAP];
+    __shared__ __attribute__((aligned(16))) unsigned lm[MC];
+    __shared__ float yl[ROWS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int bi[8];
+    ls_block_info(blocks, blockIdx.x, bi);
+    const int r0 = wave == 0 ? 0 : bi[3 + wave] >> 16, r1 = wave == 3 ? bi[1] : bi[4 + wave] >> 16;
+    {
+        const LsRunsRaw r = ls_runs(runs, bi[7], S, wave, lane);
+        ls_issue<PD>(bi, S, r, map, (const unsigned*)P, (unsigned*)pb, lm, wave, lane);
+    }
+    for (int i = r0 + lane; i < r1; i += 64) yl[i] = 0.f;  // this wavefront's rows
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    ls_rows<COMP>(bi, pb, lm, yl, wave, lane);
+    for (int i = r0 + lane; i < r1; i += 64) store_y(y, (long)bi[0] + i, yl[i], alpha, beta);
+}
+
+// values into stream order: out[i] = v[src[i]], 0 where src[i] < 0 (tile padding)
+template <typename VT>
+__global__ void lslab_gather_values_kernel(const int* __restrict__ src, long n, const VT* __restrict__ v,
+                                           VT* __restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = src[i];
+    VT z;
+    __builtin_memset(&z, 0, sizeof(VT));
+    out[i] = k >= 0 ? v[k] : z;
+}
+
+// ---------------------------------------------------------------- plan construction (host)
+spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t s) {
+    const int R = p->A.R, C = p->A.C, N = p->A.N;
+    const int epl = param0 & 0xff;
+    if (!(epl == 4 || epl == 8)) {
+        set_error("lslab: nonzeros per lane must be 4 or 8");
+        return SPMV_ERROR_INTERNAL;
+    }
+    const int W = (param0 & kLsHalfSlabs) ? kLsSlabCols / 2 : kLsSlabCols;  // columns per slab
+    const bool persist = (param0 & kLsPersistent) != 0;
+    const int pair_bytes = persist ? kLsPairBytes : kLsPairBytes1, rows_max = persist ? kLsRowsMax : kLsRowsMax1;
+    const int S = C > 0 ? (int)(((long)C + W - 1) / W) : 0;
+    if (S > kLsMaxSlabs) {
+        set_error("lslab: at most " + std::to_string(kLsMaxSlabs) + " column slabs (" +
+                  std::to_string((long)kLsMaxSlabs * W) + " columns)");
+        return SPMV_ERROR_INVALID_MATRIX;
+    }
+    const int TILE = 64 * epl;
+    std::vector<int> rp((size_t)R + 1), ci((size_t)N);
+    MUXI_HIP(hipMemcpyAsync(rp.data(), p->A.rp, sizeof(int) * ((size_t)R + 1), hipMemcpyDeviceToHost, s));
+    if (N > 0) MUXI_HIP(hipMemcpyAsync(ci.data(), p->A.ci, sizeof(int) * (size_t)N, hipMemcpyDeviceToHost, s));
+    MUXI_HIP(hipStreamSynchronize(s));
+    if (rp[0] != 0 || rp[R] != N) {
+        set_error("lslab: rowPtr[0] must be 0 and rowPtr[numRows] = nnz");
+        return SPMV_ERROR_INVALID_MATRIX;
+    }
+    // ---- the slab streams: slab, then row, then position in the row
+    std::vector<long> off((size_t)S + 1, 0);
+    for (int r = 0; r < R; ++r) {
+        if (rp[r + 1] < rp[r]) {
+            set_error("lslab: rowPtr decreases");
+            return SPMV_ERROR_INVALID_MATRIX;
+        }
+        for (int k = rp[r]; k < rp[r + 1]; ++k) {
+            const int c = ci[k];
+            if (c < 0 || c >= C) {
+                set_error("lslab: column index out of range");
+                return SPMV_ERROR_INVALID_MATRIX;
+            }
+            ++off[(size_t)(c / W) + 1];
+        }
+    }
+    for (int q = 0; q < S; ++q) off[q + 1] += off[q];
+    std::vector<int> src((size_t)N), srow((size_t)N);
+    std::vector<uint16_t> scol((size_t)N);
+    {
+        std::vector<long> fill(off.begin(), off.end() - 1);
+        for (int r = 0; r < R; ++r)
+            for (int k = rp[r]; k < rp[r + 1]; ++k) {
+                const int q = ci[k] / W;
+                const long i = fill[q]++;
+                src[i] = k;
+                srow[i] = r;
+                scol[i] = (uint16_t)(ci[k] - q * W);
+            }
+    }
+    // row ends inside each slab; pair j (a (row, slab) with nonzeros) is closed by the j-th end in stream order
+    std::vector<unsigned char> end((size_t)N);
+    std::vector<int> ends_before((size_t)N + 1);
+    long npairs = 0;
+    for (int q = 0; q < S; ++q)
+        for (long i = off[q]; i < off[q + 1]; ++i) {
+            ends_before[i] = (int)npairs;
+            end[i] = (i + 1 == off[q + 1] || srow[i + 1] != srow[i]) ? 1 : 0;
+            npairs += end[i];
+        }
+    ends_before[N] = (int)npairs;
+    if (npairs >= INT_MAX / 2) {
+        set_error("lslab: too many (row, slab) pairs");
+        return SPMV_ERROR_INVALID_MATRIX;
+    }
+    std::vector<int> pair_row((size_t)npairs), pair_slab((size_t)npairs);
+    for (int q = 0; q < S; ++q)
+        for (long i = off[q]; i < off[q + 1]; ++i)
+            if (end[i]) {
+                pair_row[ends_before[i]] = srow[i];
+                pair_slab[ends_before[i]] = q;
+            }
+    std::vector<int> pslab((size_t)S + 1, 0);  // first pair of each slab
+    for (int q = 0; q < S; ++q) pslab[q + 1] = ends_before[off[q + 1]];
+
+    // ---- combine blocks: rows cut so that a block holds at most kLsRowsMax rows and its pair sums (and map
+    // entries) fit the LDS; a block's pairs take P[rstart[first row] ..) in (slab, row) order
+    const int cap = pair_bytes / (p->comp ? 8 : 4);
+    std::vector<int> row_pairs((size_t)R + 1, 0);
+    for (long j = 0; j < npairs; ++j) ++row_pairs[pair_row[j]];
+    std::vector<int> rstart((size_t)R + 1, 0);
+    for (int r = 0; r < R; ++r) rstart[r + 1] = rstart[r] + row_pairs[r];
+    std::vector<int> rlist((size_t)std::max(npairs, 1L));
+    {
+        std::vector<int> fill(rstart.begin(), rstart.end() - 1);
+        for (long j = 0; j < npairs; ++j) rlist[fill[pair_row[j]]++] = (int)j;  // slab order within a row
+    }
+    std::vector<int> brow;  // first row of each block, then R
+    for (int r = 0; r < R;) {
+        brow.push_back(r);
+        int np = 0, nr = 0;
+        while (r < R && nr < rows_max && (nr == 0 || np + row_pairs[r] <= cap)) {  // a row has <= S <= cap
+            np += row_pairs[r];
+            ++nr;
+            ++r;
+        }
+    }
+    const int B = (int)brow.size();
+    brow.push_back(R);
+    // F[b][s]: the first pair of slab s with row >= brow[b]; P offset of (block b, slab s):
+    // delta = rstart[brow[b]] + (pairs of the block in slabs < s) - F[b][s]
+    std::vector<int> F((size_t)(B + 1) * std::max(S, 1), 0);
+    for (int q = 0; q < S; ++q) {
+        int j = pslab[q];
+        for (int b = 0; b <= B; ++b) {
+            while (j < pslab[q + 1] && pair_row[j] < brow[b]) ++j;
+            F[(size_t)b * S + q] = j;
+        }
+    }
+    std::vector<int> hrun((size_t)2 * std::max(B, 1) * (S + 1), 0);  // per block: (first pair, LDS offset) per slab
+    std::vector<unsigned> hmap((size_t)std::max(npairs, 1L));
+    std::vector<int> hblk((size_t)8 * std::max(B, 1), 0);
+    {
+        std::vector<int> L((size_t)S + 1);
+        for (int b = 0; b < B; ++b) {
+            const int r0 = brow[b], r1 = brow[b + 1];
+            const int m0 = rstart[r0], np = rstart[r1] - m0;
+            int* rt = &hrun[(size_t)2 * b * (S + 1)];
+            L[0] = 0;
+            for (int q = 0; q < S; ++q) {
+                L[q + 1] = L[q] + F[(size_t)(b + 1) * S + q] - F[(size_t)b * S + q];
+                rt[2 * q] = F[(size_t)b * S + q];
+                rt[2 * q + 1] = L[q];
+            }
+            rt[2 * S] = 0;
+            rt[2 * S + 1] = np;  // = L[S]
+            for (int r = r0; r < r1; ++r)
+                for (int k = rstart[r]; k < rstart[r + 1]; ++k) {
+                    const int j = rlist[k], q = pair_slab[j];
+                    const unsigned idx = (unsigned)(L[q] + j - F[(size_t)b * S + q]);
+                    hmap[k] = idx | ((unsigned)(r - r0) << 16) | (k + 1 == rstart[r + 1] ? 0x80000000u : 0u);
+                }
+            int* d = &hblk[(size_t)8 * b];
+            d[0] = r0;
+            d[1] = r1 - r0;
+            d[2] = m0;
+            d[3] = np;
+            // wave splits at row boundaries, near w * np / 4: (first map entry | first row << 16) of wavefront w
+            int r = r0;
+            for (int w = 1; w < 4; ++w) {
+                const long want = (long)np * w / 4;
+                while (r < r1 && rstart[r] - m0 < want) ++r;
+                d[3 + w] = (rstart[r] - m0) | ((r - r0) << 16);
+            }
+            d[7] = b * (S + 1);
+        }
+    }
+
+    // ---- chunks: each slab's stream cut at row ends into about equal parts, about one per CU in all
+    int dev = 0, cus = 256;
+    MUXI_HIP(hipGetDevice(&dev));
+    MUXI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const long target_chunks = std::max(1, cus);
+    // a cut goes at the first row end at or past the wanted size; a (row, slab) pair without repeated columns
+    // holds at most W nonzeros, so a chunk stays within kLsMaxTiles tiles
+    const long max_piece = (long)kLsMaxTiles * TILE - W;
+    struct Piece {
+        int slab;
+        long a, n;  // stream positions [a, a + n)
+    };
+    std::vector<Piece> pieces;
+    for (int q = 0; q < S; ++q) {
+        const long e = off[q + 1], nq = e - off[q];
+        if (nq == 0) continue;
+        const long k = std::max(1L, (long)std::llround((double)target_chunks * (double)nq / (double)std::max(N, 1)));
+        const long want = std::min((nq + k - 1) / k, max_piece);
+        long a = off[q];
+        while (a < e) {
+            long c = std::min(e, a + want) - 1;  // the cut goes after the first row end at or past c
+            while (!end[c]) ++c;
+            if (c + 1 - a > (long)kLsMaxTiles * TILE) {
+                set_error("lslab: a row holds more than " + std::to_string(W) +
+                          " nonzeros in one column slab (repeated columns)");
+                return SPMV_ERROR_INVALID_MATRIX;
+            }
+            pieces.push_back({q, a, c + 1 - a});
+            a = c + 1;
+        }
+    }
+    // launch order: slab q's chunks on XCD q mod 8 (workgroup b runs on XCD b mod 8), so a slab of x is read
+    // into one L2 and shared there
+    std::vector<std::vector<int>> per_xcd(8);
+    for (int i = 0; i < (int)pieces.size(); ++i) per_xcd[pieces[i].slab % 8].push_back(i);
+    size_t rounds = 0;
+    for (auto& v : per_xcd) rounds = std::max(rounds, v.size());
+    std::vector<int> order;
+    for (size_t r = 0; r < rounds; ++r)
+        for (int xc = 0; xc < 8; ++xc) order.push_back(r < per_xcd[xc].size() ? per_xcd[xc][r] : -1);
+    while (!order.empty() && order.back() < 0) order.pop_back();
+    // padded tile layout in piece order
+    std::vector<long> tile0(pieces.size());
+    long ntiles = 0;
+    for (size_t i = 0; i < pieces.size(); ++i) {
+        tile0[i] = ntiles;
+        ntiles += (pieces[i].n + TILE - 1) / TILE;
+    }
+    const long npad = ntiles * TILE;
+    if (ntiles >= INT_MAX / 2 || npad >= (long)INT_MAX) {
+        set_error("lslab: padded nonzeros exceed INT32_MAX");
+        return SPMV_ERROR_INVALID_MATRIX;
+    }
+    std::vector<uint16_t> hlc((size_t)std::max(npad, 1L), 0);
+    std::vector<int> hsrc((size_t)std::max(npad, 1L), -1), htile((size_t)std::max(ntiles, 1L), 0);
+    for (size_t i = 0; i < pieces.size(); ++i) {
+        const Piece& pc = pieces[i];
+        const long base = tile0[i] * TILE;
+        for (long e = 0; e < pc.n; ++e) {
+            const long si = pc.a + e;
+            hlc[base + e] = (uint16_t)(scol[si] | (end[si] << 15));
+            hsrc[base + e] = src[si];
+        }
+        const long nt = (pc.n + TILE - 1) / TILE;
+        for (long lt = 0; lt < nt; ++lt) {
+            const long si = pc.a + lt * TILE;
+            const bool mid = lt > 0 && !end[si - 1];
+            htile[tile0[i] + lt] = ends_before[si] | (mid ? (int)0x80000000u : 0);
+        }
+    }
+    std::vector<int> hch(4 * std::max<size_t>(order.size(), 1), 0);
+    for (size_t b = 0; b < order.size(); ++b) {
+        if (order[b] < 0) continue;
+        const size_t i = (size_t)order[b];
+        const Piece& pc = pieces[i];
+        hch[4 * b + 0] = (int)tile0[i];
+        hch[4 * b + 1] = (int)((pc.n + TILE - 1) / TILE);
+        hch[4 * b + 2] = pc.slab * W;
+    }
+    // ---- device arrays
+    p->ls_S = S;
+    p->ls_chunks = (int)order.size();
+    p->ls_blocks = B;
+    p->ls_grid = 2 * std::max(1, cus);  // two persistent combine workgroups per CU
+    p->ls_pairs = npairs;
+    const size_t vs = vsize(vt), ps = p->comp ? 8 : 4;
+    MUXI_HIP(hipMalloc(&p->d_ls_lc, sizeof(uint16_t) * hlc.size()));
+    MUXI_HIP(hipMalloc(&p->d_ls_lv, vs * hlc.size()));
+    MUXI_HIP(hipMalloc(&p->d_ls_tp, sizeof(int) * htile.size()));
+    MUXI_HIP(hipMalloc(&p->d_ls_ch, sizeof(int) * hch.size()));
+    MUXI_HIP(hipMalloc(&p->d_ls_blk, sizeof(int) * hblk.size()));
+    MUXI_HIP(hipMalloc(&p->d_ls_F, sizeof(int) * hrun.size()));
+    MUXI_HIP(hipMalloc(&p->d_ls_map, sizeof(unsigned) * hmap.size()));
+    MUXI_HIP(hipMalloc(&p->d_ls_P, ps * (size_t)std::max(npairs, 1L)));
+    MUXI_HIP(hipMemcpyAsync(p->d_ls_lc, hlc.data(), sizeof(uint16_t) * hlc.size(), hipMemcpyHostToDevice, s));
+    MUXI_HIP(hipMemcpyAsync(p->d_ls_tp, htile.data(), sizeof(int) * htile.size(), hipMemcpyHostToDevice, s));
+    MUXI_HIP(hipMemcpyAsync(p->d_ls_ch, hch.data(), sizeof(int) * hch.size(), hipMemcpyHostToDevice, s));
+    MUXI_HIP(hipMemcpyAsync(p->d_ls_blk, hblk.data(), sizeof(int) * hblk.size(), hipMemcpyHostToDevice, s));
+    MUXI_HIP(hipMemcpyAsync(p->d_ls_F, hrun.data(), sizeof(int) * hrun.size(), hipMemcpyHostToDevice, s));
+    MUXI_HIP(hipMemcpyAsync(p->d_ls_map, hmap.data(), sizeof(unsigned) * hmap.size(), hipMemcpyHostToDevice, s));
+    int* d_src = nullptr;
+    MUXI_HIP(hipMalloc(&d_src, sizeof(int) * hsrc.size()));
+    hipError_t e = hipMemcpyAsync(d_src, hsrc.data(), sizeof(int) * hsrc.size(), hipMemcpyHostToDevice, s);
+    spmv_status_t st = SPMV_SUCCESS;
+    if (e != hipSuccess) {
+        set_error(std::string("lslab: ") + hipGetErrorString(e));
+        st = SPMV_ERROR_DEVICE;
+    } else {
+        const long n = (long)hsrc.size();
+        const unsigned g = grid_for(n, 256);
+        if (vt == SPMV_TYPE_FLOAT)
+            hipLaunchKernelGGL(lslab_gather_values_kernel<float>, dim3(g), dim3(256), 0, s, d_src, n,
+                               (const float*)p->A.v, (float*)p->d_ls_lv);
+        else if (vt == SPMV_TYPE_BF16)
+            hipLaunchKernelGGL(lslab_gather_values_kernel<bf16_t>, dim3(g), dim3(256), 0, s, d_src, n,
+                               (const bf16_t*)p->A.v, (bf16_t*)p->d_ls_lv);
+        else
+            hipLaunchKernelGGL(lslab_gather_values_kernel<double>, dim3(g), dim3(256), 0, s, d_src, n,
+                               (const double*)p->A.v, (double*)p->d_ls_lv);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(s);  // the host arrays above go out of scope
+        if (e != hipSuccess) {
+            set_error(std::string("lslab: ") + hipGetErrorString(e));
+            st = SPMV_ERROR_DEVICE;
+        }
+    }
+    (void)hipFree(d_src);
+    p->scratch_bytes = (sizeof(uint16_t) + vs) * hlc.size() +
+                       sizeof(int) * (htile.size() + hch.size() + hblk.size() + hrun.size()) +
+                       sizeof(unsigned) * hmap.size() + ps * (size_t)std::max(npairs, 1L);
+    return st;
+}
+
+template <typename VT, bool COMP, int EPL, int W>
+static void launch_slab(const muxi_plan* p, const float* x, hipStream_t s) {
+    hipLaunchKernelGGL((lslab_slab_kernel<VT, COMP, EPL, W>), dim3(p->ls_chunks), dim3(kLsThreads), 0, s,
+                       (const int4*)p->d_ls_ch, p->d_ls_lc, (const VT*)p->d_ls_lv, p->d_ls_tp, x, p->A.C,
+                       (LsPair<COMP>*)p->d_ls_P);
+}
+
+template <typename VT, bool COMP>
+spmv_status_t launch_lslab(const muxi_plan* p, const float* x, float* y, float alpha, float beta, hipStream_t s,
+                           hipEvent_t ev_end) {
+    if (p->A.R <= 0) {
+        if (ev_end) MUXI_HIP(hipEventRecord(ev_end, s));
+        return SPMV_SUCCESS;
+    }
+    using PT = LsPair<COMP>;
+    const bool half = (p->param0 & kLsHalfSlabs) != 0, four = (p->param0 & 0xff) == 4;
+    if (p->ls_chunks > 0) {
+        if (half) {
+            if (four) launch_slab<VT, COMP, 4, kLsSlabCols / 2>(p, x, s);
+            else launch_slab<VT, COMP, 8, kLsSlabCols / 2>(p, x, s);
+        } else {
+            if (four) launch_slab<VT, COMP, 4, kLsSlabCols>(p, x, s);
+            else launch_slab<VT, COMP, 8, kLsSlabCols>(p, x, s);
+        }
+        MUXI_LAUNCH_CHECK();
+    }
+    if (ev_end) MUXI_HIP(hipEventRecord(ev_end, s));
+    if (p->ls_blocks > 0) {
+        if (p->param0 & kLsPersistent)
+            hipLaunchKernelGGL((lslab_combine_kernel<COMP, kLsPairBytes, kLsRowsMax>),
+                               dim3(std::min(p->ls_blocks, p->ls_grid)), dim3(256), 0, s, p->d_ls_blk, p->ls_blocks,
+                               p->ls_S, (const int2*)p->d_ls_F, p->d_ls_map, (const PT*)p->d_ls_P, y, alpha, beta);
+        else
+            hipLaunchKernelGGL((lslab_combine1_kernel<COMP, kLsPairBytes1, kLsRowsMax1>), dim3(p->ls_blocks),
+                               dim3(256), 0, s, p->d_ls_blk, p->ls_S, (const int2*)p->d_ls_F, p->d_ls_map,
+                               (const PT*)p->d_ls_P, y, alpha, beta);
+        MUXI_LAUNCH_CHECK();
+    }
+    return SPMV_SUCCESS;
+}
+
+template spmv_status_t launch_lslab<float, false>(const muxi_plan*, const float*, float*, float, float, hipStream_t,
+                                                  hipEvent_t);
+template spmv_status_t launch_lslab<float, true>(const muxi_plan*, const float*, float*, float, float, hipStream_t,
+                                                 hipEvent_t);
+template spmv_status_t launch_lslab<bf16_t, false>(const muxi_plan*, const float*, float*, float, float, hipStream_t,
+                                                   hipEvent_t);
+template spmv_status_t launch_lslab<bf16_t, true>(const muxi_plan*, const float*, float*, float, float, hipStream_t,
+                                                  hipEvent_t);
+template spmv_status_t launch_lslab<double, false>(const muxi_plan*, const float*, float*, float, float, hipStream_t,
+                                                   hipEvent_t);
+template spmv_status_t launch_lslab<double, true>(const muxi_plan*, const float*, float*, float, float, hipStream_t,
+                                                  hipEvent_t);
+
+}  // namespace muxi

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "muxi_spmv/muxi_spmv_tune.h"
@@ -28,6 +29,9 @@ static void plan_free(muxi_plan* p) {
     (void)hipFree(p->d_slab_pc);
     (void)hipFree(p->d_slab_ct);
     (void)hipFree(p->d_slab_heavy);
+    for (void* q : {(void*)p->d_ls_lc, p->d_ls_lv, (void*)p->d_ls_tp, (void*)p->d_ls_ch, (void*)p->d_ls_blk,
+                    (void*)p->d_ls_F, (void*)p->d_ls_map, p->d_ls_P})
+        (void)hipFree(q);
     delete p;
 }
 
@@ -207,6 +211,23 @@ static spmv_status_t plan_create_impl(const muxi_csr_t* Ain, spmv_type_t vt, spm
             }
             p->param0 = (S << 8) | ipt;
             if (R > 0) PLAN_TRY(xslab_build(p, S, ipt, vt, s));
+            break;
+        }
+        case MUXI_KERNEL_LSLAB: {
+            if (param0 <= 0) param0 = 4;
+            if (!((param0 & 0xff) == 4 || (param0 & 0xff) == 8) || (param0 & ~(0xff | kLsHalfSlabs | kLsPersistent))) {
+                set_error("lslab: param0 = nonzeros per lane (4 or 8) | 0x100 (half slabs) | 0x200 (persistent "
+                          "combine)");
+                plan_free(p);
+                return SPMV_ERROR_INTERNAL;
+            }
+            if (xt != SPMV_TYPE_FLOAT) {
+                set_error("lslab: x and y must be FLOAT (the slab of x in LDS is FP32)");
+                plan_free(p);
+                return SPMV_ERROR_UNSUPPORTED_TYPE;
+            }
+            p->param0 = param0;
+            if (R > 0) PLAN_TRY(lslab_build(p, param0, vt, s));
             break;
         }
         case MUXI_KERNEL_ELL: {
@@ -410,6 +431,13 @@ static spmv_status_t plan_exec(const muxi_plan* p, const void* x, void* y, doubl
                 } else {
                     return launch_xslab<VT, XT, AT>(p, xx, yy, a, b, s, ev1);
                 }
+            case MUXI_KERNEL_LSLAB:
+                if constexpr (std::is_same<XT, float>::value && std::is_same<AT, float>::value) {
+                    return launch_lslab<VT, C>(p, xx, yy, a, b, s, ev1);
+                } else {
+                    set_error("lslab: x and y must be FLOAT");
+                    return SPMV_ERROR_UNSUPPORTED_TYPE;
+                }
             case MUXI_KERNEL_ADAPTIVE:
                 return launch_adaptive<VT, XT, AT, C>(p->A, p->param0, p->nblocks, p->d_blocks, p->d_blk_chunk,
                                                       p->nlong, p->d_long_row, p->d_long_first, p->d_carry, xx, yy, a,
@@ -421,7 +449,7 @@ static spmv_status_t plan_exec(const muxi_plan* p, const void* x, void* y, doubl
     // single-kernel paths: the main kernel ends with the exec
     if (ev1 && !((p->kernel == MUXI_KERNEL_MERGE && p->A.R > 0) || (p->kernel == MUXI_KERNEL_CSR5 && p->T > 0) ||
                  (p->kernel == MUXI_KERNEL_ADAPTIVE && p->A.R > 0 && p->nblocks > 0) ||
-                 (p->kernel == MUXI_KERNEL_XSLAB && p->A.R > 0)))
+                 (p->kernel == MUXI_KERNEL_XSLAB && p->A.R > 0) || p->kernel == MUXI_KERNEL_LSLAB))
         MUXI_HIP(hipEventRecord(ev1, s));
     return SPMV_SUCCESS;
 }

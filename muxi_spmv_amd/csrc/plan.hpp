@@ -6,6 +6,18 @@
 
 namespace muxi {
 constexpr int kMaxSlabs = 16;
+// LDS column slabs (kernels_lslab.hip)
+constexpr int kLsSlabCols = 32768;   // columns per slab: 128 KiB of FP32 x in LDS, 15-bit offsets
+constexpr int kLsThreads = 1024;     // slab-kernel workgroup
+constexpr int kLsMaxTiles = 512;     // tiles per slab-kernel chunk (LDS head/tail sums)
+constexpr int kLsLdsBytes = 163840;  // LDS of one CU (MI355X)
+constexpr int kLsMaxSlabs = 256;     // slabs (8,388,608 columns)
+constexpr int kLsPairBytes = 16384;  // persistent combine: LDS for a row block's pair sums (and its map), x 2
+constexpr int kLsRowsMax = 2048;     // persistent combine: rows per block
+constexpr int kLsPairBytes1 = 8192;  // one-block combine: LDS for a row block's pair sums (and its map)
+constexpr int kLsRowsMax1 = 1024;    // one-block combine: rows per block
+constexpr int kLsHalfSlabs = 0x100;  // param0 bit: slabs of kLsSlabCols / 2 columns (two slab workgroups per CU)
+constexpr int kLsPersistent = 0x200; // param0 bit: persistent, double-buffered combine
 struct SlabArgs {
     int S;
     int rows[kMaxSlabs];       // compressed rows of each slab (none is empty)
@@ -68,6 +80,18 @@ struct muxi_plan {
     uint32_t* d_slab_flags = nullptr;
     int* d_slab_rpp = nullptr;   // row-major pair offsets [R+1]
     int* d_slab_ckey = nullptr;  // carry keys (pair ids) per tile
+    // LDS column slabs (lslab): slab streams (16-bit column offsets with row-end bits, values), tile
+    // descriptors, chunk descriptors (launch order), combine blocks, run tables, row-major pair map, pair sums
+    int ls_S = 0, ls_chunks = 0, ls_blocks = 0, ls_grid = 1;
+    long ls_pairs = 0;
+    uint16_t* d_ls_lc = nullptr;
+    void* d_ls_lv = nullptr;
+    int* d_ls_tp = nullptr;   // per tile: first pair | mid-pair bit
+    int* d_ls_ch = nullptr;   // per launch slot: tile0, tiles, first column, 0
+    int* d_ls_blk = nullptr;  // per combine block: first row, rows, first map entry, pairs, wave splits, runs
+    int* d_ls_F = nullptr;    // per combine block and slab: (first pair in P, LDS offset), then (0, pairs)
+    unsigned* d_ls_map = nullptr;
+    void* d_ls_P = nullptr;
 };
 
 namespace muxi {
@@ -126,6 +150,13 @@ spmv_status_t plan_create_rows(const muxi_csr_t* A, spmv_type_t vt, spmv_type_t 
 spmv_status_t xslab_build(muxi_plan* p, int S, int ipt, spmv_type_t vt, hipStream_t s);
 template <typename VT, typename XT, typename AT>
 spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT beta, hipStream_t s,
+                           hipEvent_t ev_end);
+
+// LDS column-slab plan and launcher (kernels_lslab.hip); param0 = nonzeros per lane (4 or 8) | kLsHalfSlabs |
+// kLsPersistent; FP32 x and y
+spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t s);
+template <typename VT, bool COMP>
+spmv_status_t launch_lslab(const muxi_plan* p, const float* x, float* y, float alpha, float beta, hipStream_t s,
                            hipEvent_t ev_end);
 
 }  // namespace muxi

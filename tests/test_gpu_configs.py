@@ -19,7 +19,7 @@ from gpu_util import assert_close_fp32, dev_csr
 pytestmark = pytest.mark.gpu
 
 KALL = [("scalar", 0), ("vector", 0), ("vwave", 8), ("adaptive", 1024), ("merge", 4), ("merge", 8), ("csr5", 256),
-        ("ell", 0), ("xslab", 1028), ("xslab", 0), ("auto", 0)]
+        ("ell", 0), ("xslab", 1028), ("xslab", 0), ("lslab", 8), ("auto", 0)]
 
 
 @pytest.fixture(scope="module")
@@ -64,7 +64,8 @@ def test_banded_4m_precision(banded_4m_rcm, vt):
     Av = A64.with_values(api.convert_values(A64.d_values, vt))
     y64v = api.oracle_f64(Av, x)  # the stored values
     errs = {}
-    for kernel, param, comp in (("merge", 4, False), ("merge", 4, True), ("xslab", 0, False), ("auto", 0, False)):
+    for kernel, param, comp in (("merge", 4, False), ("merge", 4, True), ("xslab", 0, False), ("lslab", 8, False),
+                                ("lslab", 8, True), ("auto", 0, False)):
         y = torch.full((A64.numRows,), float("nan"), device="cuda")
         api.Plan(Av, kernel, param, compensated=comp).exec(x, y)
         torch.cuda.synchronize()
@@ -73,6 +74,7 @@ def test_banded_4m_precision(banded_4m_rcm, vt):
         if vt == torch.bfloat16:
             assert _rel_l2(y, y64) <= 2.0 ** -8  # the storage rounding, not the kernel
     assert errs[("merge", True)] <= errs[("merge", False)] * 1.01
+    assert errs[("lslab", True)] <= errs[("lslab", False)] * 1.01
 
 
 @pytest.mark.parametrize("vt", [torch.float32, torch.bfloat16])

@@ -156,7 +156,7 @@ def test_powerlaw_2m_precision(powerlaw_2m, vt):
     y64, _ = oracle.spmv_f64(h.rowPtr, h.colIdx, h.values.astype(np.float64), x.astype(np.float64))
     errs = {}
     for kernel, param, comp in (("auto", 0, False), ("auto", 0, True), ("merge", 4, False), ("merge", 4, True),
-                                ("xslab", 1032, False)):
+                                ("xslab", 1032, False), ("lslab", 8, False), ("lslab", 8, True)):
         y = torch.full((h.numRows,), float("nan"), device="cuda")
         api.Plan(Av, kernel, param, compensated=comp).exec(xd, y)
         torch.cuda.synchronize()
@@ -165,7 +165,7 @@ def test_powerlaw_2m_precision(powerlaw_2m, vt):
         assert e <= 1e-6, (kernel, comp, e)
         if vt == torch.bfloat16:
             assert _rel_l2(yy, y64) <= 2.0 ** -8  # BF16 storage rounding dominates
-    for kernel in ("auto", "merge"):
+    for kernel in ("auto", "merge", "lslab"):
         assert errs[(kernel, True)] <= errs[(kernel, False)] * 1.01, (kernel, errs)
     print(f"powerlaw_2m {vt}: " + ", ".join(f"{k}{'+dot2' if c else ''} {v:.3g}" for (k, c), v in errs.items()),
           flush=True)
