@@ -229,8 +229,9 @@ typedef enum {
                                   1, 2, 4, 8 that keep a slab of x within 2 MiB; items 0 = 4 */
     MUXI_KERNEL_LSLAB = 9,     /* column slabs of 32768 columns with the slab of x in LDS, pair sums combined
                                   per row block in LDS; param0 = nonzeros per lane (4 or 8; 0 = 4)
-                                  | 0x100 (slabs of 16384 columns) | 0x200 (persistent combine);
-                                  FP32 x and y, at most 8,388,608 columns */
+                                  | 0x100 (slabs of 16384 columns; always with 8 nonzeros a lane or
+                                  compensated sums); FP32 x and y; at most 256 slabs (8,388,608
+                                  columns with full slabs, 4,194,304 with half ones) */
     MUXI_KERNEL_COUNT = 10
 } muxi_kernel_t;
 

@@ -75,30 +75,38 @@ template <typename AT> struct Acc<AT, false> {
     __device__ __forceinline__ Acc shfl(int src) const { Acc r; r.s = __shfl(s, src); return r; }
 };
 
-// Dot2: s + c carries the sum to about twice the working precision.
+// Dot2: s + c carries the sum to about twice the working precision. The arithmetic is written out here with
+// contraction off: a product fused into the following TwoSum addition would make TwoSum compute the error of a
+// different sum.
 template <> struct Acc<float, true> {
     float s, c;
     __device__ __forceinline__ void zero() { s = 0.f; c = 0.f; }
     __device__ __forceinline__ void two_sum_add(float a) {
-        float t = __fadd_rn(s, a);
-        float bp = __fsub_rn(t, s);
-        float e = __fadd_rn(__fsub_rn(s, __fsub_rn(t, bp)), __fsub_rn(a, bp));
+#pragma clang fp contract(off)
+        const float t = s + a;
+        const float bp = t - s;
+        const float e = (s - (t - bp)) + (a - bp);
         s = t;
-        c = __fadd_rn(c, e);
+        c = c + e;
     }
     template <typename VT, typename XT>
     __device__ __forceinline__ void fma(VT v, XT xv) {
-        float a = to_f32(v), b = to_f32(xv);
-        float p = __fmul_rn(a, b);
-        float ep = __fmaf_rn(a, b, -p);  // exact product error
+#pragma clang fp contract(off)
+        const float a = to_f32(v), b = to_f32(xv);
+        const float p = a * b;
+        const float ep = __builtin_fmaf(a, b, -p);  // exact product error
         two_sum_add(p);
-        c = __fadd_rn(c, ep);
+        c = c + ep;
     }
     __device__ __forceinline__ void add(const Acc& o) {
+#pragma clang fp contract(off)
         two_sum_add(o.s);
-        c = __fadd_rn(c, o.c);
+        c = c + o.c;
     }
-    __device__ __forceinline__ float value() const { return __fadd_rn(s, c); }
+    __device__ __forceinline__ float value() const {
+#pragma clang fp contract(off)
+        return s + c;
+    }
     __device__ __forceinline__ Acc shfl_xor(int m) const { Acc r; r.s = __shfl_xor(s, m); r.c = __shfl_xor(c, m); return r; }
     __device__ __forceinline__ Acc shfl_up(int d) const { Acc r; r.s = __shfl_up(s, d); r.c = __shfl_up(c, d); return r; }
     __device__ __forceinline__ Acc shfl(int src) const { Acc r; r.s = __shfl(s, src); r.c = __shfl(c, src); return r; }
@@ -117,16 +125,18 @@ template <> struct Prod<float, true> {
     float p, e;
     template <typename VT, typename XT>
     __device__ __forceinline__ static Prod make(VT v, XT xv) {
-        float a = to_f32(v), b = to_f32(xv);
+#pragma clang fp contract(off)
+        const float a = to_f32(v), b = to_f32(xv);
         Prod r;
-        r.p = __fmul_rn(a, b);
-        r.e = __fmaf_rn(a, b, -r.p);
+        r.p = a * b;
+        r.e = __builtin_fmaf(a, b, -r.p);
         return r;
     }
     __device__ __forceinline__ static Prod zero() { Prod r; r.p = 0.f; r.e = 0.f; return r; }
     __device__ __forceinline__ void into(Acc<float, true>& a) const {
+#pragma clang fp contract(off)
         a.two_sum_add(p);
-        a.c = __fadd_rn(a.c, e);
+        a.c = a.c + e;
     }
 };
 

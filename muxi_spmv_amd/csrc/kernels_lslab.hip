@@ -50,26 +50,33 @@ template <> struct LsPair<false> {
     __device__ __forceinline__ static LsPair prod(VT v, float xv) { LsPair r; r.s = to_f32(v) * xv; return r; }
     __device__ __forceinline__ float value() const { return s; }
 };
-// Dot2 (Ogita, Rump, Oishi 2005): TwoProduct by FMA, TwoSum, errors summed in c.
+// Dot2 (Ogita, Rump, Oishi 2005): TwoProduct by FMA, TwoSum, errors summed in c. The arithmetic is written out
+// here with contraction off: a product fused into the next sum's addition would leave TwoSum computing the error
+// of a different sum.
 template <> struct LsPair<true> {
     float s, c;
     __device__ __forceinline__ static LsPair zero() { LsPair r; r.s = 0.f; r.c = 0.f; return r; }
     __device__ __forceinline__ void add(const LsPair& o) {
-        const float t = __fadd_rn(s, o.s);
-        const float bp = __fsub_rn(t, s);
-        const float e = __fadd_rn(__fsub_rn(s, __fsub_rn(t, bp)), __fsub_rn(o.s, bp));
+#pragma clang fp contract(off)
+        const float t = s + o.s;
+        const float bp = t - s;
+        const float e = (s - (t - bp)) + (o.s - bp);
         s = t;
-        c = __fadd_rn(c, __fadd_rn(o.c, e));
+        c = c + (o.c + e);
     }
     template <typename VT>
     __device__ __forceinline__ static LsPair prod(VT v, float xv) {
+#pragma clang fp contract(off)
         const float a = to_f32(v);
         LsPair r;
-        r.s = __fmul_rn(a, xv);
-        r.c = __fmaf_rn(a, xv, -r.s);  // the product's exact rounding error
+        r.s = a * xv;
+        r.c = __builtin_fmaf(a, xv, -r.s);  // the product's exact rounding error
         return r;
     }
-    __device__ __forceinline__ float value() const { return __fadd_rn(s, c); }
+    __device__ __forceinline__ float value() const {
+#pragma clang fp contract(off)
+        return s + c;
+    }
 };
 
 // DPP cross-lane moves (GFX9 encodings: row_shr:n = 0x110 | n within a row of
@@ -166,26 +173,32 @@ struct LsLoad<bf16_t, N> {
 };
 
 // ---------------------------------------------------------------- the slab kernel
-// chunks[4b ..] = (tile0, ntiles, first column of the slab, 0); ntiles = 0 is an empty launch slot.
-// tile_pair[t] = index of the pair holding tile t's first element, bit 31 set when that pair started in an
-// earlier tile. The sum of pair j goes to P[j] (pairs in stream order: slab, then row).
+// chunks[4b ..] = (tile0, ntiles, first column of the slab, run table offset); ntiles = 0 is an empty launch
+// slot. tiles[t] = (index of the pair holding tile t's first element, bit 31 set when that pair started in an
+// earlier tile; the chunk run that pair falls in). Pairs are numbered in stream order (slab, then row); a
+// chunk's pairs split into runs by combine block, runs[off + r] = (first pair of run r, its offset) with a
+// sentinel first pair INT_MAX after the last run, and pair j of run r goes to P[j + offset]: every combine
+// block then finds its pair sums in one contiguous range of P.
 template <typename VT, bool COMP, int EPL, int W>
 __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __restrict__ chunks,
                                                                 const uint16_t* __restrict__ lc,
                                                                 const VT* __restrict__ lv,
-                                                                const int* __restrict__ tile_pair,
+                                                                const int2* __restrict__ tiles,
+                                                                const int2* __restrict__ runs,
                                                                 const float* __restrict__ x, int C,
                                                                 LsPair<COMP>* __restrict__ P) {
     using PT = LsPair<COMP>;
     constexpr int TILE = 64 * EPL;
     constexpr int NWAVE = kLsThreads / 64;
+    static_assert(W * 4 + kLsThreads * EPL * (int)sizeof(PT) + kLsMaxTiles * (2 * (int)sizeof(PT) + 1) +
+                          (kLsMaxRuns + 1) * 8 <= kLsLdsBytes,
+                  "slab kernel LDS");
     __shared__ float xs[W];
-    // per wavefront: one tile's pair sums, where the LDS holds them beside the slab of x
-    constexpr bool STAGE = !COMP && (W + kLsThreads * EPL) * 4 + kLsMaxTiles * 9 <= kLsLdsBytes;
-    __shared__ float stage[STAGE ? kLsThreads * EPL : 1];
+    __shared__ PT stage[kLsThreads * EPL];  // per wavefront: one tile's pair sums in pair order
     __shared__ PT head[kLsMaxTiles];
     __shared__ PT tail[kLsMaxTiles];
     __shared__ unsigned char tflag[kLsMaxTiles];  // bit 0: the tile has a row end; bit 1: starts mid-pair
+    __shared__ int2 lrun[kLsMaxRuns + 1];
     const int4 ch = chunks[blockIdx.x];
     const int ntiles = ch.y;
     if (ntiles == 0) return;  // whole workgroup, before any barrier
@@ -202,8 +215,11 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
     };
     uint16_t cn[EPL];
     VT vn[EPL];
-    // the slab of x into LDS: a whole slab with eight 16-byte loads per lane in flight together (then the first
-    // tile's loads join them), the last, partial slab one element at a time
+    // the chunk's run table: at most kLsMaxRuns runs and the sentinel (the table array ends with kLsMaxRuns + 1
+    // more sentinels, so these loads stay inside it for the last chunk too)
+    for (int i = tid; i <= kLsMaxRuns; i += kLsThreads) lrun[i] = runs[ch.w + i];
+    // the slab of x into LDS: a whole slab with 16-byte loads, all in flight together (then the first tile's
+    // loads join them), the last, partial slab one element at a time
     const float* xg = x + x0;
     if (width == W && (((uintptr_t)xg) & 15) == 0) {
         constexpr int K = W / (4 * kLsThreads);
@@ -219,12 +235,14 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
     }
     __syncthreads();
 
-    // one tile: products from LDS x, the lane's segmented sums, a DPP scan across lanes, pair sums to P
+    PT* st = stage + wave * TILE;
+    // one tile: products from LDS x, the lane's segmented sums, a DPP scan across lanes, the tile's pair sums
+    // through the LDS stage to P
     auto process = [&](int lt, const uint16_t* c, const VT* v) {
         const long t = (long)ch.x + lt;
-        const int tp = tile_pair[t];  // t is uniform: a scalar load
-        const bool mid = tp < 0;
-        const int pair0 = tp & 0x7fffffff;
+        const int2 td = tiles[t];  // t is uniform: scalar loads
+        const bool mid = td.x < 0;
+        const int pair0 = td.x & 0x7fffffff;
         PT p[EPL];
         unsigned f = 0;
 #pragma unroll
@@ -245,37 +263,39 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
         int fl = f != 0u;
         ls_seg_scan<COMP>(sc, fl);
         PT run = ls_dpp<0x138, 0xf>(sc);  // open sum entering the lane (0 for lane 0)
-        // the lane's k-th row end closes pair pair0 + before + k; the tile's first end, when its pair started in
-        // an earlier tile, is finished after the barrier
-        if constexpr (STAGE) {
-            // the tile's pair sums go to the wavefront's LDS stage in pair order, then to P with contiguous,
-            // whole-wavefront stores (scattered per-lane stores cost more than everything else here)
-            float* st = stage + wave * (64 * EPL);
+        // the lane's k-th row end closes the tile's pair before + k (pair pair0 + before + k)
 #pragma unroll
-            for (int j = 0; j < EPL; ++j) {
-                run.add(p[j]);
-                if ((f >> j) & 1u) {
-                    st[before + __popc(f & ((1u << j) - 1u))] = run.value();
-                    run = PT::zero();
-                }
-            }
-            const int first = mid ? 1 : 0;
-            for (int i = first + lane; i < total; i += 64) P[pair0 + i].s = st[i];
-            if (mid && lane == 0 && total > 0) head[lt].s = st[0];
-        } else {
-            PT* out = P + pair0 + before;
-            const bool skip_first = mid && before == 0;
-#pragma unroll
-            for (int j = 0; j < EPL; ++j) {
-                run.add(p[j]);
-                if ((f >> j) & 1u) {
-                    const int k = __popc(f & ((1u << j) - 1u));
-                    if (k == 0 && skip_first) head[lt] = run;
-                    else out[k] = run;
-                    run = PT::zero();
-                }
+        for (int j = 0; j < EPL; ++j) {
+            run.add(p[j]);
+            if ((f >> j) & 1u) {
+                st[before + __popc(f & ((1u << j) - 1u))] = run;
+                run = PT::zero();
             }
         }
+        // the tile's pairs to P with whole-wavefront stores of consecutive pairs; the tile's first pair, when it
+        // started in an earlier tile, is finished after the barrier. Runs are taken eight at a time: lanes 0-8
+        // read run r .. r + 8 in one LDS access, and a lane picks its pair's offset from those by comparisons.
+        int r = td.y, j = pair0 + (mid ? 1 : 0);
+        const int jend = pair0 + total;
+        while (j < jend) {
+            const int2 rr = lrun[min(r + min(lane, 8), kLsMaxRuns)];
+            int bnd[8], off[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                bnd[i] = __builtin_amdgcn_readlane(rr.x, i + 1);  // first pair after run r + i
+                off[i] = __builtin_amdgcn_readlane(rr.y, i);
+            }
+            const int je = min(jend, bnd[7]);
+            for (int q = j + lane; q < je; q += 64) {
+                int d = off[0];
+#pragma unroll
+                for (int i = 1; i < 8; ++i) d = q >= bnd[i - 1] ? off[i] : d;
+                P[q + d] = st[q - pair0];
+            }
+            j = je;
+            r += 8;
+        }
+        if (mid && lane == 0 && total > 0) head[lt] = st[0];
         const PT open = ls_readlane63<COMP>(sc);  // the sum open at the tile end (0 after a final row end)
         if (lane == 0) {
             tail[lt] = open;
@@ -303,63 +323,17 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
             s.add(tail[q]);
             if ((tflag[q] & 1) || q == 0) break;
         }
-        P[tile_pair[ch.x + lt] & 0x7fffffff] = s;
+        const int2 td = tiles[(long)ch.x + lt];
+        P[(td.x & 0x7fffffff) + lrun[td.y].y] = s;
     }
 }
 
 // ---------------------------------------------------------------- the combine kernel
-// blocks[8b ..] = (first row, rows, first map entry, pairs, wave split 1, 2, 3, run table offset); wave split w =
-// (first map entry | first row << 16) of wavefront w, at a row boundary (split 0 = (0, 0), split 4 = (pairs,
-// rows)). The block's
-// pairs, slab by slab, form S runs of P (pairs in stream order): runs[off + s] = (first pair of the run in P,
-// its offset in the block's LDS copy), runs[off + S] = (0, pairs). map[first .. first + pairs) lists the block's
-// pairs in row-major order: LDS index (bits 0-15), row offset in the block (bits 16-30), bit 31 on the row's
-// last pair. Wavefront w sums the rows of its map entries and writes their y.
-
-// A wavefront's share of one block's run table, as loaded: lane l holds runs[off + wave + 4 l] and the entry
-// after it (its LDS end). Nothing reads these until the next block starts, so the loads stay in flight.
-struct LsRunsRaw {
-    int2 d0, d1;
-};
-
-__device__ __forceinline__ void ls_block_info(const int* __restrict__ blocks, int b, int* bi) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) bi[k] = blocks[8 * (long)b + k];  // uniform: scalar loads
-}
-
-__device__ __forceinline__ LsRunsRaw ls_runs(const int2* __restrict__ runs, int roff, int S, int wave, int lane) {
-    LsRunsRaw r{{0, 0}, {0, 0}};
-    if (lane < (S - wave + 3) / 4) {
-        const int q = wave + 4 * lane;
-        r.d0 = runs[roff + q];
-        r.d1 = runs[roff + q + 1];
-    }
-    return r;
-}
-
-// Starts the direct global->LDS loads of one block's map entries and pair sums (nothing waits here).
-template <int PD>
-__device__ __forceinline__ void ls_issue(const int* bi, int S, const LsRunsRaw& r, const unsigned* __restrict__ map,
-                                         const unsigned* __restrict__ P, unsigned* __restrict__ pb,
-                                         unsigned* __restrict__ lmap, int wave, int lane) {
-    const int mbase = bi[2], np = bi[3];
-    for (int e0 = wave * 64; e0 < np; e0 += 256)
-        if (e0 + lane < np)
-            __builtin_amdgcn_global_load_lds((const void*)(map + mbase + e0 + lane),
-                                             (__attribute__((address_space(3))) void*)(lmap + e0), 4, 0, 0);
-    const int nr = (S - wave + 3) / 4;
-    const int len = r.d1.y - r.d0.y;
-    for (int l = 0; l < nr; ++l) {
-        const int src = __builtin_amdgcn_readlane(r.d0.x, l), dst = __builtin_amdgcn_readlane(r.d0.y, l);
-        const int n = __builtin_amdgcn_readlane(len, l) * PD;
-        const unsigned* g = P + (long)src * PD;
-        unsigned* d = pb + (long)dst * PD;
-        for (int e0 = 0; e0 < n; e0 += 64)
-            if (e0 + lane < n)
-                __builtin_amdgcn_global_load_lds((const void*)(g + e0 + lane),
-                                                 (__attribute__((address_space(3))) void*)(d + e0), 4, 0, 0);
-    }
-}
+// blocks[8b ..] = (first row, rows, first pair, pairs, wave split 1, 2, 3, 0); wave split w = (first pair |
+// first row << 16) of wavefront w, at a row boundary (split 0 = (0, 0), split 4 = (pairs, rows)). The block's
+// pair sums are P[first .. first + pairs) (slab by slab, rows ascending inside a slab) and its map entries
+// map[first .. first + pairs) list them in row-major order: LDS index (bits 0-15), row offset in the block
+// (bits 16-30), bit 31 on the row's last pair. Wavefront w sums the rows of its map entries and writes their y.
 
 // A wavefront sums its rows of one block from LDS (its map range starts and ends on row boundaries): rounds of
 // 64 * E entries start on a multiple of E, a lane takes E consecutive entries (two 16-byte LDS reads), and
@@ -420,112 +394,45 @@ __device__ __forceinline__ void ls_rows(const int* bi, const LsPair<COMP>* __res
     }
 }
 
-// One step of the combine: start the loads of the next block into one pair of LDS buffers, the run tables of
-// the block after it into registers, then sum the current block from the other pair. All buffers are
-// distinct (restrict), so the sums read LDS without waiting for the loads in flight.
-template <bool COMP>
-__device__ __forceinline__ void ls_step(const int* bc, const int* bn, bool more, const int* bnn, bool more2, int S,
-                                        const LsRunsRaw& rn, LsRunsRaw& rnn, const int2* __restrict__ runs,
-                                        const unsigned* __restrict__ map, const unsigned* __restrict__ P,
-                                        const LsPair<COMP>* __restrict__ cur_pb, const unsigned* __restrict__ cur_lm,
-                                        unsigned* __restrict__ nxt_pb, unsigned* __restrict__ nxt_lm,
-                                        float* __restrict__ yl, int wave, int lane) {
-    constexpr int PD = (int)sizeof(LsPair<COMP>) / 4;
-    if (more) ls_issue<PD>(bn, S, rn, map, P, nxt_pb, nxt_lm, wave, lane);
-    if (more2) rnn = ls_runs(runs, bnn[7], S, wave, lane);
-    ls_rows<COMP>(bc, cur_pb, cur_lm, yl, wave, lane);
+__device__ __forceinline__ void ls_block_info(const int* __restrict__ blocks, int b, int* bi) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) bi[k] = blocks[8 * (long)b + k];  // uniform: scalar loads
 }
 
-// Persistent: workgroup g takes blocks g, g + grid, ... with two LDS buffers. While it sums one block from LDS,
-// the direct loads of its next block are in flight, and the run table of the block after that is loading.
+// Copies n dwords from global src to LDS dst with the wavefront's direct global->LDS loads (no register round
+// trip; nothing waits here). The workgroup's four wavefronts take every fourth 64-dword piece.
+__device__ __forceinline__ void ls_copy_lds(const unsigned* __restrict__ src, unsigned* __restrict__ dst, int n,
+                                            int wave, int lane) {
+    for (int e0 = wave * 64; e0 < n; e0 += 256)
+        if (e0 + lane < n)
+            __builtin_amdgcn_global_load_lds((const void*)(src + e0 + lane),
+                                             (__attribute__((address_space(3))) void*)(dst + e0), 4, 0, 0);
+}
+
+// One block per workgroup: the block's map entries and pair sums (two contiguous ranges) go to LDS by direct
+// loads, all in flight before anything waits; the rows are then summed from LDS alone and each wavefront writes
+// y for its own rows. Small blocks let many workgroups share a CU and hide each other's load latency.
 template <bool COMP, int PAIR_BYTES, int ROWS>
-__global__ __launch_bounds__(256) void lslab_combine_kernel(const int* __restrict__ blocks, int B, int S,
-                                                            const int2* __restrict__ runs,
+__global__ __launch_bounds__(256) void lslab_combine_kernel(const int* __restrict__ blocks,
                                                             const unsigned* __restrict__ map,
                                                             const LsPair<COMP>* __restrict__ P, float* __restrict__ y,
                                                             float alpha, float beta) {
     using PT = LsPair<COMP>;
-    constexpr int PD = (int)sizeof(PT) / 4;  // dwords per pair sum
-    constexpr int CAP = PAIR_BYTES / (int)sizeof(PT);
-    constexpr int MC = PAIR_BYTES / 4 + 8;  // map entries (+ the tail of a lane's last 16-byte reads)
-    __shared__ __attribute__((aligned(16))) PT pb0[CAP];
-    __shared__ __attribute__((aligned(16))) PT pb1[CAP];
-    __shared__ __attribute__((aligned(16))) unsigned lm0[MC];
-    __shared__ __attribute__((aligned(16))) unsigned lm1[MC];
-    __shared__ float yl[ROWS];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int b = blockIdx.x;
-    if (b >= B) return;  // whole workgroup
-    const unsigned* P32 = (const unsigned*)P;
-    for (int i = tid; i < ROWS; i += 256) yl[i] = 0.f;
-    int bc[8], bn[8], bnn[8];
-    ls_block_info(blocks, b, bc);
-    {
-        const LsRunsRaw rc = ls_runs(runs, bc[7], S, wave, lane);
-        ls_issue<PD>(bc, S, rc, map, P32, (unsigned*)pb0, lm0, wave, lane);
-    }
-    int b1 = b + (int)gridDim.x;
-    LsRunsRaw rn{{0, 0}, {0, 0}};
-    if (b1 < B) {
-        ls_block_info(blocks, b1, bn);
-        rn = ls_runs(runs, bn[7], S, wave, lane);
-    }
-    for (int it = 0;; ++it) {
-        __builtin_amdgcn_s_waitcnt(0);  // this wavefront's loads of the current block (and the next run table) landed
-        __syncthreads();
-        const bool more = b1 < B;
-        const int b2 = b1 + (int)gridDim.x;
-        const bool more2 = b2 < B;
-        if (more2) ls_block_info(blocks, b2, bnn);
-        LsRunsRaw rnn{{0, 0}, {0, 0}};
-        const bool odd = it & 1;
-        ls_step<COMP>(bc, bn, more, bnn, more2, S, rn, rnn, runs, map, P32, odd ? pb1 : pb0, odd ? lm1 : lm0,
-                      (unsigned*)(odd ? pb0 : pb1), odd ? lm0 : lm1, yl, wave, lane);
-        // each wavefront writes y for its own rows (only it wrote their sums: no barrier, so the next block's
-        // direct loads stay in flight meanwhile)
-        {
-            const int r0 = wave == 0 ? 0 : bc[3 + wave] >> 16, r1 = wave == 3 ? bc[1] : bc[4 + wave] >> 16;
-            for (int i = r0 + lane; i < r1; i += 64) {
-                store_y(y, (long)bc[0] + i, yl[i], alpha, beta);
-                yl[i] = 0.f;  // for the next block (the barrier at the top orders this before its sums)
-            }
-        }
-        if (!more) break;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            bc[k] = bn[k];
-            bn[k] = bnn[k];
-        }
-        rn = rnn;
-        b1 = b2;
-    }
-}
-
-// One block per workgroup, one LDS buffer: small blocks, so that many workgroups share a CU and hide each
-// other's load latency (run table, then the map entries and pair sums).
-template <bool COMP, int PAIR_BYTES, int ROWS>
-__global__ __launch_bounds__(256) void lslab_combine1_kernel(const int* __restrict__ blocks, int S,
-                                                             const int2* __restrict__ runs,
-                                                             const unsigned* __restrict__ map,
-                                                             const LsPair<COMP>* __restrict__ P, float* __restrict__ y,
-                                                             float alpha, float beta) {
-    using PT = LsPair<COMP>;
     constexpr int PD = (int)sizeof(PT) / 4;
     constexpr int CAP = PAIR_BYTES / (int)sizeof(PT);
-    constexpr int MC = PAIR_BYTES / 4 + 8;
+    constexpr int MC = PAIR_BYTES / 4 + 8;  // map entries (+ the tail of a lane's last 16-byte reads)
     __shared__ __attribute__((aligned(16))) PT pb[CAP];
     __shared__ __attribute__((aligned(16))) unsigned lm[MC];
     __shared__ float yl[ROWS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int bi[8];
     ls_block_info(blocks, blockIdx.x, bi);
+    const int base = bi[2], np = bi[3];
+    ls_copy_lds(map + base, lm, np, wave, lane);
+    ls_copy_lds((const unsigned*)P + (long)base * PD, (unsigned*)pb, np * PD, wave, lane);
     const int r0 = wave == 0 ? 0 : bi[3 + wave] >> 16, r1 = wave == 3 ? bi[1] : bi[4 + wave] >> 16;
-    {
-        const LsRunsRaw r = ls_runs(runs, bi[7], S, wave, lane);
-        ls_issue<PD>(bi, S, r, map, (const unsigned*)P, (unsigned*)pb, lm, wave, lane);
-    }
     for (int i = r0 + lane; i < r1; i += 64) yl[i] = 0.f;  // this wavefront's rows
-    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_s_waitcnt(0);                      // this wavefront's direct loads have landed
     __syncthreads();
     ls_rows<COMP>(bi, pb, lm, yl, wave, lane);
     for (int i = r0 + lane; i < r1; i += 64) store_y(y, (long)bi[0] + i, yl[i], alpha, beta);
@@ -551,9 +458,8 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
         set_error("lslab: nonzeros per lane must be 4 or 8");
         return SPMV_ERROR_INTERNAL;
     }
-    const int W = (param0 & kLsHalfSlabs) ? kLsSlabCols / 2 : kLsSlabCols;  // columns per slab
-    const bool persist = (param0 & kLsPersistent) != 0;
-    const int pair_bytes = persist ? kLsPairBytes : kLsPairBytes1, rows_max = persist ? kLsRowsMax : kLsRowsMax1;
+    const int W = lslab_slab_cols(param0, p->comp);  // columns per slab
+    const int pair_bytes = kLsPairBytes, rows_max = kLsRowsMax;
     const int S = C > 0 ? (int)(((long)C + W - 1) / W) : 0;
     if (S > kLsMaxSlabs) {
         set_error("lslab: at most " + std::to_string(kLsMaxSlabs) + " column slabs (" +
@@ -637,19 +543,22 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
         for (long j = 0; j < npairs; ++j) rlist[fill[pair_row[j]]++] = (int)j;  // slab order within a row
     }
     std::vector<int> brow;  // first row of each block, then R
+    std::vector<int> row_block((size_t)std::max(R, 1));
     for (int r = 0; r < R;) {
         brow.push_back(r);
         int np = 0, nr = 0;
         while (r < R && nr < rows_max && (nr == 0 || np + row_pairs[r] <= cap)) {  // a row has <= S <= cap
             np += row_pairs[r];
+            row_block[r] = (int)brow.size() - 1;
             ++nr;
             ++r;
         }
     }
     const int B = (int)brow.size();
     brow.push_back(R);
-    // F[b][s]: the first pair of slab s with row >= brow[b]; P offset of (block b, slab s):
-    // delta = rstart[brow[b]] + (pairs of the block in slabs < s) - F[b][s]
+    // F[b][s]: the first pair of slab s with row >= brow[b]. Block b's pairs take P[rstart[brow[b]] ..) slab by
+    // slab, so pair j of slab s goes to j + delta[b][s], delta = rstart[brow[b]] + (block pairs in slabs < s) -
+    // F[b][s]; its map entry holds its LDS index (the same offset inside the block)
     std::vector<int> F((size_t)(B + 1) * std::max(S, 1), 0);
     for (int q = 0; q < S; ++q) {
         int j = pslab[q];
@@ -658,23 +567,27 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
             F[(size_t)b * S + q] = j;
         }
     }
-    std::vector<int> hrun((size_t)2 * std::max(B, 1) * (S + 1), 0);  // per block: (first pair, LDS offset) per slab
+    // Inside a block the slabs' runs are ordered by (slab mod 8, slab): the chunks of slab q run on XCD q mod 8
+    // (see the launch order below), so each XCD writes one contiguous part of every block and no cache line of
+    // P is written from two L2s.
+    std::vector<int> xorder((size_t)S);
+    for (int i = 0, xc = 0; xc < 8; ++xc)
+        for (int q = xc; q < S; q += 8) xorder[i++] = q;
+    std::vector<int> delta((size_t)std::max(B, 1) * std::max(S, 1), 0);
     std::vector<unsigned> hmap((size_t)std::max(npairs, 1L));
     std::vector<int> hblk((size_t)8 * std::max(B, 1), 0);
     {
-        std::vector<int> L((size_t)S + 1);
+        std::vector<int> L((size_t)S + 1);  // L[q]: offset of slab q's run in the block
         for (int b = 0; b < B; ++b) {
             const int r0 = brow[b], r1 = brow[b + 1];
             const int m0 = rstart[r0], np = rstart[r1] - m0;
-            int* rt = &hrun[(size_t)2 * b * (S + 1)];
-            L[0] = 0;
-            for (int q = 0; q < S; ++q) {
-                L[q + 1] = L[q] + F[(size_t)(b + 1) * S + q] - F[(size_t)b * S + q];
-                rt[2 * q] = F[(size_t)b * S + q];
-                rt[2 * q + 1] = L[q];
+            int acc = 0;
+            for (int i = 0; i < S; ++i) {
+                const int q = xorder[i];
+                L[q] = acc;
+                acc += F[(size_t)(b + 1) * S + q] - F[(size_t)b * S + q];
+                delta[(size_t)b * S + q] = m0 + L[q] - F[(size_t)b * S + q];
             }
-            rt[2 * S] = 0;
-            rt[2 * S + 1] = np;  // = L[S]
             for (int r = r0; r < r1; ++r)
                 for (int k = rstart[r]; k < rstart[r + 1]; ++k) {
                     const int j = rlist[k], q = pair_slab[j];
@@ -686,18 +599,19 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
             d[1] = r1 - r0;
             d[2] = m0;
             d[3] = np;
-            // wave splits at row boundaries, near w * np / 4: (first map entry | first row << 16) of wavefront w
+            // wave splits at row boundaries, near w * np / 4: (first pair | first row << 16) of wavefront w
             int r = r0;
             for (int w = 1; w < 4; ++w) {
                 const long want = (long)np * w / 4;
                 while (r < r1 && rstart[r] - m0 < want) ++r;
                 d[3 + w] = (rstart[r] - m0) | ((r - r0) << 16);
             }
-            d[7] = b * (S + 1);
+            d[7] = 0;
         }
     }
 
-    // ---- chunks: each slab's stream cut at row ends into about equal parts, about one per CU in all
+    // ---- chunks: each slab's stream cut at row ends into about equal parts, about one per CU in all; a chunk
+    // also ends before its pairs would touch more than kLsMaxRuns combine blocks (a block change is a row end)
     int dev = 0, cus = 256;
     MUXI_HIP(hipGetDevice(&dev));
     MUXI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -717,15 +631,25 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
         const long want = std::min((nq + k - 1) / k, max_piece);
         long a = off[q];
         while (a < e) {
-            long c = std::min(e, a + want) - 1;  // the cut goes after the first row end at or past c
-            while (!end[c]) ++c;
-            if (c + 1 - a > (long)kLsMaxTiles * TILE) {
+            long i = a;
+            int runs = 1, blk = row_block[srow[a]];
+            while (i + 1 < e) {
+                if (end[i] && i + 1 - a >= want) break;
+                const int nb = row_block[srow[i + 1]];
+                if (nb != blk) {
+                    if (runs == kLsMaxRuns) break;  // i ends a row: the next element is in another block
+                    ++runs;
+                    blk = nb;
+                }
+                ++i;
+            }
+            if (i + 1 - a > (long)kLsMaxTiles * TILE) {
                 set_error("lslab: a row holds more than " + std::to_string(W) +
                           " nonzeros in one column slab (repeated columns)");
                 return SPMV_ERROR_INVALID_MATRIX;
             }
-            pieces.push_back({q, a, c + 1 - a});
-            a = c + 1;
+            pieces.push_back({q, a, i + 1 - a});
+            a = i + 1;
         }
     }
     // launch order: slab q's chunks on XCD q mod 8 (workgroup b runs on XCD b mod 8), so a slab of x is read
@@ -738,12 +662,32 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
     for (size_t r = 0; r < rounds; ++r)
         for (int xc = 0; xc < 8; ++xc) order.push_back(r < per_xcd[xc].size() ? per_xcd[xc][r] : -1);
     while (!order.empty() && order.back() < 0) order.pop_back();
-    // padded tile layout in piece order
+    // padded tile layout and run tables in piece order
     std::vector<long> tile0(pieces.size());
+    std::vector<int> run0(pieces.size());
+    std::vector<int> hrun;  // (first pair, offset) per run, then (INT_MAX, 0) per piece
     long ntiles = 0;
     for (size_t i = 0; i < pieces.size(); ++i) {
+        const Piece& pc = pieces[i];
         tile0[i] = ntiles;
-        ntiles += (pieces[i].n + TILE - 1) / TILE;
+        ntiles += (pc.n + TILE - 1) / TILE;
+        run0[i] = (int)(hrun.size() / 2);
+        const int jA = ends_before[pc.a], jB = ends_before[pc.a + pc.n];
+        int prev = -1;
+        for (int j = jA; j < jB; ++j) {
+            const int b = row_block[pair_row[j]];
+            if (b != prev) {
+                hrun.push_back(j);
+                hrun.push_back(delta[(size_t)b * S + pc.slab]);
+                prev = b;
+            }
+        }
+        hrun.push_back(INT_MAX);
+        hrun.push_back(0);
+    }
+    for (int i = 0; i <= kLsMaxRuns; ++i) {  // every chunk reads kLsMaxRuns + 1 entries of its table
+        hrun.push_back(INT_MAX);
+        hrun.push_back(0);
     }
     const long npad = ntiles * TILE;
     if (ntiles >= INT_MAX / 2 || npad >= (long)INT_MAX) {
@@ -751,7 +695,7 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
         return SPMV_ERROR_INVALID_MATRIX;
     }
     std::vector<uint16_t> hlc((size_t)std::max(npad, 1L), 0);
-    std::vector<int> hsrc((size_t)std::max(npad, 1L), -1), htile((size_t)std::max(ntiles, 1L), 0);
+    std::vector<int> hsrc((size_t)std::max(npad, 1L), -1), htile((size_t)2 * std::max(ntiles, 1L), 0);
     for (size_t i = 0; i < pieces.size(); ++i) {
         const Piece& pc = pieces[i];
         const long base = tile0[i] * TILE;
@@ -761,10 +705,14 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
             hsrc[base + e] = src[si];
         }
         const long nt = (pc.n + TILE - 1) / TILE;
+        int r = run0[i];
         for (long lt = 0; lt < nt; ++lt) {
             const long si = pc.a + lt * TILE;
             const bool mid = lt > 0 && !end[si - 1];
-            htile[tile0[i] + lt] = ends_before[si] | (mid ? (int)0x80000000u : 0);
+            const int pj = ends_before[si];
+            while (hrun[2 * (r + 1)] <= pj) ++r;  // the sentinel stops the walk
+            htile[2 * (tile0[i] + lt)] = pj | (mid ? (int)0x80000000u : 0);
+            htile[2 * (tile0[i] + lt) + 1] = r - run0[i];
         }
     }
     std::vector<int> hch(4 * std::max<size_t>(order.size(), 1), 0);
@@ -775,12 +723,12 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
         hch[4 * b + 0] = (int)tile0[i];
         hch[4 * b + 1] = (int)((pc.n + TILE - 1) / TILE);
         hch[4 * b + 2] = pc.slab * W;
+        hch[4 * b + 3] = run0[i];
     }
     // ---- device arrays
     p->ls_S = S;
     p->ls_chunks = (int)order.size();
     p->ls_blocks = B;
-    p->ls_grid = 2 * std::max(1, cus);  // two persistent combine workgroups per CU
     p->ls_pairs = npairs;
     const size_t vs = vsize(vt), ps = p->comp ? 8 : 4;
     MUXI_HIP(hipMalloc(&p->d_ls_lc, sizeof(uint16_t) * hlc.size()));
@@ -833,8 +781,8 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
 template <typename VT, bool COMP, int EPL, int W>
 static void launch_slab(const muxi_plan* p, const float* x, hipStream_t s) {
     hipLaunchKernelGGL((lslab_slab_kernel<VT, COMP, EPL, W>), dim3(p->ls_chunks), dim3(kLsThreads), 0, s,
-                       (const int4*)p->d_ls_ch, p->d_ls_lc, (const VT*)p->d_ls_lv, p->d_ls_tp, x, p->A.C,
-                       (LsPair<COMP>*)p->d_ls_P);
+                       (const int4*)p->d_ls_ch, p->d_ls_lc, (const VT*)p->d_ls_lv, (const int2*)p->d_ls_tp,
+                       (const int2*)p->d_ls_F, x, p->A.C, (LsPair<COMP>*)p->d_ls_P);
 }
 
 template <typename VT, bool COMP>
@@ -845,27 +793,22 @@ spmv_status_t launch_lslab(const muxi_plan* p, const float* x, float* y, float a
         return SPMV_SUCCESS;
     }
     using PT = LsPair<COMP>;
-    const bool half = (p->param0 & kLsHalfSlabs) != 0, four = (p->param0 & 0xff) == 4;
     if (p->ls_chunks > 0) {
-        if (half) {
-            if (four) launch_slab<VT, COMP, 4, kLsSlabCols / 2>(p, x, s);
-            else launch_slab<VT, COMP, 8, kLsSlabCols / 2>(p, x, s);
+        // the slab width follows lslab_slab_cols: full slabs only with four nonzeros a lane and plain sums
+        const bool four = (p->param0 & 0xff) == 4;
+        if (lslab_slab_cols(p->param0, COMP) == kLsSlabCols) {
+            if constexpr (!COMP) launch_slab<VT, false, 4, kLsSlabCols>(p, x, s);
+        } else if (four) {
+            launch_slab<VT, COMP, 4, kLsSlabCols / 2>(p, x, s);
         } else {
-            if (four) launch_slab<VT, COMP, 4, kLsSlabCols>(p, x, s);
-            else launch_slab<VT, COMP, 8, kLsSlabCols>(p, x, s);
+            launch_slab<VT, COMP, 8, kLsSlabCols / 2>(p, x, s);
         }
         MUXI_LAUNCH_CHECK();
     }
     if (ev_end) MUXI_HIP(hipEventRecord(ev_end, s));
     if (p->ls_blocks > 0) {
-        if (p->param0 & kLsPersistent)
-            hipLaunchKernelGGL((lslab_combine_kernel<COMP, kLsPairBytes, kLsRowsMax>),
-                               dim3(std::min(p->ls_blocks, p->ls_grid)), dim3(256), 0, s, p->d_ls_blk, p->ls_blocks,
-                               p->ls_S, (const int2*)p->d_ls_F, p->d_ls_map, (const PT*)p->d_ls_P, y, alpha, beta);
-        else
-            hipLaunchKernelGGL((lslab_combine1_kernel<COMP, kLsPairBytes1, kLsRowsMax1>), dim3(p->ls_blocks),
-                               dim3(256), 0, s, p->d_ls_blk, p->ls_S, (const int2*)p->d_ls_F, p->d_ls_map,
-                               (const PT*)p->d_ls_P, y, alpha, beta);
+        hipLaunchKernelGGL((lslab_combine_kernel<COMP, kLsPairBytes, kLsRowsMax>), dim3(p->ls_blocks), dim3(256), 0,
+                           s, p->d_ls_blk, p->d_ls_map, (const PT*)p->d_ls_P, y, alpha, beta);
         MUXI_LAUNCH_CHECK();
     }
     return SPMV_SUCCESS;

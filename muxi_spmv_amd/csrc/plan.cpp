@@ -215,9 +215,8 @@ static spmv_status_t plan_create_impl(const muxi_csr_t* Ain, spmv_type_t vt, spm
         }
         case MUXI_KERNEL_LSLAB: {
             if (param0 <= 0) param0 = 4;
-            if (!((param0 & 0xff) == 4 || (param0 & 0xff) == 8) || (param0 & ~(0xff | kLsHalfSlabs | kLsPersistent))) {
-                set_error("lslab: param0 = nonzeros per lane (4 or 8) | 0x100 (half slabs) | 0x200 (persistent "
-                          "combine)");
+            if (!((param0 & 0xff) == 4 || (param0 & 0xff) == 8) || (param0 & ~(0xff | kLsHalfSlabs))) {
+                set_error("lslab: param0 = nonzeros per lane (4 or 8) | 0x100 (slabs of 16384 columns)");
                 plan_free(p);
                 return SPMV_ERROR_INTERNAL;
             }

@@ -12,12 +12,15 @@ constexpr int kLsThreads = 1024;     // slab-kernel workgroup
 constexpr int kLsMaxTiles = 512;     // tiles per slab-kernel chunk (LDS head/tail sums)
 constexpr int kLsLdsBytes = 163840;  // LDS of one CU (MI355X)
 constexpr int kLsMaxSlabs = 256;     // slabs (8,388,608 columns)
-constexpr int kLsPairBytes = 16384;  // persistent combine: LDS for a row block's pair sums (and its map), x 2
-constexpr int kLsRowsMax = 2048;     // persistent combine: rows per block
-constexpr int kLsPairBytes1 = 8192;  // one-block combine: LDS for a row block's pair sums (and its map)
-constexpr int kLsRowsMax1 = 1024;    // one-block combine: rows per block
-constexpr int kLsHalfSlabs = 0x100;  // param0 bit: slabs of kLsSlabCols / 2 columns (two slab workgroups per CU)
-constexpr int kLsPersistent = 0x200; // param0 bit: persistent, double-buffered combine
+constexpr int kLsPairBytes = 8192;   // combine: LDS for a row block's pair sums (and as much for its map)
+constexpr int kLsRowsMax = 1024;     // combine: rows per block
+constexpr int kLsMaxRuns = 1024;     // combine blocks one slab-kernel chunk may write to
+constexpr int kLsHalfSlabs = 0x100;  // param0 bit: slabs of kLsSlabCols / 2 columns
+// Columns per slab: full slabs (128 KiB of x) leave room in LDS for the staged pair sums only with four nonzeros
+// a lane and plain FP32 sums; otherwise, or when asked, half slabs.
+inline int lslab_slab_cols(int param0, bool comp) {
+    return ((param0 & 0xff) == 4 && !comp && !(param0 & kLsHalfSlabs)) ? kLsSlabCols : kLsSlabCols / 2;
+}
 struct SlabArgs {
     int S;
     int rows[kMaxSlabs];       // compressed rows of each slab (none is empty)
@@ -82,14 +85,14 @@ struct muxi_plan {
     int* d_slab_ckey = nullptr;  // carry keys (pair ids) per tile
     // LDS column slabs (lslab): slab streams (16-bit column offsets with row-end bits, values), tile
     // descriptors, chunk descriptors (launch order), combine blocks, run tables, row-major pair map, pair sums
-    int ls_S = 0, ls_chunks = 0, ls_blocks = 0, ls_grid = 1;
+    int ls_S = 0, ls_chunks = 0, ls_blocks = 0;
     long ls_pairs = 0;
     uint16_t* d_ls_lc = nullptr;
     void* d_ls_lv = nullptr;
-    int* d_ls_tp = nullptr;   // per tile: first pair | mid-pair bit
-    int* d_ls_ch = nullptr;   // per launch slot: tile0, tiles, first column, 0
-    int* d_ls_blk = nullptr;  // per combine block: first row, rows, first map entry, pairs, wave splits, runs
-    int* d_ls_F = nullptr;    // per combine block and slab: (first pair in P, LDS offset), then (0, pairs)
+    int* d_ls_tp = nullptr;   // per tile: (first pair | mid-pair bit, run in its chunk)
+    int* d_ls_ch = nullptr;   // per launch slot: tile0, tiles, first column, run table offset
+    int* d_ls_blk = nullptr;  // per combine block: first row, rows, first pair, pairs, wave splits
+    int* d_ls_F = nullptr;    // run tables of the chunks: (first pair, P offset) per run, then (INT_MAX, 0)
     unsigned* d_ls_map = nullptr;
     void* d_ls_P = nullptr;
 };
@@ -152,8 +155,8 @@ template <typename VT, typename XT, typename AT>
 spmv_status_t launch_xslab(const muxi_plan* p, const XT* x, XT* y, AT alpha, AT beta, hipStream_t s,
                            hipEvent_t ev_end);
 
-// LDS column-slab plan and launcher (kernels_lslab.hip); param0 = nonzeros per lane (4 or 8) | kLsHalfSlabs |
-// kLsPersistent; FP32 x and y
+// LDS column-slab plan and launcher (kernels_lslab.hip); param0 = nonzeros per lane (4 or 8) | kLsHalfSlabs;
+// FP32 x and y
 spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t s);
 template <typename VT, bool COMP>
 spmv_status_t launch_lslab(const muxi_plan* p, const float* x, float* y, float alpha, float beta, hipStream_t s,

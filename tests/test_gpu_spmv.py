@@ -19,8 +19,7 @@ KCONF = [("scalar", 0), ("vector", 0)] + [("vwave", k) for k in (1, 2, 4, 8, 16,
         [("merge", k) for k in (2, 4, 8, 16)] + [("csr5", s) for s in (64, 128, 256, 512, 1024)] + [("ell", 0)] + \
         [("adaptive", b) for b in (256, 512, 1024, 2048)] + \
         [("xslab", (S << 8) | ipt) for S, ipt in ((8, 4), (8, 8), (1, 8), (3, 8), (6, 4), (16, 4), (2, 4), (4, 4))] + \
-        [("xslab", 0)] + [("lslab", 4), ("lslab", 8), ("lslab", 0), ("lslab", 0x108), ("lslab", 0x208),
-                          ("lslab", 0x304)]
+        [("xslab", 0)] + [("lslab", 4), ("lslab", 8), ("lslab", 0), ("lslab", 0x104)]
 
 
 def xvec(n, seed=11, dtype=np.float32):
@@ -332,7 +331,7 @@ def test_full_size_powerlaw_properties(powerlaw_full, kernel, param):
 
 @pytest.mark.parametrize("kernel,param", [("auto", 0), ("xslab", 1032), ("xslab", 1028), ("xslab", 2056),
                                           ("xslab", (16 << 8) | 4), ("merge", 4), ("lslab", 8), ("lslab", 4),
-                                          ("lslab", 0x108), ("lslab", 0x208), ("lslab", 0x304)])
+                                          ("lslab", 0x104)])
 def test_full_size_powerlaw_integers_exact(powerlaw_full, kernel, param):
     """The power-law pattern at full size with integer values in [-3, 3] and x in [-4, 4]: every partial sum is
     an integer below 2^24 in magnitude, so FP32 y is exact in any summation order and must equal the integer
@@ -493,7 +492,7 @@ def _dense_rows_matrix(seed=17):
     return G.HostCSR(rp, ci, v, len(rows), C)
 
 
-@pytest.mark.parametrize("param", [4, 8, 0x108, 0x204, 0x308])
+@pytest.mark.parametrize("param", [4, 8, 0x104])
 @pytest.mark.parametrize("comp", [False, True])
 def test_lslab_dense_rows_and_slab_edges(param, comp):
     h = _dense_rows_matrix()
@@ -503,8 +502,9 @@ def test_lslab_dense_rows_and_slab_edges(param, comp):
 
 
 def test_lslab_integer_exact_many_slabs():
-    """A 300 x 8,000,000 matrix (245 slabs, the last one partial) with integer values and x: y equals the
-    host's integer product, so every slab offset, chunk, pair and combine map entry lands in the right place."""
+    """A 300 x 8,000,000 matrix (245 slabs of 32768 columns, the last one partial) with integer values and x: y
+    equals the host's integer product, so every slab offset, chunk, pair and combine map entry lands in the right
+    place; the same with 16384-column slabs on the first 4,000,000 columns."""
     rng = np.random.default_rng(23)
     R, C = 300, 8_000_000
     lens = rng.integers(0, 4000, R)
@@ -517,9 +517,22 @@ def test_lslab_integer_exact_many_slabs():
     exp = np.zeros(R, np.int64)
     rows = np.repeat(np.arange(R), np.diff(h.rowPtr))
     np.add.at(exp, rows, h.values.astype(np.int64) * xi[h.colIdx])
-    for param in (4, 8, 0x208):  # half slabs would need 489 slabs here (over the 256 a plan takes)
-        y, _, _, _ = run_plan(h, "lslab", param, x=xi.astype(np.float32))
-        assert np.array_equal(y.cpu().numpy(), exp.astype(np.float32)), param
+    y, _, _, _ = run_plan(h, "lslab", 4, x=xi.astype(np.float32))  # full slabs (half ones would need 489)
+    assert np.array_equal(y.cpu().numpy(), exp.astype(np.float32))
+    for param in (8, 0x104):  # 16384-column slabs: over the 256 a plan takes
+        with pytest.raises(api.SpmvError, match="INVALID_MATRIX"):
+            run_plan(h, "lslab", param, x=xi.astype(np.float32))
+    # the same pattern on its first 4,000,000 columns (245 half slabs) with 16384-column slabs
+    keep = h.colIdx < 4_000_000
+    rows = np.repeat(np.arange(R), np.diff(h.rowPtr))[keep]
+    rp2 = np.zeros(R + 1, np.int32)
+    np.add.at(rp2, rows + 1, 1)
+    h2 = G.HostCSR(np.cumsum(rp2).astype(np.int32), h.colIdx[keep], h.values[keep], R, 4_000_000)
+    exp2 = np.zeros(R, np.int64)
+    np.add.at(exp2, rows, h2.values.astype(np.int64) * xi[h2.colIdx])
+    for param in (8, 0x104):
+        y, _, _, _ = run_plan(h2, "lslab", param, x=xi[:4_000_000].astype(np.float32))
+        assert np.array_equal(y.cpu().numpy(), exp2.astype(np.float32)), param
 
 
 def test_lslab_rejects_fp64_x_and_too_many_columns():
