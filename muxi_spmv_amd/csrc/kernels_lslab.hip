@@ -176,8 +176,8 @@ struct LsLoad<bf16_t, N> {
 // chunks[4b ..] = (tile0, ntiles, first column of the slab, run table offset); ntiles = 0 is an empty launch
 // slot. tiles[t] = (index of the pair holding tile t's first element, bit 31 set when that pair started in an
 // earlier tile; the chunk run that pair falls in). Pairs are numbered in stream order (slab, then row); a
-// chunk's pairs split into runs by combine block, runs[off + r] = (first pair of run r, its offset) with a
-// sentinel first pair INT_MAX after the last run, and pair j of run r goes to P[j + offset]: every combine
+// chunk's pairs split into runs by combine block, runs[off + r] = (first pair of run r, its offset) with nine
+// sentinels (first pair INT_MAX) after the last run, and pair j of run r goes to P[j + offset]: every combine
 // block then finds its pair sums in one contiguous range of P.
 template <typename VT, bool COMP, int EPL, int W>
 __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __restrict__ chunks,
@@ -190,15 +190,14 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
     using PT = LsPair<COMP>;
     constexpr int TILE = 64 * EPL;
     constexpr int NWAVE = kLsThreads / 64;
-    static_assert(W * 4 + kLsThreads * EPL * (int)sizeof(PT) + kLsMaxTiles * (2 * (int)sizeof(PT) + 1) +
-                          (kLsMaxRuns + 1) * 8 <= kLsLdsBytes,
+    static_assert(W * 4 + kLsThreads * EPL * (int)sizeof(PT) + kLsMaxTiles * (2 * (int)sizeof(PT) + 1) <=
+                      kLsLdsBytes,
                   "slab kernel LDS");
     __shared__ float xs[W];
     __shared__ PT stage[kLsThreads * EPL];  // per wavefront: one tile's pair sums in pair order
     __shared__ PT head[kLsMaxTiles];
     __shared__ PT tail[kLsMaxTiles];
     __shared__ unsigned char tflag[kLsMaxTiles];  // bit 0: the tile has a row end; bit 1: starts mid-pair
-    __shared__ int2 lrun[kLsMaxRuns + 1];
     const int4 ch = chunks[blockIdx.x];
     const int ntiles = ch.y;
     if (ntiles == 0) return;  // whole workgroup, before any barrier
@@ -215,9 +214,6 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
     };
     uint16_t cn[EPL];
     VT vn[EPL];
-    // the chunk's run table: at most kLsMaxRuns runs and the sentinel (the table array ends with kLsMaxRuns + 1
-    // more sentinels, so these loads stay inside it for the last chunk too)
-    for (int i = tid; i <= kLsMaxRuns; i += kLsThreads) lrun[i] = runs[ch.w + i];
     // the slab of x into LDS: a whole slab with 16-byte loads, all in flight together (then the first tile's
     // loads join them), the last, partial slab one element at a time
     const float* xg = x + x0;
@@ -273,17 +269,17 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
             }
         }
         // the tile's pairs to P with whole-wavefront stores of consecutive pairs; the tile's first pair, when it
-        // started in an earlier tile, is finished after the barrier. Runs are taken eight at a time: lanes 0-8
-        // read run r .. r + 8 in one LDS access, and a lane picks its pair's offset from those by comparisons.
+        // started in an earlier tile, is finished after the barrier. Runs are taken eight at a time, read with
+        // scalar loads (the run index is uniform), and a lane picks its pair's offset by comparisons.
         int r = td.y, j = pair0 + (mid ? 1 : 0);
         const int jend = pair0 + total;
         while (j < jend) {
-            const int2 rr = lrun[min(r + min(lane, 8), kLsMaxRuns)];
+            const int2* rt = runs + ch.w + r;
             int bnd[8], off[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                bnd[i] = __builtin_amdgcn_readlane(rr.x, i + 1);  // first pair after run r + i
-                off[i] = __builtin_amdgcn_readlane(rr.y, i);
+                bnd[i] = rt[i + 1].x;  // first pair after run r + i (each chunk's table ends in 9 sentinels)
+                off[i] = rt[i].y;
             }
             const int je = min(jend, bnd[7]);
             for (int q = j + lane; q < je; q += 64) {
@@ -324,7 +320,7 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
             if ((tflag[q] & 1) || q == 0) break;
         }
         const int2 td = tiles[(long)ch.x + lt];
-        P[(td.x & 0x7fffffff) + lrun[td.y].y] = s;
+        P[(td.x & 0x7fffffff) + runs[ch.w + td.y].y] = s;
     }
 }
 
@@ -682,13 +678,12 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
                 prev = b;
             }
         }
-        hrun.push_back(INT_MAX);
-        hrun.push_back(0);
+        for (int k = 0; k < 9; ++k) {  // a tile reads up to 8 entries past its run: they stay in its chunk's table
+            hrun.push_back(INT_MAX);
+            hrun.push_back(0);
+        }
     }
-    for (int i = 0; i <= kLsMaxRuns; ++i) {  // every chunk reads kLsMaxRuns + 1 entries of its table
-        hrun.push_back(INT_MAX);
-        hrun.push_back(0);
-    }
+    if (hrun.empty()) hrun.assign(2, 0);
     const long npad = ntiles * TILE;
     if (ntiles >= INT_MAX / 2 || npad >= (long)INT_MAX) {
         set_error("lslab: padded nonzeros exceed INT32_MAX");
