@@ -13,11 +13,13 @@ same way). --kernel sweep times the sweep's best instead, --kernel k:p one
 configuration.
 
 Prints one JSON line: value = algorithmic GB/s over the K timed steps
-(bytes = N*(4+4) + 4*(R+1) + 4*C + 4*R per step); roofline = the main
-kernel's average duration from HIP events recorded around it in every
+(bytes = N*(4+4) + 4*(R+1) + 4*C + 4*R per step); roofline = those bytes
+over the launch's kernel time from HIP events recorded around it in every
 fourth timed step (a sample spread over the timed region; markers on every
-step would add their own cost to a short step); cpu_baseline = the oracle's
-FP32 CSR loop on the host cores.
+step would add their own cost to a short step). The window holds every
+kernel of the launch: for the two-kernel LDS slab path the slab kernel and
+the combine after it, whose shares are reported beside it;
+cpu_baseline = the oracle's FP32 CSR loop on the host cores.
 """
 from __future__ import annotations
 
@@ -35,15 +37,16 @@ CANDIDATES = [("merge", 4), ("merge", 8), ("merge", 16), ("csr5", 256), ("csr5",
               ("adaptive", 512), ("adaptive", 1024), ("adaptive", 2048), ("ell", 0),
               ("xslab", (2 << 8) | 4), ("xslab", (4 << 8) | 4), ("xslab", (8 << 8) | 4),
               ("xslab", (2 << 8) | 8), ("xslab", (4 << 8) | 8), ("xslab", (8 << 8) | 8),
-              ("vwave", 4), ("vwave", 8), ("vector", 0), ("scalar", 0)]
+              ("lslab", 4), ("lslab", 8), ("vwave", 4), ("vwave", 8), ("vector", 0), ("scalar", 0)]
 
 
-# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
+# HBM bytes per launch (every kernel of the launch), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
 # `bench.py --kernel K:P` (KiB per dispatch, FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors a 1 GiB copy
-# kernel reads); the runs and commands are in profiles/xslab_spmv_powerlaw_2m_fp32.md,
-# profiles/ell_spmv_laplacian_4096_fp64.md and profiles/ell_spmv_uniform_100k_fp32.md. Configurations not
-# measured report null.
+# kernel reads); the runs and commands are in profiles/lslab_spmv_powerlaw_2m_fp32.md,
+# profiles/xslab_spmv_powerlaw_2m_fp32.md, profiles/ell_spmv_laplacian_4096_fp64.md and
+# profiles/ell_spmv_uniform_100k_fp32.md. Configurations not measured report null.
 MEASURED_TRAFFIC = {
+    ("powerlaw_2m_fp32", "lslab", 4): int(((48184.8 + 25049.7) * 2.0 + (26488.3 + 9748.6) * 1.0) * 1024),
     ("powerlaw_2m_fp32", "xslab", 1028): int((66336.6 * 2.0 + 15477.4 * 1.0) * 1024),
     ("powerlaw_2m_fp32", "xslab", 1032): int((68168.4 * 2.0 + 15791.2 * 1.0) * 1024),
     ("laplacian_4096_fp64", "ell", 5): int((483392.3 * 2.0 + 131072.0 * 1.0) * 1024),
@@ -66,6 +69,16 @@ def make_workload(name: str):
     if name == "laplacian":
         return G.cfg_laplacian_4096(), np.float64, "laplacian_4096_fp64"
     raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(h, x, budget_s: float = 12.0) -> dict:
@@ -94,7 +107,7 @@ def cpu_baseline(h, x, budget_s: float = 12.0) -> dict:
         if time.perf_counter() - t0 >= budget_s / 4:
             break
     ms1 = (time.perf_counter() - t0) * 1e3 / reps1
-    return {"value": round(b / ms / 1e6, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+    return {"value": round(b / ms / 1e6, 3), "unit": "GB/s", "cores": threads, "kind": "port", "cpu": cpu_model(),
             "sample": f"full matrix, {reps} SpMVs of the FP32 CSR loop ({ms:.2f} ms each), OpenMP rows",
             "single_thread": {"value": round(b / ms1 / 1e6, 3), "ms": round(ms1, 3), "reps": reps1}}
 
@@ -161,23 +174,27 @@ def main(argv=None) -> int:
                  and not (c[0] == "ell" and h.numRows * maxlen > 4 * h.nnz)]
         for k, p in cands:
             plans = [api.Plan(m, k, p, xtype=tdt) for m in mats]
-            m = measure_plans(plans, xs, ys, a.sweep_steps, 3, record_kernel=False)
-            sweep[f"{k}:{p}"] = round(m.ms_per_step, 5)
-            log(f"sweep {k}:{p} {m.ms_per_step * 1e3:.1f} us/step")
+            ms_k = measure_plans(plans, xs, ys, a.sweep_steps, 3, record_kernel=False).ms_per_step
+            sweep[f"{k}:{p}"] = round(ms_k, 5)
+            log(f"sweep {k}:{p} {ms_k * 1e3:.1f} us/step")
             del plans
         best = min(sweep, key=sweep.get)
     selector = None
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter()
     if a.kernel == "auto":
         from muxi_spmv_amd._lib import load
-        plans = [api.Plan(m, "auto", 0, xtype=tdt) for m in mats]
+        plans = [api.Plan(mats[0], "auto", 0, xtype=tdt)]
+        torch.cuda.synchronize()
+        t_plan = time.perf_counter() - t_plan
+        plans += [api.Plan(m, "auto", 0, xtype=tdt) for m in mats[1:]]
         info = plans[0].info()
         pick = f"{info.kernel}:{info.param0}"
         key = pick
         if info.kernel == "ell" and info.param0 == maxlen and "ell:0" in sweep:
             key = "ell:0"  # the same plan: an ELL width of 0 means the longest row
         if key not in sweep:  # time the pick the way the sweep timed its candidates
-            m = measure_plans(plans, xs, ys, a.sweep_steps, 3, record_kernel=False)
-            sweep[key] = round(m.ms_per_step, 5)
+            sweep[key] = round(measure_plans(plans, xs, ys, a.sweep_steps, 3, record_kernel=False).ms_per_step, 5)
         state = {0: "fixed rule", 1: "learned (built in)", 2: "learned (set by caller)"}[load().muxi_selector_state()]
         selector = {"state": state, "pick": pick, "sweep_best": best,
                     "regret": round(sweep[key] / sweep[best] - 1.0, 4)}
@@ -186,7 +203,10 @@ def main(argv=None) -> int:
         if a.kernel != "sweep":
             best = a.kernel
         kname, kparam = best.split(":")
-        plans = [api.Plan(m, kname, int(kparam), xtype=tdt) for m in mats]
+        plans = [api.Plan(mats[0], kname, int(kparam), xtype=tdt)]
+        torch.cuda.synchronize()
+        t_plan = time.perf_counter() - t_plan
+        plans += [api.Plan(m, kname, int(kparam), xtype=tdt) for m in mats[1:]]
         info = plans[0].info()
 
     torch.cuda.synchronize()
@@ -194,9 +214,14 @@ def main(argv=None) -> int:
     ms = meas.ms_per_step
     value = bytes_step / ms / 1e6
     kms = meas.kernel_ms_mean
-    achieved = bytes_step / kms / 1e6
-    log(f"{info.kernel}:{info.param0}: {ms * 1e3:.2f} us/step, main kernel {kms * 1e3:.2f} us "
-        f"(median {meas.kernel_ms_median * 1e3:.2f})")
+    ems = meas.exec_ms_mean
+    if not ems == ems:  # no sample of the whole launch (very few steps): the wall clock per step
+        ems = ms
+    if not kms == kms:
+        kms = ems
+    achieved = bytes_step / ems / 1e6
+    log(f"{info.kernel}:{info.param0}: {ms * 1e3:.2f} us/step, launch {ems * 1e3:.2f} us, main kernel "
+        f"{kms * 1e3:.2f} us (median {meas.kernel_ms_median * 1e3:.2f})")
 
     # correctness of the measured configuration against the device FP64 oracle
     y = torch.zeros(h.numRows, dtype=tdt, device="cuda")
@@ -204,6 +229,7 @@ def main(argv=None) -> int:
     y64 = api.oracle_f64(A0, xs[0])
     torch.cuda.synchronize()
     rel_l2 = float(torch.linalg.vector_norm(y.double() - y64) / torch.linalg.vector_norm(y64))
+    max_abs = float((y.double() - y64).abs().max()) if h.numRows else 0.0
 
     out = {
         "metric": "SpMV GB/s (frac of HBM peak) + GFLOP/s; learned-pick regret vs sweep",
@@ -225,11 +251,16 @@ def main(argv=None) -> int:
         "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
         "frac_of_device_copy": round(value / copy_gbs, 4),
         "rel_l2_vs_fp64": rel_l2,
+        "max_abs_vs_fp64": max_abs,
+        "plan": {"build_ms": round(t_plan * 1e3, 2), "scratch_bytes": info.scratch_bytes},
         "hbm_copy_gbs": round(copy_gbs, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get((wname, info.kernel, info.param0)),
-                     "kernel_us_mean": round(kms * 1e3, 3), "bytes_per_launch": bytes_step},
+                     "kernels": "slab kernel + combine" if info.kernel == "lslab" else f"{info.kernel} main kernel"
+                     + (" + carry fix-up" if info.kernel in ("merge", "csr5", "adaptive", "xslab") else ""),
+                     "launch_us_mean": round(ems * 1e3, 3), "main_kernel_us_mean": round(kms * 1e3, 3),
+                     "bytes_per_launch": bytes_step},
         "sweep_ms": sweep,
         "selector": selector,
     }

@@ -73,6 +73,7 @@ class Events:
 class Measurement:
     ms_per_step: float          # wall clock over the timed launches / steps
     kernel_ms: list = field(default_factory=list)  # main kernel per launch (HIP events)
+    exec_ms: list = field(default_factory=list)    # the whole launch, every kernel (HIP events)
     steps: int = 0
 
     @property
@@ -82,6 +83,10 @@ class Measurement:
     @property
     def kernel_ms_median(self) -> float:
         return statistics.median(self.kernel_ms) if self.kernel_ms else float("nan")
+
+    @property
+    def exec_ms_mean(self) -> float:
+        return statistics.fmean(self.exec_ms) if self.exec_ms else float("nan")
 
 
 def copies_for_cold(bytes_per_copy: int, floor: int = 3) -> int:
@@ -94,11 +99,13 @@ def measure_plans(plans: list[Plan], xs: list[torch.Tensor], ys: list[torch.Tens
                   alpha: float = 1.0, beta: float = 0.0, nrhs: int = 1, record_kernel: bool = True,
                   event_every: int = 4) -> Measurement:
     """Launches plans[i % n] on xs/ys round-robin: `warmup` untimed, then
-    `steps` timed between two device synchronisations. HIP events around the
-    main kernel of every `event_every`-th timed launch (a sample spread evenly
-    over the timed region and over the rotating copies) give the per-kernel
-    durations; the other launches carry no event markers, which on a short
-    kernel would add their own cost to the wall clock."""
+    `steps` timed between two device synchronisations. Every `event_every`-th
+    timed launch (a sample spread evenly over the timed region and over the
+    rotating copies) carries two HIP events: on the first, third, ... sample
+    they bracket the whole launch (the main kernel and any combine or fix-up
+    kernel after it, with no marker between them: exec_ms), on the others the
+    main kernel alone (kernel_ms). The other launches carry no event markers,
+    which on a short kernel would add their own cost to the wall clock."""
     n = len(plans)
     s = _stream(None)
     lib = load()
@@ -114,16 +121,21 @@ def measure_plans(plans: list[Plan], xs: list[torch.Tensor], ys: list[torch.Tens
     for i in range(steps):
         j = i % n
         k = slot.get(i)
-        if k is not None:
+        if k is None:
+            check(lib.muxi_plan_exec(plans[j]._p, _ptr(xs[j]), _ptr(ys[j]), alpha, beta, nrhs, s), "plan_exec")
+        elif k % 2 == 1:  # the main kernel
             check(lib.muxi_plan_exec_ev(plans[j]._p, _ptr(xs[j]), _ptr(ys[j]), alpha, beta, nrhs, s,
                                         ev.ev[2 * k], ev.ev[2 * k + 1]), "plan_exec_ev")
-        else:
-            check(lib.muxi_plan_exec(plans[j]._p, _ptr(xs[j]), _ptr(ys[j]), alpha, beta, nrhs, s), "plan_exec")
+        else:  # the whole launch, with no marker between its kernels
+            check(lib.muxi_plan_exec_ev(plans[j]._p, _ptr(xs[j]), _ptr(ys[j]), alpha, beta, nrhs, s,
+                                        ev.ev[2 * k], None), "plan_exec_ev")
+            check(lib.muxi_event_record(ev.ev[2 * k + 1], s), "event_record")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     m = Measurement(ms_per_step=(t1 - t0) * 1e3 / steps, steps=steps)
     if ev is not None:
-        m.kernel_ms = [ev.elapsed(2 * k, 2 * k + 1) for k in range(len(sampled))]
+        m.exec_ms = [ev.elapsed(2 * k, 2 * k + 1) for k in range(0, len(sampled), 2)]
+        m.kernel_ms = [ev.elapsed(2 * k, 2 * k + 1) for k in range(1, len(sampled), 2)]
     return m
 
 

@@ -105,8 +105,11 @@ static spmv_status_t plan_create_impl(const muxi_csr_t* Ain, spmv_type_t vt, spm
     if (kernel == MUXI_KERNEL_AUTO) {
         int k = MUXI_KERNEL_MERGE, q = 4;
         PLAN_TRY(selector_pick_rows(Ain, xt, stream, row_base, rows_full, &k, &q));
-        if (p->comp && k == MUXI_KERNEL_XSLAB) {  // no compensated column-slab variant
-            k = MUXI_KERNEL_MERGE;
+        // Compensated sums: the L2 column-slab kernel has no Dot2 variant; its place goes to the LDS column slabs
+        // (Dot2 in half slabs, so at most kLsMaxSlabs * kLsSlabCols / 2 columns with FP32 x), else to merge-path.
+        if (p->comp && (k == MUXI_KERNEL_XSLAB || k == MUXI_KERNEL_LSLAB)) {
+            const bool ls_ok = xt == SPMV_TYPE_FLOAT && (long)Ain->numCols <= (long)kLsMaxSlabs * (kLsSlabCols / 2);
+            k = ls_ok ? MUXI_KERNEL_LSLAB : MUXI_KERNEL_MERGE;
             q = 4;
         }
         kernel = (muxi_kernel_t)k;

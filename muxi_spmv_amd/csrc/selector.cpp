@@ -74,10 +74,15 @@ static void feature_input(const double* f, bool fp64, float* out) {
     out[10] = fp64 ? 1.0f : 0.0f;
 }
 
-static bool candidate_allowed(int kernel, const double* f) {
+// The candidates the sweep times on a matrix (muxi_spmv_amd/tune.py, feasible): ELL within 4x padding, one lane or
+// one wavefront per row only up to 20,000 nonzeros a row, LDS column slabs only with FP32 x and 65,536 to
+// 8,388,608 columns.
+static bool candidate_allowed(int kernel, const double* f, spmv_type_t xtype, long cols) {
     const double rows = f[0], nnz = f[1], mx = f[4];
     if (kernel == MUXI_KERNEL_ELL) return nnz > 0 && rows * mx <= 4.0 * nnz && rows * mx < 2147483648.0;
     if (kernel == MUXI_KERNEL_SCALAR || kernel == MUXI_KERNEL_VECTOR) return mx <= 20000;
+    if (kernel == MUXI_KERNEL_LSLAB)
+        return xtype == SPMV_TYPE_FLOAT && cols >= (1L << 16) && cols <= (long)kLsMaxSlabs * kLsSlabCols;
     return true;
 }
 
@@ -185,7 +190,7 @@ spmv_status_t muxi::selector_pick_rows(const muxi_csr_t* A, spmv_type_t xtype, v
     MUXI_HIP(hipStreamSynchronize(hs));
     int best = -1;
     for (int i = 0; i < s.n_out; ++i) {
-        if (!candidate_allowed(s.kernel[i], f)) continue;
+        if (!candidate_allowed(s.kernel[i], f, xtype, A->numCols)) continue;
         if (best < 0 || out[i] < out[best]) best = i;
     }
     if (best < 0) {

@@ -101,8 +101,9 @@ def _todo(rec, cands, retime, retime_tag=""):
     f = np.array(rec["features"])
     if retime_tag and retime_tag in rec.get("retimed", ()):
         retime = ()
+    fp64 = rec.get("dtype") == "fp64"
     return [(k, p) for k, p in cands
-            if tune.feasible(k, f) and (f"{k}:{p}" not in rec["times"] or k in retime)]
+            if tune.feasible(k, f, fp64) and (f"{k}:{p}" not in rec["times"] or k in retime)]
 
 
 def run_sweeps(indices, db_path: Path, rows_min, rows_max, reps, log_every=25, time_budget_s=None,

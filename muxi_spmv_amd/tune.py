@@ -34,6 +34,7 @@ CANDIDATES: list[tuple[str, int]] = [
     ("xslab", (2 << 8) | 4), ("xslab", (4 << 8) | 4), ("xslab", (8 << 8) | 4),
     ("xslab", (2 << 8) | 8), ("xslab", (4 << 8) | 8), ("xslab", (8 << 8) | 8),
     ("ell", 0),
+    ("lslab", 4),
 ]
 CAND_NAMES = [f"{k}:{p}" for k, p in CANDIDATES]
 FEATURE_NAMES = ["rows", "nnz", "mean_len", "var_len", "max_len", "bandwidth", "diag_block_fill", "empty_frac"]
@@ -152,12 +153,19 @@ def time_plan_cold(plan: Plan, x, y, reps: int, flusher: Flusher) -> float:
     return statistics.median(ts)
 
 
-def feasible(kernel: str, f: np.ndarray) -> bool:
-    """Whether the sweep times this kernel on a matrix with features f."""
+LSLAB_MIN_COLS = 1 << 16  # below this x sits in every L2 and the two launches of lslab cannot pay off
+LSLAB_MAX_COLS = 256 * 32768  # 256 slabs of 32768 columns
+
+
+def feasible(kernel: str, f: np.ndarray, fp64: bool = False) -> bool:
+    """Whether the sweep times this kernel on a (square) matrix with features f and FP64 x if fp64. The library's
+    selector applies the same rules (selector.cpp, candidate_allowed)."""
     if kernel == "ell":
         return ell_feasible(f)
     if kernel in ("scalar", "vector"):
         return f[4] <= 20000  # one lane / one wavefront per row on 10^4+ rows: hopeless and slow to time
+    if kernel == "lslab":  # FP32 x in LDS
+        return not fp64 and LSLAB_MIN_COLS <= f[0] <= LSLAB_MAX_COLS
     return True
 
 
@@ -165,9 +173,10 @@ def sweep(A, x: torch.Tensor, y: torch.Tensor, flusher: Flusher, reps: int = 5,
           candidates=CANDIDATES, f: np.ndarray | None = None) -> dict[str, float]:
     """Cold time (ms) of every feasible candidate; infeasible ones are absent."""
     f = features(A) if f is None else f
+    fp64 = x.dtype == torch.float64
     out = {}
     for k, p in candidates:
-        if not feasible(k, f):
+        if not feasible(k, f, fp64):
             continue
         plan = Plan(A, k, p, xtype=x.dtype)
         time_plan_cold(plan, x, y, 1, flusher)  # warm the code path
@@ -329,8 +338,8 @@ def library_pick(A, xtype=torch.float32) -> str:
     return f"{_lib.KERNEL_NAMES[k.value]}:{p.value}"
 
 
-def feasible_mask(f: np.ndarray) -> np.ndarray:
-    return np.array([1.0 if feasible(k, f) else 0.0 for k, _ in CANDIDATES], np.float32)
+def feasible_mask(f: np.ndarray, fp64: bool = False) -> np.ndarray:
+    return np.array([1.0 if feasible(k, f, fp64) else 0.0 for k, _ in CANDIDATES], np.float32)
 
 
 def targets_from_sweep(times: dict[str, float]) -> tuple[np.ndarray, np.ndarray]:

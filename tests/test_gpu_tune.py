@@ -158,7 +158,7 @@ def test_library_auto_pick_matches_python_selector():
                 f = tune.features(A)
                 for xt in (torch.float32, torch.float64):
                     P = sel.predict(tune.feature_vector(f, xt == torch.float64)[None])[0]
-                    P = np.where(tune.feasible_mask(f) > 0, P, np.inf)
+                    P = np.where(tune.feasible_mask(f, xt == torch.float64) > 0, P, np.inf)
                     name = tune.library_pick(A, xt)
                     assert name in tune.CAND_NAMES
                     assert P[tune.CAND_NAMES.index(name)] <= P.min() + 1e-4 * (1 + abs(P.min()))

@@ -171,6 +171,16 @@ def test_powerlaw_2m_precision(powerlaw_2m, vt):
           flush=True)
 
 
+@pytest.mark.parametrize("comp", [False, True])
+def test_powerlaw_2m_auto_plan_kernel(powerlaw_2m, comp):
+    """AUTO on the power-law matrix takes the LDS slab kernel, with Dot2 accumulation too (the compensated
+    variant keeps it, with half-width slabs, rather than falling back to a kernel that gathers from L2)."""
+    h, _, _ = powerlaw_2m
+    A = api.CSRMatrix(h.rowPtr, h.colIdx, torch.from_numpy(h.values), h.numCols)
+    info = api.Plan(A, "auto", 0, compensated=comp).info()
+    assert (info.kernel, info.param0) == ("lslab", 4), info
+
+
 # ---------------------------------------------------------------- learned selector spot-check
 def _shipped():
     st = json.loads((PKG / "selector_weights.json").read_text())
@@ -183,7 +193,7 @@ def _network_pick(st, rec, fp64):
     X = tune.feature_vector(f, fp64)[None, :].astype(np.float64)
     Xn = (X - np.array(st["mu"])) / np.array(st["sd"])
     Y, _ = omlp.forward(np.array(st["params"], np.float64), Xn, st["n_in"], st["hidden"], st["n_out"])
-    y = np.where(tune.feasible_mask(f) > 0, Y[0], np.inf)
+    y = np.where(tune.feasible_mask(f, fp64) > 0, Y[0], np.inf)
     return st["candidates"][int(np.argmin(y))]
 
 

@@ -57,7 +57,7 @@ def test_shipped_selector_regret_on_held_out_matrices(precision):
     learned = []
     for r, y in zip(test, Y):
         f = np.array(r["features"])
-        y = np.where(tune.feasible_mask(f) > 0, y, np.inf)
+        y = np.where(tune.feasible_mask(f, fp64) > 0, y, np.inf)
         learned.append(regret(r, names[int(np.argmin(y))]))
     learned = np.array(learned)
     # the method without the learned part: the reference's fixed rule, and the best single configuration
