@@ -171,6 +171,13 @@ spmv_status_t muxi_spmv_merge_based_v2_f32(const muxi_csr_t* A, const float* x, 
                                            void* stream, bool useOptimized);
 spmv_status_t muxi_spmv_merge_based_v2_f64(const muxi_csr_t* A, const double* x, double* y,
                                            void* stream, bool useOptimized);
+/* No counterpart in the reference: muxi_spmv_merge_based_f32 for callers that cannot allocate (code under
+ * stream capture, serving loops, harnesses that forbid hipMalloc). Same kernels and result bit for bit; its
+ * scratch (partition diagonals, row coordinates, carries) is a static 2 MiB device array per GPU, so
+ * numRows + nnz may be at most about 357 million (SPMV_ERROR_INVALID_MATRIX beyond, with the limit in
+ * muxi_spmv_last_error). All calls on one GPU share that array: calls that overlap on a GPU (two streams
+ * or two threads) race; calls on one stream are ordered. */
+spmv_status_t muxi_spmv_merge_based_noalloc_f32(const muxi_csr_t* A, const float* x, float* y, void* stream);
 /* spmv_sparse_optimized (src/spmv/csr/spmv_sparse_optimized.cu:80-121): zeroes y when nnz==0. */
 spmv_status_t muxi_spmv_sparse_optimized_f32(const muxi_csr_t* A, const float* x, float* y, void* stream);
 spmv_status_t muxi_spmv_sparse_optimized_f64(const muxi_csr_t* A, const double* x, double* y, void* stream);

@@ -108,6 +108,7 @@ SIGNATURES: dict[str, tuple] = {
     "muxi_spmv_merge_based_f64": (_I, [_P_CSR, _VP, _VP, _VP]),
     "muxi_spmv_merge_based_v2_f32": (_I, [_P_CSR, _VP, _VP, _VP, C.c_bool]),
     "muxi_spmv_merge_based_v2_f64": (_I, [_P_CSR, _VP, _VP, _VP, C.c_bool]),
+    "muxi_spmv_merge_based_noalloc_f32": (_I, [_P_CSR, _VP, _VP, _VP]),
     "muxi_spmv_sparse_optimized_f32": (_I, [_P_CSR, _VP, _VP, _VP]),
     "muxi_spmv_sparse_optimized_f64": (_I, [_P_CSR, _VP, _VP, _VP]),
     "muxi_merge_partition_count": (_I, [_I, _I, _I, _I, _I]),
