@@ -109,24 +109,6 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
 
 
 CPP_TESTS = [ROOT / "tests" / "cpp" / "test_spmv_known_answer.cpp", ROOT / "tests" / "cpp" / "test_formats_io.cpp"]
-PROBES = [PKG / "probe" / "gather_probe.hip", PKG / "probe" / "stream_probe.hip"]
-
-
-def build_probes(verbose: bool = True) -> list[Path]:
-    """Stand-alone measurement programs (the random-gather rate that bounds the scattered-column SpMV, the HBM
-    streaming rate that bounds the stencil SpMV)."""
-    out = []
-    for src in PROBES:
-        exe = src.with_suffix("")
-        cmd = [HIPCC, "-O3", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-               str(src), "-o", str(exe)]
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            raise RuntimeError(f"compile failed: {src.name}\n{r.stdout}\n{r.stderr}")
-        out.append(exe)
-    if verbose:
-        print(f"built {len(out)} probe(s)", flush=True)
-    return out
 
 
 def build_cpp_tests(verbose: bool = True) -> list[Path]:
@@ -151,7 +133,6 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     build(a.jobs)
     build_cpp_tests()
-    build_probes()
     return 0
 
 

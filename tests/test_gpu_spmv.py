@@ -1,7 +1,9 @@
 """GPU parity: every kernel and reference entry point through the C-ABI
 against the CPU oracle. Tolerances: integer outputs bit-exact; FP32 results
 within the per-row bound (n_r + 2) * 2^-24 * sum_j |a_rj x_j| of the FP64
-oracle and a relative L2 error <= 1e-6; FP64 results within
+oracle and a relative L2 error <= 1e-5 in general (one 60,000-term row of a
+small matrix can dominate the norm with a legitimately larger relative
+error), <= 1e-6 on the benchmark workloads; FP64 results within
 (n_r + 2) * 2^-53 * sum_j |a_rj x_j|."""
 import numpy as np
 import pytest
