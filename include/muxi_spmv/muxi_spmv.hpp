@@ -20,6 +20,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdlib>
 #include <type_traits>
 
@@ -107,19 +108,93 @@ struct CSRMatrix {
     }
 };
 
+// The reference's public fields in its order (sparse_formats.h:284-315), laid out as muxi_csr5_t (checked
+// below), so reference callers read csr5.numTiles, csr5.tile_row_ptr[t], ... directly.
 template <typename FloatType>
 struct CSR5Matrix {
-    ::muxi_csr5_t c{};
-    CSR5Matrix() { c.sigma = 256; }
-    CSR5Matrix(const CSR5Matrix&) = delete;
+    int numRows = 0;
+    int numCols = 0;
+    int nnz = 0;
+    int* rowPtr = nullptr;
+    int* colIdx = nullptr;
+    FloatType* values = nullptr;
+    int sigma = 256;
+    int numTiles = 0;
+    int* tile_row_ptr = nullptr;     // host, numTiles (filled by csr5_preprocess)
+    int* tile_nnz_offset = nullptr;  // host, numTiles
+    int* d_rowPtr = nullptr;
+    int* d_colIdx = nullptr;
+    FloatType* d_values = nullptr;
+    int* d_tile_row_ptr = nullptr;
+    int* d_tile_nnz_offset = nullptr;
+    bool ownsCsrData = false;
+    bool ownsTileData = false;
+    bool ownsDeviceMemory = false;
+    double conversionTimeMs = 0.0;
+
+    CSR5Matrix() = default;
+    CSR5Matrix(const CSR5Matrix&) = delete;  // the reference's implicit copy double-frees
     CSR5Matrix& operator=(const CSR5Matrix&) = delete;
-    ~CSR5Matrix() { ::muxi_csr5_destroy(&c); }
-    int sigma() const { return c.sigma; }
-    int numTiles() const { return c.numTiles; }
-    const int* tile_row_ptr() const { return c.tile_row_ptr; }
-    const int* tile_nnz_offset() const { return c.tile_nnz_offset; }
-    double conversionTimeMs() const { return c.conversionTimeMs; }
+    ~CSR5Matrix() { ::muxi_csr5_destroy(abi()); }
+
+    // Reference CSR data from an existing CSRMatrix (no copy), as the reference's referenceFromCSR.
+    void referenceFromCSR(const CSRMatrix<FloatType>& csr) {
+        numRows = csr.numRows; numCols = csr.numCols; nnz = csr.nnz;
+        rowPtr = csr.rowPtr; colIdx = csr.colIdx; values = csr.values;
+        d_rowPtr = csr.d_rowPtr; d_colIdx = csr.d_colIdx; d_values = csr.d_values;
+        ownsCsrData = false;
+    }
+    void allocateTileMetadata(int tileCount) {
+        numTiles = tileCount;
+        tile_row_ptr = (int*)std::malloc((size_t)numTiles * sizeof(int));
+        tile_nnz_offset = (int*)std::malloc((size_t)numTiles * sizeof(int));
+        ownsTileData = true;
+    }
+    void allocateDeviceTileMetadata() {
+        (void)hipMalloc(&d_tile_row_ptr, (size_t)numTiles * sizeof(int));
+        (void)hipMalloc(&d_tile_nnz_offset, (size_t)numTiles * sizeof(int));
+        ownsDeviceMemory = true;
+    }
+    void copyTileMetadataToDevice(hipStream_t stream = 0) {
+        (void)hipMemcpyAsync(d_tile_row_ptr, tile_row_ptr, (size_t)numTiles * sizeof(int), hipMemcpyHostToDevice,
+                             stream);
+        (void)hipMemcpyAsync(d_tile_nnz_offset, tile_nnz_offset, (size_t)numTiles * sizeof(int),
+                             hipMemcpyHostToDevice, stream);
+    }
+
+    // The C-ABI view: this object itself.
+    ::muxi_csr5_t* abi() { return reinterpret_cast<::muxi_csr5_t*>(this); }
+    const ::muxi_csr5_t* abi() const { return reinterpret_cast<const ::muxi_csr5_t*>(this); }
 };
+
+namespace detail {
+template <typename T>
+constexpr bool csr5_layout_matches() {
+    using M = CSR5Matrix<T>;
+    return std::is_standard_layout<M>::value && sizeof(M) == sizeof(::muxi_csr5_t) &&
+           offsetof(M, numRows) == offsetof(::muxi_csr5_t, numRows) &&
+           offsetof(M, numCols) == offsetof(::muxi_csr5_t, numCols) &&
+           offsetof(M, nnz) == offsetof(::muxi_csr5_t, nnz) &&
+           offsetof(M, rowPtr) == offsetof(::muxi_csr5_t, rowPtr) &&
+           offsetof(M, colIdx) == offsetof(::muxi_csr5_t, colIdx) &&
+           offsetof(M, values) == offsetof(::muxi_csr5_t, values) &&
+           offsetof(M, sigma) == offsetof(::muxi_csr5_t, sigma) &&
+           offsetof(M, numTiles) == offsetof(::muxi_csr5_t, numTiles) &&
+           offsetof(M, tile_row_ptr) == offsetof(::muxi_csr5_t, tile_row_ptr) &&
+           offsetof(M, tile_nnz_offset) == offsetof(::muxi_csr5_t, tile_nnz_offset) &&
+           offsetof(M, d_rowPtr) == offsetof(::muxi_csr5_t, d_rowPtr) &&
+           offsetof(M, d_colIdx) == offsetof(::muxi_csr5_t, d_colIdx) &&
+           offsetof(M, d_values) == offsetof(::muxi_csr5_t, d_values) &&
+           offsetof(M, d_tile_row_ptr) == offsetof(::muxi_csr5_t, d_tile_row_ptr) &&
+           offsetof(M, d_tile_nnz_offset) == offsetof(::muxi_csr5_t, d_tile_nnz_offset) &&
+           offsetof(M, ownsCsrData) == offsetof(::muxi_csr5_t, ownsCsrData) &&
+           offsetof(M, ownsTileData) == offsetof(::muxi_csr5_t, ownsTileData) &&
+           offsetof(M, ownsDeviceMemory) == offsetof(::muxi_csr5_t, ownsDeviceMemory) &&
+           offsetof(M, conversionTimeMs) == offsetof(::muxi_csr5_t, conversionTimeMs);
+}
+}  // namespace detail
+static_assert(detail::csr5_layout_matches<float>(), "CSR5Matrix<float> must be laid out as muxi_csr5_t");
+static_assert(detail::csr5_layout_matches<double>(), "CSR5Matrix<double> must be laid out as muxi_csr5_t");
 
 template <typename FloatType>
 struct ELLPACKMatrix {
@@ -170,13 +245,13 @@ inline spmv_status_t spmv_sparse_optimized(const CSRMatrix<T>& A, const T* x, T*
 template <typename T>
 inline spmv_status_t csr5_preprocess(const CSRMatrix<T>& csr, CSR5Matrix<T>& csr5, int sigma, hipStream_t stream) {
     ::muxi_csr_t c = csr.abi();
-    return ::muxi_csr5_preprocess(&c, &csr5.c, sigma, stream);
+    return ::muxi_csr5_preprocess(&c, csr5.abi(), sigma, stream);
 }
 
 template <typename T>
 inline spmv_status_t spmv_csr5(const CSR5Matrix<T>& A, const T* x, T* y, T alpha, T beta, const spmv_opts_t& opts) {
-    if constexpr (std::is_same<T, float>::value) return ::muxi_spmv_csr5_f32(&A.c, x, y, alpha, beta, &opts);
-    else return ::muxi_spmv_csr5_f64(&A.c, x, y, alpha, beta, &opts);
+    if constexpr (std::is_same<T, float>::value) return ::muxi_spmv_csr5_f32(A.abi(), x, y, alpha, beta, &opts);
+    else return ::muxi_spmv_csr5_f64(A.abi(), x, y, alpha, beta, &opts);
 }
 
 template <typename T>
@@ -209,7 +284,7 @@ inline spmv_status_t spmv_exec(spmv_handle_t* h, spmv_format_t format, const voi
     ::muxi_csr_t c;
     const void* m = matrix;
     if (format == SPMV_FORMAT_CSR) { c = static_cast<const CSRMatrix<T>*>(matrix)->abi(); m = &c; }
-    else if (format == SPMV_FORMAT_CSR5) m = &static_cast<const CSR5Matrix<T>*>(matrix)->c;
+    else if (format == SPMV_FORMAT_CSR5) m = static_cast<const CSR5Matrix<T>*>(matrix)->abi();
     if constexpr (std::is_same<T, float>::value) return ::muxi_spmv_exec_f32(h, format, m, x, y, alpha, beta);
     else return ::muxi_spmv_exec_f64(h, format, m, x, y, alpha, beta);
 }
@@ -220,7 +295,7 @@ inline spmv_status_t spmv_exec_transpose(spmv_handle_t* h, spmv_format_t format,
     ::muxi_csr_t c;
     const void* m = matrix;
     if (format == SPMV_FORMAT_CSR) { c = static_cast<const CSRMatrix<T>*>(matrix)->abi(); m = &c; }
-    else if (format == SPMV_FORMAT_CSR5) m = &static_cast<const CSR5Matrix<T>*>(matrix)->c;
+    else if (format == SPMV_FORMAT_CSR5) m = static_cast<const CSR5Matrix<T>*>(matrix)->abi();
     if constexpr (std::is_same<T, float>::value) return ::muxi_spmv_exec_transpose_f32(h, format, m, x, y, alpha, beta);
     else return ::muxi_spmv_exec_transpose_f64(h, format, m, x, y, alpha, beta);
 }

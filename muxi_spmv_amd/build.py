@@ -108,7 +108,8 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
     return LIB
 
 
-CPP_TESTS = [ROOT / "tests" / "cpp" / "test_spmv_known_answer.cpp", ROOT / "tests" / "cpp" / "test_formats_io.cpp"]
+CPP_TESTS = [ROOT / "tests" / "cpp" / "test_spmv_known_answer.cpp", ROOT / "tests" / "cpp" / "test_formats_io.cpp",
+             ROOT / "tests" / "cpp" / "test_csr5_fields.cpp"]
 
 
 def build_cpp_tests(verbose: bool = True) -> list[Path]:

@@ -60,7 +60,7 @@ int main(int argc, char** argv) {
     if (spmv_csr5<float>(A5, dx, dy, 1.0f, 0.0f, opts) != SPMV_SUCCESS) return 4;
     (void)hipMemcpy(hy, dy, n * sizeof(float), hipMemcpyDeviceToHost);
     ok &= verify(hy, hx, n);
-    ok &= A5.numTiles() == (n + 255) / 256 && A5.tile_row_ptr()[A5.numTiles() - 1] == (A5.numTiles() - 1) * 256;
+    ok &= A5.numTiles == (n + 255) / 256 && A5.tile_row_ptr[A5.numTiles - 1] == (A5.numTiles - 1) * 256;
 
     ELLPACKMatrix<float> E;
     if (csr_to_ellpack<float>(A, E) != SPMV_SUCCESS) return 5;

@@ -292,6 +292,20 @@ def test_cpp_reference_caller():
     assert "PASSED" in r.stdout
 
 
+def test_cpp_csr5_fields_caller():
+    """A C++ caller that reads CSR5Matrix<float>'s public fields (numTiles, sigma, tile_row_ptr[t],
+    tile_nnz_offset[t]) as the reference's benchmark does, checks every tile against its definition and y exactly
+    against a host loop on integer-valued data."""
+    import subprocess
+    from pathlib import Path
+
+    exe = Path(__file__).resolve().parent / "cpp" / "test_csr5_fields"
+    assert exe.exists(), "build with python -m muxi_spmv_amd.build"
+    r = subprocess.run([str(exe), "50000"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "PASSED" in r.stdout
+
+
 # ---------------------------------------------------------------- full-size properties
 @pytest.fixture(scope="module")
 def powerlaw_full():
