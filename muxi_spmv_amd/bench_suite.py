@@ -1,6 +1,6 @@
 """Secondary benchmark workloads (one JSON line each on stdout).
 
-python -m muxi_spmv_amd.bench_suite precision|batched|spmm|laplacian [--out FILE]
+python -m muxi_spmv_amd.bench_suite precision|batched|spmm|laplacian|shard|host [--out FILE] [--results FILE]
 
 precision  values stored FP64 / FP32 / BF16, x FP32, FP32 accumulation plain
            and compensated (Dot2), on the 2M power-law matrix and on a
@@ -23,6 +23,13 @@ host       x and y in host memory (muxi_spmv_host.h): pinned and pageable buffer
            host plan's own events (median of 10) next to the device-resident plan.
 All timings except host's are cold: each timed launch is preceded by a 512 MiB cache flush,
 and HIP events bracket the launch.
+
+Every run also writes a results file (--results, default spmv_results_<workload>.json) in the reference's
+results layout (writeResultsJSON, src/benchmark/performance_benchmark.cu:298-335): one entry per measured
+configuration with matrix, device, performance {timeMs (median), min/max/avg/stdDev, gflops, bandwidthGBs,
+bandwidthUtilization} and correctnessPassed (rel L2 against the FP64 oracle within the stated tolerance),
+extended with the errors, the HBM roofline fraction and a CPU time for the same product (scipy.sparse CSR, one
+thread) -- see benchmark.performance_result.
 """
 from __future__ import annotations
 
@@ -39,7 +46,7 @@ import torch
 
 from . import api, host, prep, tune
 from . import generators as G
-from .benchmark import HBM_PEAK_GBS, algorithmic_bytes
+from .benchmark import HBM_PEAK_GBS, algorithmic_bytes, device_record, performance_result, write_results_json
 from ._lib import check, load
 
 
@@ -55,7 +62,7 @@ class Timer:
         self.fl = tune.Flusher()
 
     def time(self, fn, reps=10, flush=True) -> float:
-        """Median ms of fn() over reps, each preceded by a cache flush."""
+        """Median ms of fn() over reps, each preceded by a cache flush; every rep's ms stays in self.last."""
         s = torch.cuda.current_stream().cuda_stream or None
         lib = load()
         fn()
@@ -69,13 +76,40 @@ class Timer:
             ms = C.c_float()
             check(lib.muxi_event_elapsed_ms(self.ev[0], self.ev[1], C.byref(ms)), "elapsed")
             ts.append(ms.value)
+        self.last = ts
         return statistics.median(ts)
 
 
 def errors(y: torch.Tensor, y64: torch.Tensor) -> dict:
     d = y.double() - y64
-    return {"rel_l2": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(y64)),
-            "max_abs": float(d.abs().max()), "max_abs_y": float(y64.abs().max())}
+    nrm = float(torch.linalg.vector_norm(y64))
+    return {"rel_l2": float(torch.linalg.vector_norm(d)) / nrm if nrm > 0 else float(torch.linalg.vector_norm(d)),
+            "max_abs": float(d.abs().max()) if d.numel() else 0.0,
+            "max_abs_y": float(y64.abs().max()) if d.numel() else 0.0}
+
+
+TOL_FP32 = 1e-6   # rel L2 of an FP32-accumulated y against the FP64 oracle on the same stored values
+TOL_FP64 = 1e-12  # FP64 accumulation
+
+
+def cpu_ms(rp, ci, va, X, reps=3) -> dict:
+    """The same product on the host: scipy.sparse CSR times x (or X), one thread, median of reps."""
+    import scipy.sparse as sp
+
+    rp = np.asarray(rp)
+    A = sp.csr_matrix((np.asarray(va), np.asarray(ci), rp), shape=(rp.size - 1, int(np.asarray(X).shape[0])))
+    X = np.asarray(X)
+    A @ X
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        A @ X
+        ts.append((time.perf_counter() - t) * 1e3)
+    return {"timeMs": statistics.median(ts), "kind": "scipy.sparse CSR product", "threads": 1}
+
+
+def _dims(A) -> tuple[int, int, int]:
+    return int(A.numRows), int(A.numCols), int(A.nnz)
 
 
 def precision(args) -> list[dict]:
@@ -94,28 +128,39 @@ def precision(args) -> list[dict]:
     log(f"banded 4M: RCM + permutation {time.perf_counter() - t:.1f}s, bandwidth {prep.bandwidth(rp, ci)} -> "
         f"{prep.bandwidth(nrp, nci)}")
     mats["banded_4m_rcm"] = (nrp, nci, nva, 4_000_000)
+    dev = device_record()
     for name, (rp, ci, va64, ncol) in mats.items():
-        x = torch.from_numpy(G.uniform_vector(ncol, 99)).cuda()
+        xh = G.uniform_vector(ncol, 99)
+        x = torch.from_numpy(xh).cuda()
         A64 = api.CSRMatrix(rp, ci, torch.from_numpy(va64), ncol)
         y64 = api.oracle_f64(A64, x)  # FP64 values, FP32 x, exact-ish FP64 sum
+        cpu = cpu_ms(rp, ci, va64.astype(np.float32), xh)
         for vt in (torch.float64, torch.float32, torch.bfloat16):
             Av = A64.with_values(api.convert_values(A64.d_values, vt))
-            for kernel, param in (("merge", 4), ("xslab", 0), ("ell", 0), ("auto", 0)):
+            y64s = api.oracle_f64(Av, x)  # the oracle on the values as stored
+            for kernel, param in (("merge", 4), ("xslab", 0), ("lslab", 4), ("ell", 0), ("auto", 0)):
                 if kernel == "ell" and name != "banded_4m_rcm":
                     continue  # ELL pads every row of the power-law matrix to its longest
                 for comp in (False, True):
-                    if kernel in ("xslab", "auto") and comp:
-                        continue
+                    if kernel == "xslab" and comp:
+                        continue  # no Dot2 variant
                     p = api.Plan(Av, kernel, param, compensated=comp)
+                    info = p.info()
                     y = torch.empty(Av.numRows, device="cuda")
                     ms = tm.time(lambda: p.exec(x, y))
                     bv = {torch.float64: 8, torch.float32: 4, torch.bfloat16: 2}[vt]
                     b = algorithmic_bytes(Av.numRows, ncol, Av.nnz, bv, 4)
+                    e, es = errors(y, y64), errors(y, y64s)
                     rec = {"workload": name, "values": str(vt).split(".")[-1], "kernel": f"{kernel}:{param}",
-                           "accumulation": "fp32_dot2" if comp else "fp32", "ms": round(ms, 5),
-                           "GBps": round(b / ms / 1e6, 1), **errors(y, y64)}
+                           "runs": f"{info.kernel}:{info.param0}", "accumulation": "fp32_dot2" if comp else "fp32",
+                           "ms": round(ms, 5), "GBps": round(b / ms / 1e6, 1), **e,
+                           "rel_l2_vs_stored_values": es["rel_l2"]}
                     log(json.dumps(rec))
                     out.append(rec)
+                    args.results.append(performance_result(
+                        *_dims(Av), tm.last, b, rel_l2=es["rel_l2"], max_abs=es["max_abs"], tolerance=TOL_FP32,
+                        device=dev, cpu=cpu, workload=name, values=rec["values"], kernel=rec["runs"],
+                        accumulation=rec["accumulation"], relL2VsFp64Values=e["rel_l2"]))
                     p.destroy()
     return out
 
@@ -149,24 +194,47 @@ def batched(args) -> list[dict]:
         bp.destroy()
     best_ipt = min(ipt_ms, key=ipt_ms.get)
     bp = BatchPlan(mats, xs, ys, best_ipt)
+    times = {}
+
+    def worst_err():
+        w, m = 0.0, 0.0
+        for A, x, y in zip(mats, xs, ys):
+            e = errors(y, api.oracle_f64(A, x))
+            w, m = max(w, e["rel_l2"]), max(m, e["max_abs"])
+        return w, m
+
     with torch.cuda.stream(s):
         g_batch = bp.graph(stream=s)
         g_vec = bp.graph("vector", 0, stream=s)
         t_batch = tm.time(lambda: bp.exec())
+        times["batched_persistent"] = tm.last
         t_batch_g = tm.time(lambda: g_batch.launch(s))
-        t_vec = tm.time(lambda: bp.exec_per_matrix("vector", 0))
-        t_vec_g = tm.time(lambda: g_vec.launch(s))
+        times["batched_persistent_graph"] = tm.last
         # correctness of the batched result
         bp.exec()
-        worst = 0.0
-        for A, x, y in zip(mats, xs, ys):
-            y64 = api.oracle_f64(A, x)
-            worst = max(worst, errors(y, y64)["rel_l2"])
+        worst, worst_abs = worst_err()
+        t_vec = tm.time(lambda: bp.exec_per_matrix("vector", 0))
+        times["per_matrix_vector"] = tm.last
+        t_vec_g = tm.time(lambda: g_vec.launch(s))
+        times["per_matrix_vector_graph"] = tm.last
+        bp.exec_per_matrix("vector", 0)
+        worst_v, worst_v_abs = worst_err()
     torch.cuda.synchronize()
+    cpu_t = 0.0
+    for A, x in zip(mats, xs):
+        rp, ci, va = A.host()
+        cpu_t += cpu_ms(rp, ci, va, x.cpu().numpy(), reps=1)["timeMs"]
+    cpu = {"timeMs": cpu_t, "kind": "scipy.sparse CSR product per matrix, summed", "threads": 1}
+    dev = device_record()
+    R_all, C_all = sum(m.numRows for m in mats), sum(m.numCols for m in mats)
     for name, ms in [("batched_persistent", t_batch), ("batched_persistent_graph", t_batch_g),
                      ("per_matrix_vector", t_vec), ("per_matrix_vector_graph", t_vec_g)]:
         out.append({"workload": "batched_256", "path": name, "ms": round(ms, 4), "GBps": round(tot_b / ms / 1e6, 1),
                     "nnz": nnz})
+        w, wa = (worst, worst_abs) if name.startswith("batched") else (worst_v, worst_v_abs)
+        args.results.append(performance_result(R_all, C_all, nnz, times[name], tot_b, rel_l2=w, max_abs=wa,
+                                               tolerance=TOL_FP32, device=dev, cpu=cpu, workload="batched_256",
+                                               path=name, matrices=len(mats), errors_are="worst member"))
     out.append({"workload": "batched_256", "speedup_graph_vs_per_matrix_graph": round(t_vec_g / t_batch_g, 2),
                 "speedup_eager": round(t_vec / t_batch, 2), "worst_rel_l2": worst, "info": bp.info(),
                 "items_per_lane": best_ipt, "ms_by_items_per_lane": {str(k): round(v, 4) for k, v in ipt_ms.items()}})
@@ -180,9 +248,12 @@ def spmm(args) -> list[dict]:
     A = api.CSRMatrix(h.rowPtr, h.colIdx, torch.from_numpy(h.values), h.numCols)
     tm = Timer()
     out = []
+    dev = device_record()
     for k in (8, 32):
         X = torch.rand((h.numCols, k), device="cuda") * 2 - 1
         Y = torch.empty((h.numRows, k), device="cuda")
+        y64 = api.oracle_f64(A, X[:, 3].contiguous())
+        cpu = cpu_ms(h.rowPtr, h.colIdx, h.values, X.cpu().numpy(), reps=1)
         best = None
         for ipt in (4, 8, 16):
             p = api.Plan.spmm(A, k, ipt)
@@ -191,6 +262,11 @@ def spmm(args) -> list[dict]:
             rec = {"workload": "spmm_powerlaw_2m", "k": k, "ipt": ipt, "ms": round(ms, 4),
                    "GBps": round(b / ms / 1e6, 1), "GFLOPs": round(2 * h.nnz * k / ms / 1e6, 1)}
             log(json.dumps(rec))
+            e = errors(Y[:, 3], y64)
+            args.results.append(performance_result(h.numRows, h.numCols, h.nnz, tm.last, b, k=k, rel_l2=e["rel_l2"],
+                                                   max_abs=e["max_abs"], tolerance=TOL_FP32, device=dev, cpu=cpu,
+                                                   workload="spmm_powerlaw_2m", kernel=f"spmm:{ipt}", rhs=k,
+                                                   errors_are="column 3"))
             if best is None or ms < best["ms"]:
                 best = rec
         p = api.Plan.spmm(A, k, best["ipt"])
@@ -209,6 +285,9 @@ def laplacian(args) -> list[dict]:
     tm = Timer()
     b = algorithmic_bytes(h.numRows, h.numCols, h.nnz, 8, 8)
     out = []
+    dev = device_record()
+    y64 = api.oracle_f64(A, x)
+    cpu = cpu_ms(h.rowPtr, h.colIdx, h.values, x.cpu().numpy(), reps=1)
     for k, p in [("ell", 0), ("adaptive", 512), ("adaptive", 1024), ("vwave", 2), ("vwave", 4), ("csr5", 512),
                  ("merge", 4), ("xslab", 0), ("auto", 0)]:
         pl = api.Plan(A, k, p, xtype=torch.float64)
@@ -218,6 +297,10 @@ def laplacian(args) -> list[dict]:
                "ms": round(ms, 4), "GBps": round(b / ms / 1e6, 1), "frac_hbm": round(b / ms / 1e6 / HBM_PEAK_GBS, 4)}
         log(json.dumps(rec))
         out.append(rec)
+        e = errors(y, y64)
+        args.results.append(performance_result(h.numRows, h.numCols, h.nnz, tm.last, b, rel_l2=e["rel_l2"],
+                                               max_abs=e["max_abs"], tolerance=TOL_FP64, device=dev, cpu=cpu,
+                                               workload="laplacian_4096_fp64", kernel=rec["runs"]))
         pl.destroy()
     return out
 
@@ -250,6 +333,15 @@ def shard(args) -> list[dict]:
             nb = algorithmic_bytes(R, h.numCols, A.nnz, esz, esz)
             info = p.info()
             res[part] = {"kernel": f"{info.kernel}:{info.param0}", "ms": round(ms, 4), "GBps": round(nb / ms / 1e6, 1)}
+            e = errors(y, api.oracle_f64(A, x))
+            if part == "shard":
+                cpu = cpu_ms(rp, h.colIdx[n0:n1], h.values[n0:n1], x.cpu().numpy(), reps=1)
+            else:
+                cpu = cpu_ms(h.rowPtr, h.colIdx, h.values, x.cpu().numpy(), reps=1)
+            args.results.append(performance_result(
+                R, h.numCols, A.nnz, tm.last, nb, rel_l2=e["rel_l2"], max_abs=e["max_abs"],
+                tolerance=TOL_FP64 if dt == torch.float64 else TOL_FP32, device=device_record(), cpu=cpu,
+                workload=name, part=part, kernel=res[part]["kernel"], gpus=G_, shard_index=g))
             p.destroy()
         res["speedup_8_gpus_if_x_replicated"] = round(res["whole"]["ms"] / res["shard"]["ms"], 2)
         log(json.dumps(res))
@@ -271,6 +363,8 @@ def host_path(args) -> list[dict]:
         xd = torch.from_numpy(xs).cuda()
         yd = torch.empty(h.numRows, dtype=dt, device="cuda")
         dev_ms = tm.time(lambda: p.exec(xd, yd), flush=False)
+        y64 = api.oracle_f64(A, xd)
+        cpu = cpu_ms(h.rowPtr, h.colIdx, h.values, xs, reps=1)
         info = p.info()
         p.destroy()
         base = {"workload": name, "device_resident_ms": round(dev_ms, 4), "device_kernel": f"{info.kernel}:{info.param0}",
@@ -290,6 +384,12 @@ def host_path(args) -> list[dict]:
                     hp.exec(x, y)
                     ts.append(hp.timing())
                 t = sorted(ts, key=lambda v: v.total_ms)[len(ts) // 2]
+                e = errors(torch.from_numpy(np.asarray(y)).cuda(), y64)
+                args.results.append(performance_result(
+                    h.numRows, h.numCols, h.nnz, [v.total_ms for v in ts], nb, rel_l2=e["rel_l2"],
+                    max_abs=e["max_abs"], tolerance=TOL_FP64 if dt == torch.float64 else TOL_FP32,
+                    device=device_record(), cpu=cpu, workload=name, path="host_buffers", pinned=pinned,
+                    chunks=chunks, timing="end to end, host x in and host y out (PCIe included)"))
                 rec = dict(base, pinned=pinned, chunks=chunks, total_ms=round(t.total_ms, 4),
                            first_copy_in_ms=round(t.first_copy_in_ms, 4), kernels_ms=round(t.kernels_ms, 4),
                            copy_out_tail_ms=round(t.copy_out_tail_ms, 4),
@@ -304,8 +404,11 @@ def host_path(args) -> list[dict]:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("what", choices=["precision", "batched", "spmm", "laplacian", "shard", "host"])
-    ap.add_argument("--out", default=None)
+    ap.add_argument("--out", default=None, help="the JSON lines printed on stdout, also to this file")
+    ap.add_argument("--results", dest="results_file", default=None,
+                    help="results file (default spmv_results_<workload>.json)")
     a = ap.parse_args(argv)
+    a.results = []  # performance_result entries, filled by the workload
     torch.cuda.set_device(0)
     res = {"precision": precision, "batched": batched, "spmm": spmm, "laplacian": laplacian, "shard": shard,
            "host": host_path}[a.what](a)
@@ -314,7 +417,15 @@ def main(argv=None) -> int:
     if a.out:
         with open(a.out, "w") as f:
             f.write(txt + "\n")
+    path = results_path(a.what, a.results_file)
+    write_results_json(path, a.results)
+    log(f"{len(a.results)} results -> {path}")
     return 0
+
+
+def results_path(what: str, given: str | None = None) -> str:
+    """The results file of a run: the one given, else spmv_results_<workload>.json in the working directory."""
+    return given or f"spmv_results_{what}.json"
 
 
 if __name__ == "__main__":

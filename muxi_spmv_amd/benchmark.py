@@ -9,12 +9,16 @@ Counterpart of src/benchmark/performance_benchmark.{h,cu}:
                         warmups, then timed launches; here every launch reads a
                         different copy of the matrix so that no launch finds its
                         data in L2 or the 256 MiB Infinity Cache ("cold").
-  write_results_json    writeResultsJSON (:298-335) schema, extended.
+  time_stats            calculateStats (:171-185) + the median as timeMs
+  performance_result    PerformanceResult (performance_benchmark.h:21-53) as one results entry
+  write_results_json    writeResultsJSON (:298-335) layout, extended and validated
+                        (validate_results)
 """
 from __future__ import annotations
 
 import ctypes as C
 import json
+import math
 import statistics
 import time
 from dataclasses import asdict, dataclass, field
@@ -139,10 +143,90 @@ def measure_plans(plans: list[Plan], xs: list[torch.Tensor], ys: list[torch.Tens
     return m
 
 
+def time_stats(times_ms: list[float]) -> dict:
+    """minTimeMs / maxTimeMs / avgTimeMs / stdDevMs as calculateStats (performance_benchmark.cu:171-185; population
+    standard deviation) and timeMs = the median, as measureCSRPerformance reports it (:135)."""
+    ts = sorted(float(t) for t in times_ms)
+    if not ts:
+        raise ValueError("no timings")
+    avg = statistics.fmean(ts)
+    return {"timeMs": ts[len(ts) // 2], "minTimeMs": ts[0], "maxTimeMs": ts[-1], "avgTimeMs": avg,
+            "stdDevMs": statistics.pstdev(ts) if len(ts) > 1 else 0.0}
+
+
+def performance_result(rows: int, cols: int, nnz: int, times_ms: list[float], bytes_: int, *, k: int = 1,
+                       rel_l2: float | None = None, max_abs: float | None = None, tolerance: float | None = None,
+                       device: dict | None = None, cpu: dict | None = None, **extra) -> dict:
+    """One entry of the results file: the reference's PerformanceResult fields in its JSON layout
+    (writeResultsJSON, performance_benchmark.cu:298-335) -- matrix, device, performance, correctnessPassed --
+    extended with the error against the FP64 oracle, the fraction of the HBM roofline, the CPU time of the same
+    product where one was measured, and the workload's own keys (`extra`). `bytes_` is the algorithmic byte count
+    of one launch; correctnessPassed is rel_l2 <= tolerance (true when no check was made, as the reference does
+    when checkCorrectness is off)."""
+    st = time_stats(times_ms)
+    ms = st["timeMs"]
+    dev = device if device is not None else {"id": 0, "warpSize": 64, "peakBandwidthGBs": HBM_PEAK_GBS}
+    gbs = calculate_bandwidth(bytes_, ms)
+    perf = dict(st, gflops=calculate_gflops(nnz, ms, k), bandwidthGBs=gbs,
+                bandwidthUtilization=100.0 * gbs / float(dev["peakBandwidthGBs"]), iterations=len(times_ms))
+    passed = True if (rel_l2 is None or tolerance is None) else bool(rel_l2 <= tolerance)
+    rec = {"matrix": {"rows": int(rows), "cols": int(cols), "nnz": int(nnz),
+                      "sparsity": float(nnz) / (float(rows) * float(cols)) if rows and cols else 0.0},
+           "device": dev, "performance": perf, "correctnessPassed": passed,
+           "errors": {"relL2VsFp64": rel_l2, "maxAbsVsFp64": max_abs, "tolerance": tolerance},
+           "rooflineFrac": gbs / HBM_PEAK_GBS, "cpu": cpu}
+    rec.update(extra)
+    return rec
+
+
+_NUM = (int, float)
+_SCHEMA = {
+    "matrix": {"rows": int, "cols": int, "nnz": int, "sparsity": _NUM},
+    "device": {"id": int, "warpSize": int, "peakBandwidthGBs": _NUM},
+    "performance": {"timeMs": _NUM, "minTimeMs": _NUM, "maxTimeMs": _NUM, "avgTimeMs": _NUM, "stdDevMs": _NUM,
+                    "gflops": _NUM, "bandwidthGBs": _NUM, "bandwidthUtilization": _NUM},
+    "errors": {"relL2VsFp64": (*_NUM, type(None)), "maxAbsVsFp64": (*_NUM, type(None)),
+               "tolerance": (*_NUM, type(None))},
+}
+
+
+def validate_results(doc: dict) -> None:
+    """Raises ValueError naming the first entry and field that does not follow the results-file layout."""
+    if not isinstance(doc, dict) or not isinstance(doc.get("results"), list):
+        raise ValueError("results file: top level must be {\"results\": [...]}")
+    for i, r in enumerate(doc["results"]):
+        if not isinstance(r, dict):
+            raise ValueError(f"results[{i}]: not an object")
+        for sec, fields in _SCHEMA.items():
+            if not isinstance(r.get(sec), dict):
+                raise ValueError(f"results[{i}].{sec}: missing")
+            for f, t in fields.items():
+                v = r[sec].get(f, KeyError)
+                if v is KeyError:
+                    raise ValueError(f"results[{i}].{sec}.{f}: missing")
+                if isinstance(v, bool) or not isinstance(v, t):
+                    raise ValueError(f"results[{i}].{sec}.{f}: {type(v).__name__}")
+                if isinstance(v, float) and not math.isfinite(v):
+                    raise ValueError(f"results[{i}].{sec}.{f}: not finite")
+        if not isinstance(r.get("correctnessPassed"), bool):
+            raise ValueError(f"results[{i}].correctnessPassed: missing or not a bool")
+        if not isinstance(r.get("rooflineFrac"), _NUM) or isinstance(r.get("rooflineFrac"), bool):
+            raise ValueError(f"results[{i}].rooflineFrac: missing")
+        cpu = r.get("cpu")
+        if cpu is not None and not (isinstance(cpu, dict) and isinstance(cpu.get("timeMs"), _NUM)):
+            raise ValueError(f"results[{i}].cpu: needs timeMs")
+        p = r["performance"]
+        if not p["minTimeMs"] <= p["timeMs"] <= p["maxTimeMs"]:
+            raise ValueError(f"results[{i}].performance: timeMs outside [minTimeMs, maxTimeMs]")
+
+
 def write_results_json(path: str, results: list[dict]) -> None:
-    """{results:[{matrix{rows,cols,nnz,sparsity}, device{...}, performance{...}, correctnessPassed}]}"""
+    """{results:[{matrix{rows,cols,nnz,sparsity}, device{...}, performance{...}, correctnessPassed, ...}]},
+    validated before it is written."""
+    doc = {"results": results}
+    validate_results(doc)
     with open(path, "w") as f:
-        json.dump({"results": results}, f, indent=2)
+        json.dump(doc, f, indent=2)
 
 
 def device_record() -> dict:
