@@ -271,6 +271,23 @@ spmv_status_t muxi_plan_create(const muxi_csr_t* A, spmv_type_t vtype, spmv_type
 spmv_status_t muxi_plan_create_rows(const muxi_csr_t* A, spmv_type_t vtype, spmv_type_t xtype,
                                     muxi_kernel_t kernel, int param0, int compensated, void* stream,
                                     long long row_base, long long rows_full, muxi_plan** plan);
+/* Checked plan (no counterpart in the reference, whose FP64 path is the whole
+ * spmv_csr<double> dispatch, src/spmv/csr/spmv_csr.cu:164-249): FP32 x and y,
+ * values FLOAT/BF16/DOUBLE, FP32 (compensated = 0) or Dot2 (1) accumulation on
+ * the LDS column-slab kernels (half slabs: at most 4,194,304 columns). Each
+ * exec also sums |a_rj x_j| per row beside y and tests the row's a-posteriori
+ * error bound -- (n_r+2)u S_r for FP32 sums, 2u|y_r| + 2((n_r+2)u)^2 S_r for
+ * Dot2, u = 2^-24, S_r the sum of |products|, slightly inflated for the
+ * rounding of S_r itself -- against rtol * |y_r|; rows that fail are
+ * recomputed in FP64 (Dot2 in double) by one more kernel of the same exec
+ * (one workgroup per failing row) and their y written from that. rtol in
+ * (0, 1). For callers that need each row of y within a stated relative
+ * error. */
+spmv_status_t muxi_plan_create_checked(const muxi_csr_t* A, spmv_type_t vtype, int compensated, double rtol,
+                                       void* stream, muxi_plan** plan);
+/* The rows a checked plan recomputed in FP64 in its last exec on `stream`
+ * (waits for it): *count of them and up to cap row ids, in no fixed order. */
+spmv_status_t muxi_plan_checked_rows(const muxi_plan* plan, int* count, int* rows, int cap, void* stream);
 spmv_status_t muxi_plan_destroy(muxi_plan* plan);
 /* y = alpha*A*x + beta*y with nrhs right-hand sides: for nrhs > 1, X is
  * numCols x nrhs and Y numRows x nrhs, both row-major. */

@@ -127,6 +127,8 @@ SIGNATURES: dict[str, tuple] = {
     "muxi_plan_create_rows": (_I, [_P_CSR, _I, _I, _I, _I, _I, _VP, C.c_longlong, C.c_longlong, C.POINTER(_VP)]),
     "muxi_plan_destroy": (_I, [_VP]),
     "muxi_plan_create_spmm": (_I, [_P_CSR, _I, _I, _I, _I, _VP, C.POINTER(_VP)]),
+    "muxi_plan_create_checked": (_I, [_P_CSR, _I, _I, C.c_double, _VP, C.POINTER(_VP)]),
+    "muxi_plan_checked_rows": (_I, [_VP, C.POINTER(_I), C.POINTER(_I), _I, _VP]),
     "muxi_plan_exec": (_I, [_VP, _VP, _VP, _D, _D, _I, _VP]),
     "muxi_plan_info": (_I, [_VP, C.POINTER(_I), C.POINTER(_I), C.POINTER(_Z)]),
     "muxi_plan_exec_ev": (_I, [_VP, _VP, _VP, _D, _D, _I, _VP, _VP, _VP]),

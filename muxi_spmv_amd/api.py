@@ -367,6 +367,28 @@ class Plan:
                                            C.byref(self._p)), "plan_create_spmm")
         return self
 
+    @classmethod
+    def checked(cls, A: CSRMatrix, rtol: float, compensated: bool = False, stream=None) -> "Plan":
+        """Checked plan (muxi_plan_create_checked): FP32 x and y on the LDS column slabs; every row whose
+        a-posteriori FP32 (or Dot2) error bound exceeds rtol * |y_r| is recomputed in FP64 in the same exec."""
+        self = cls.__new__(cls)
+        self.A = A
+        self.xtype = torch.float32
+        self.nrhs = 1
+        self._p = C.c_void_p()
+        check(load().muxi_plan_create_checked(C.byref(A.c), A.vtype, 1 if compensated else 0, float(rtol),
+                                              _stream(stream), C.byref(self._p)), "plan_create_checked")
+        return self
+
+    def checked_rows(self, stream=None) -> np.ndarray:
+        """Ids of the rows the last exec recomputed in FP64 (sorted); waits for the stream."""
+        n = C.c_int()
+        cap = max(int(self.A.numRows), 1)
+        buf = (C.c_int * cap)()
+        check(load().muxi_plan_checked_rows(self._p, C.byref(n), buf, cap, _stream(stream)), "plan_checked_rows")
+        k = min(n.value, cap)
+        return np.sort(np.frombuffer(buf, dtype=np.int32, count=k).copy())
+
     def info(self) -> PlanInfo:
         k, p0, sb = C.c_int(), C.c_int(), C.c_size_t()
         check(load().muxi_plan_info(self._p, C.byref(k), C.byref(p0), C.byref(sb)), "plan_info")

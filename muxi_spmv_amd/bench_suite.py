@@ -6,7 +6,9 @@ precision  values stored FP64 / FP32 / BF16, x FP32, FP32 accumulation plain
            and compensated (Dot2), on the 2M power-law matrix and on a
            4M-row banded matrix (half-bandwidth 64, 16 nnz/row, N(0,1)) after
            RCM; rel-L2 and max-abs error of y against the FP64 oracle (the
-           FP64 values and x), plus time and GB/s for each.
+           FP64 values and x), plus time and GB/s for each; and checked
+           plans (FP32 values, rtol 1e-3, plain and Dot2): time, rows
+           recomputed in FP64 and the largest relative row error.
 batched    256 CSR matrices, rows log-uniform in [1e3, 2e5], mean nnz/row
            log-uniform in [4, 64], Poisson row lengths: one persistent
            launch (eager and hipGraph) against one CSR-vector launch per
@@ -161,6 +163,29 @@ def precision(args) -> list[dict]:
                         *_dims(Av), tm.last, b, rel_l2=es["rel_l2"], max_abs=es["max_abs"], tolerance=TOL_FP32,
                         device=dev, cpu=cpu, workload=name, values=rec["values"], kernel=rec["runs"],
                         accumulation=rec["accumulation"], relL2VsFp64Values=e["rel_l2"]))
+                    p.destroy()
+            if vt == torch.float32:  # checked plans: rows whose bound fails at rtol recomputed in FP64
+                for comp in (False, True):
+                    rtol = 1e-3
+                    p = api.Plan.checked(Av, rtol, compensated=comp)
+                    y = torch.empty(Av.numRows, device="cuda")
+                    ms = tm.time(lambda: p.exec(x, y))
+                    nre = int(p.checked_rows().size)
+                    info = p.info()
+                    b = algorithmic_bytes(Av.numRows, ncol, Av.nnz, 4, 4)
+                    e, es = errors(y, y64), errors(y, y64s)
+                    rel_row = float(((y.double() - y64s).abs() / y64s.abs().clamp_min(1e-300)).max())
+                    rec = {"workload": name, "values": "float32", "kernel": f"checked:{rtol:g}",
+                           "runs": f"{info.kernel}:{info.param0}",
+                           "accumulation": "fp32_dot2" if comp else "fp32", "ms": round(ms, 5),
+                           "GBps": round(b / ms / 1e6, 1), **e, "rel_l2_vs_stored_values": es["rel_l2"],
+                           "rows_recomputed_fp64": nre, "max_row_rel_err": rel_row}
+                    log(json.dumps(rec))
+                    out.append(rec)
+                    args.results.append(performance_result(
+                        *_dims(Av), tm.last, b, rel_l2=es["rel_l2"], max_abs=es["max_abs"], tolerance=TOL_FP32,
+                        device=dev, cpu=cpu, workload=name, values="float32", kernel=f"checked:{rtol:g}",
+                        accumulation=rec["accumulation"], rowsRecomputedFp64=nre, maxRowRelErr=rel_row))
                     p.destroy()
     return out
 

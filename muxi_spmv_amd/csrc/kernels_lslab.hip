@@ -41,42 +41,51 @@
 namespace muxi {
 
 // ---------------------------------------------------------------- pair sums
-template <bool COMP> struct LsPair;
-template <> struct LsPair<false> {
-    float s;
-    __device__ __forceinline__ static LsPair zero() { LsPair r; r.s = 0.f; return r; }
-    __device__ __forceinline__ void add(const LsPair& o) { s += o.s; }
-    template <typename VT>
-    __device__ __forceinline__ static LsPair prod(VT v, float xv) { LsPair r; r.s = to_f32(v) * xv; return r; }
-    __device__ __forceinline__ float value() const { return s; }
-};
-// Dot2 (Ogita, Rump, Oishi 2005): TwoProduct by FMA, TwoSum, errors summed in c. The arithmetic is written out
-// here with contraction off: a product fused into the next sum's addition would leave TwoSum computing the error
-// of a different sum.
-template <> struct LsPair<true> {
-    float s, c;
-    __device__ __forceinline__ static LsPair zero() { LsPair r; r.s = 0.f; r.c = 0.f; return r; }
+// A (row, slab) pair's sum. K bit 0: Dot2 (Ogita, Rump, Oishi 2005): TwoProduct by FMA, TwoSum, errors summed in
+// c. K bit 1 (checked plans): the sum of |products| rides along in a, for the row's a-posteriori error bound. The
+// Dot2 arithmetic is written out with contraction off: a product fused into the next sum's addition would leave
+// TwoSum computing the error of a different sum.
+template <int K>
+struct LsPair {
+    static constexpr bool kComp = (K & 1) != 0, kAbs = (K & 2) != 0;
+    static constexpr int NF = 1 + (kComp ? 1 : 0) + (kAbs ? 1 : 0);
+    float f[NF];  // f[0] the sum; f[1] its compensation (Dot2); f[NF - 1] the sum of |products| (checked)
+
+    __device__ __forceinline__ static LsPair zero() {
+        LsPair r;
+#pragma unroll
+        for (int i = 0; i < NF; ++i) r.f[i] = 0.f;
+        return r;
+    }
     __device__ __forceinline__ void add(const LsPair& o) {
 #pragma clang fp contract(off)
-        const float t = s + o.s;
-        const float bp = t - s;
-        const float e = (s - (t - bp)) + (o.s - bp);
-        s = t;
-        c = c + (o.c + e);
+        if constexpr (kComp) {
+            const float t = f[0] + o.f[0];
+            const float bp = t - f[0];
+            const float e = (f[0] - (t - bp)) + (o.f[0] - bp);
+            f[0] = t;
+            f[1] = f[1] + (o.f[1] + e);
+        } else {
+            f[0] += o.f[0];
+        }
+        if constexpr (kAbs) f[NF - 1] += o.f[NF - 1];
     }
     template <typename VT>
     __device__ __forceinline__ static LsPair prod(VT v, float xv) {
 #pragma clang fp contract(off)
         const float a = to_f32(v);
         LsPair r;
-        r.s = a * xv;
-        r.c = __builtin_fmaf(a, xv, -r.s);  // the product's exact rounding error
+        r.f[0] = a * xv;
+        if constexpr (kComp) r.f[1] = __builtin_fmaf(a, xv, -r.f[0]);  // the product's exact rounding error
+        if constexpr (kAbs) r.f[NF - 1] = __builtin_fabsf(r.f[0]);
         return r;
     }
     __device__ __forceinline__ float value() const {
 #pragma clang fp contract(off)
-        return s + c;
+        if constexpr (kComp) return f[0] + f[1];
+        return f[0];
     }
+    __device__ __forceinline__ float abs_sum() const { return kAbs ? f[NF - 1] : 0.f; }
 };
 
 // DPP cross-lane moves (GFX9 encodings: row_shr:n = 0x110 | n within a row of
@@ -86,25 +95,19 @@ template <int CTRL, int ROWS>
 __device__ __forceinline__ int ls_dpp(int v) {
     return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
 }
-template <int CTRL, int ROWS>
-__device__ __forceinline__ LsPair<false> ls_dpp(LsPair<false> v) {
-    LsPair<false> r;
-    r.s = __int_as_float(ls_dpp<CTRL, ROWS>(__float_as_int(v.s)));
-    return r;
-}
-template <int CTRL, int ROWS>
-__device__ __forceinline__ LsPair<true> ls_dpp(LsPair<true> v) {
-    LsPair<true> r;
-    r.s = __int_as_float(ls_dpp<CTRL, ROWS>(__float_as_int(v.s)));
-    r.c = __int_as_float(ls_dpp<CTRL, ROWS>(__float_as_int(v.c)));
+template <int CTRL, int ROWS, int K>
+__device__ __forceinline__ LsPair<K> ls_dpp(const LsPair<K>& v) {
+    LsPair<K> r;
+#pragma unroll
+    for (int i = 0; i < LsPair<K>::NF; ++i) r.f[i] = __int_as_float(ls_dpp<CTRL, ROWS>(__float_as_int(v.f[i])));
     return r;
 }
 
 // One step of an inclusive segmented scan: a lane whose flag is set starts a
 // segment and takes nothing from below.
-template <int CTRL, int ROWS, bool COMP>
-__device__ __forceinline__ void ls_seg_step(LsPair<COMP>& v, int& f) {
-    LsPair<COMP> u = ls_dpp<CTRL, ROWS>(v);
+template <int CTRL, int ROWS, int K>
+__device__ __forceinline__ void ls_seg_step(LsPair<K>& v, int& f) {
+    LsPair<K> u = ls_dpp<CTRL, ROWS>(v);
     const int fu = ls_dpp<CTRL, ROWS>(f);
     if (!f) {
         u.add(v);
@@ -114,8 +117,8 @@ __device__ __forceinline__ void ls_seg_step(LsPair<COMP>& v, int& f) {
 }
 // Shifts of 1, 2, 4, 8 inside each row of 16 lanes, then row 0's total into
 // rows 1 and 3 and rows 0-1's total into rows 2 and 3.
-template <bool COMP>
-__device__ __forceinline__ void ls_seg_scan(LsPair<COMP>& v, int& f) {
+template <int K>
+__device__ __forceinline__ void ls_seg_scan(LsPair<K>& v, int& f) {
     ls_seg_step<0x111, 0xf>(v, f);
     ls_seg_step<0x112, 0xf>(v, f);
     ls_seg_step<0x114, 0xf>(v, f);
@@ -123,19 +126,12 @@ __device__ __forceinline__ void ls_seg_scan(LsPair<COMP>& v, int& f) {
     ls_seg_step<0x142, 0xa>(v, f);
     ls_seg_step<0x143, 0xc>(v, f);
 }
-template <bool COMP>
-__device__ __forceinline__ LsPair<COMP> ls_readlane63(LsPair<COMP> v);
-template <>
-__device__ __forceinline__ LsPair<false> ls_readlane63(LsPair<false> v) {
-    LsPair<false> r;
-    r.s = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.s), 63));
-    return r;
-}
-template <>
-__device__ __forceinline__ LsPair<true> ls_readlane63(LsPair<true> v) {
-    LsPair<true> r;
-    r.s = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.s), 63));
-    r.c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.c), 63));
+template <int K>
+__device__ __forceinline__ LsPair<K> ls_readlane63(const LsPair<K>& v) {
+    LsPair<K> r;
+#pragma unroll
+    for (int i = 0; i < LsPair<K>::NF; ++i)
+        r.f[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.f[i]), 63));
     return r;
 }
 
@@ -178,16 +174,18 @@ struct LsLoad<bf16_t, N> {
 // earlier tile; the chunk run that pair falls in). Pairs are numbered in stream order (slab, then row); a
 // chunk's pairs split into runs by combine block, runs[off + r] = (first pair of run r, its offset) with nine
 // sentinels (first pair INT_MAX) after the last run, and pair j of run r goes to P[j + offset]: every combine
-// block then finds its pair sums in one contiguous range of P.
-template <typename VT, bool COMP, int EPL, int W>
+// block then finds its pair sums in one contiguous range of P. A checked plan's count of recomputed rows
+// (chk_count, else null) is cleared here, before the combine kernel counts them.
+template <typename VT, int K, int EPL, int W>
 __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __restrict__ chunks,
                                                                 const uint16_t* __restrict__ lc,
                                                                 const VT* __restrict__ lv,
                                                                 const int2* __restrict__ tiles,
                                                                 const int2* __restrict__ runs,
                                                                 const float* __restrict__ x, int C,
-                                                                LsPair<COMP>* __restrict__ P) {
-    using PT = LsPair<COMP>;
+                                                                LsPair<K>* __restrict__ P,
+                                                                int* __restrict__ chk_count) {
+    using PT = LsPair<K>;
     constexpr int TILE = 64 * EPL;
     constexpr int NWAVE = kLsThreads / 64;
     static_assert(W * 4 + kLsThreads * EPL * (int)sizeof(PT) + kLsMaxTiles * (2 * (int)sizeof(PT) + 1) <=
@@ -198,6 +196,7 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
     __shared__ PT head[kLsMaxTiles];
     __shared__ PT tail[kLsMaxTiles];
     __shared__ unsigned char tflag[kLsMaxTiles];  // bit 0: the tile has a row end; bit 1: starts mid-pair
+    if (LsPair<K>::kAbs && blockIdx.x == 0 && threadIdx.x == 0) *chk_count = 0;
     const int4 ch = chunks[blockIdx.x];
     const int ntiles = ch.y;
     if (ntiles == 0) return;  // whole workgroup, before any barrier
@@ -218,13 +217,13 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
     // loads join them), the last, partial slab one element at a time
     const float* xg = x + x0;
     if (width == W && (((uintptr_t)xg) & 15) == 0) {
-        constexpr int K = W / (4 * kLsThreads);
-        float4 v[K];
+        constexpr int NV = W / (4 * kLsThreads);
+        float4 v[NV];
 #pragma unroll
-        for (int k = 0; k < K; ++k) v[k] = *(const float4*)(xg + (tid + k * kLsThreads) * 4);
+        for (int k = 0; k < NV; ++k) v[k] = *(const float4*)(xg + (tid + k * kLsThreads) * 4);
         if (wave < ntiles) load_tile(wave, cn, vn);
 #pragma unroll
-        for (int k = 0; k < K; ++k) *(float4*)(xs + (tid + k * kLsThreads) * 4) = v[k];
+        for (int k = 0; k < NV; ++k) *(float4*)(xs + (tid + k * kLsThreads) * 4) = v[k];
     } else {
         if (wave < ntiles) load_tile(wave, cn, vn);
         for (int i = tid; i < width; i += kLsThreads) xs[i] = xg[i];
@@ -257,7 +256,7 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
         const int before = ls_count_prefix(__popc(f), total);
         PT sc = seg;
         int fl = f != 0u;
-        ls_seg_scan<COMP>(sc, fl);
+        ls_seg_scan<K>(sc, fl);
         PT run = ls_dpp<0x138, 0xf>(sc);  // open sum entering the lane (0 for lane 0)
         // the lane's k-th row end closes the tile's pair before + k (pair pair0 + before + k)
 #pragma unroll
@@ -292,7 +291,7 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
             r += 8;
         }
         if (mid && lane == 0 && total > 0) head[lt] = st[0];
-        const PT open = ls_readlane63<COMP>(sc);  // the sum open at the tile end (0 after a final row end)
+        const PT open = ls_readlane63<K>(sc);  // the sum open at the tile end (0 after a final row end)
         if (lane == 0) {
             tail[lt] = open;
             tflag[lt] = (unsigned char)((total > 0 ? 1 : 0) | (mid ? 2 : 0));
@@ -333,12 +332,13 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
 
 // A wavefront sums its rows of one block from LDS (its map range starts and ends on row boundaries): rounds of
 // 64 * E entries start on a multiple of E, a lane takes E consecutive entries (two 16-byte LDS reads), and
-// entries outside [a, b) count as zero and end nothing. Each row's last pair writes the row's sum to yl.
-template <bool COMP>
-__device__ __forceinline__ void ls_rows(const int* bi, const LsPair<COMP>* __restrict__ pb,
-                                        const unsigned* __restrict__ lmap, float* __restrict__ yl, int wave,
-                                        int lane) {
-    using PT = LsPair<COMP>;
+// entries outside [a, b) count as zero and end nothing. Each row's last pair writes the row's sum to yl (and,
+// in a checked plan, its sum of |products| to al).
+template <int K>
+__device__ __forceinline__ void ls_rows(const int* bi, const LsPair<K>* __restrict__ pb,
+                                        const unsigned* __restrict__ lmap, float* __restrict__ yl,
+                                        float* __restrict__ al, int wave, int lane) {
+    using PT = LsPair<K>;
     constexpr int E = 8;
     const int np = bi[3];
     const int a = wave == 0 ? 0 : bi[3 + wave] & 0xffff, b = wave == 3 ? np : bi[4 + wave] & 0xffff;
@@ -375,7 +375,7 @@ __device__ __forceinline__ void ls_rows(const int* bi, const LsPair<COMP>* __res
             sc = c;
         }
         int fl = f != 0u;
-        ls_seg_scan<COMP>(sc, fl);
+        ls_seg_scan<K>(sc, fl);
         PT run = ls_dpp<0x138, 0xf>(sc);
         if (lane == 0) run = carry;
 #pragma unroll
@@ -383,10 +383,11 @@ __device__ __forceinline__ void ls_rows(const int* bi, const LsPair<COMP>* __res
             run.add(p[j]);
             if ((f >> j) & 1u) {
                 yl[(m[j] >> 16) & 0x7fffu] = run.value();
+                if constexpr (PT::kAbs) al[(m[j] >> 16) & 0x7fffu] = run.abs_sum();
                 run = PT::zero();
             }
         }
-        carry = ls_readlane63<COMP>(sc);
+        carry = ls_readlane63<K>(sc);
     }
 }
 
@@ -405,21 +406,119 @@ __device__ __forceinline__ void ls_copy_lds(const unsigned* __restrict__ src, un
                                              (__attribute__((address_space(3))) void*)(dst + e0), 4, 0, 0);
 }
 
+// ---- checked plans: a row's a-posteriori error bound, and its FP64 recomputation
+// With u = 2^-24, n the row's nonzeros, S its sum of |products| (summed in FP32 beside y) and
+// g = (n + 2) u / (1 - (n + 2) u):
+//   plain FP32 sums in any order: |y - exact| <= g S (Higham, Accuracy and Stability, (3.5)), S itself within a
+//     factor 1 / (1 - g) of the exact sum of |products|: bound = g S / (1 - g);
+//   Dot2: every product and TwoSum error is below u S (n + 1 of them), their FP32 sum adds g times their total and
+//     the final s + c one rounding: bound = 2 u |y| + 2 g^2 S / (1 - g).
+// A row fails when bound > rtol |y| (a row whose y is 0 with S > 0 fails).
+template <int K>
+__device__ __forceinline__ bool ls_row_fails(int n, float yv, float S, float rtol) {
+    if (n <= 0) return false;
+    const double u = 0x1p-24, m = (double)(n + 2) * u;
+    if (m >= 0.5) return true;
+    const double g = m / (1.0 - m);
+    const double a = __builtin_fabs((double)yv);
+    const double bound = (K & 1) ? 2.0 * u * a + 2.0 * g * g * (double)S / (1.0 - g) : g * (double)S / (1.0 - g);
+    return bound > (double)rtol * a;
+}
+
+__device__ __forceinline__ double ls_to_f64(float v) { return (double)v; }
+__device__ __forceinline__ double ls_to_f64(double v) { return v; }
+__device__ __forceinline__ double ls_to_f64(bf16_t v) { return (double)to_f32(v); }
+
+// The rows a checked exec found failing, recomputed in FP64: workgroup g takes list entries g, g + grid, ... (the
+// list is complete when this kernel starts: the combine kernel before it wrote every entry). One row per
+// workgroup of 1024 lanes: lane t takes nonzeros k0 + t, k0 + t + 1024, ..., four at a time, in Dot2 in double
+// (TwoProduct by FMA, TwoSum); the lanes' (s, c) pairs are combined by a butterfly of TwoSums in each wavefront
+// and then wavefront by wavefront in index order, so a row's value does not depend on the list order.
+__device__ __forceinline__ void ls_two_sum_add(double& s, double& c, double p, double ep) {
+#pragma clang fp contract(off)
+    const double t = s + p;
+    const double bp = t - s;
+    c = c + (((s - (t - bp)) + (p - bp)) + ep);
+    s = t;
+}
+
+template <typename VT>
+__global__ __launch_bounds__(1024) void lslab_fp64_rows_kernel(const int* __restrict__ chk,
+                                                               const int* __restrict__ rp,
+                                                               const int* __restrict__ ci,
+                                                               const VT* __restrict__ v,
+                                                               const float* __restrict__ x, float* __restrict__ y,
+                                                               float alpha, float beta) {
+#pragma clang fp contract(off)
+    __shared__ double ws[16], wc[16];
+    const int count = chk[0];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int j = blockIdx.x; j < count; j += gridDim.x) {
+        const long row = chk[1 + j];
+        const int k0 = rp[row], k1 = rp[row + 1];
+        double s = 0.0, c = 0.0;
+        int k = k0 + tid;
+        for (; k + 3 * 1024 < k1; k += 4 * 1024) {
+            double a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a[q] = ls_to_f64(v[k + q * 1024]);
+                b[q] = (double)x[ci[k + q * 1024]];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double pr = a[q] * b[q];
+                ls_two_sum_add(s, c, pr, __builtin_fma(a[q], b[q], -pr));
+            }
+        }
+        for (; k < k1; k += 1024) {
+            const double a = ls_to_f64(v[k]), b = (double)x[ci[k]];
+            const double pr = a * b;
+            ls_two_sum_add(s, c, pr, __builtin_fma(a, b, -pr));
+        }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            const double os = __shfl_xor(s, m), oc = __shfl_xor(c, m);
+            ls_two_sum_add(s, c, os, oc);
+        }
+        if (lane == 0) {
+            ws[wave] = s;
+            wc[wave] = c;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double ts = ws[0], tc = wc[0];
+            for (int w = 1; w < 16; ++w) ls_two_sum_add(ts, tc, ws[w], wc[w]);
+            double r = (double)alpha * (ts + tc);
+            if (beta != 0.f) r += (double)beta * (double)y[row];
+            y[row] = (float)r;
+        }
+        __syncthreads();  // ws / wc are reused by the next row
+    }
+}
+
 // One block per workgroup: the block's map entries and pair sums (two contiguous ranges) go to LDS by direct
 // loads, all in flight before anything waits; the rows are then summed from LDS alone and each wavefront writes
 // y for its own rows. Small blocks let many workgroups share a CU and hide each other's load latency.
-template <bool COMP, int PAIR_BYTES, int ROWS>
+// Checked plans (K bit 1): each wavefront tests its rows' error bounds; a row that fails is appended to
+// chk[1 ..] (chk[0] counts them) and its y left for lslab_fp64_rows_kernel, which runs next (y keeps its old
+// value for beta meanwhile).
+template <typename VT, int K, int PAIR_BYTES, int ROWS>
 __global__ __launch_bounds__(256) void lslab_combine_kernel(const int* __restrict__ blocks,
                                                             const unsigned* __restrict__ map,
-                                                            const LsPair<COMP>* __restrict__ P, float* __restrict__ y,
-                                                            float alpha, float beta) {
-    using PT = LsPair<COMP>;
+                                                            const LsPair<K>* __restrict__ P, float* __restrict__ y,
+                                                            float alpha, float beta, const int* __restrict__ rp,
+                                                            float rtol, int* __restrict__ chk) {
+    using PT = LsPair<K>;
+    constexpr bool CHK = PT::kAbs;
     constexpr int PD = (int)sizeof(PT) / 4;
     constexpr int CAP = PAIR_BYTES / (int)sizeof(PT);
     constexpr int MC = PAIR_BYTES / 4 + 8;  // map entries (+ the tail of a lane's last 16-byte reads)
     __shared__ __attribute__((aligned(16))) PT pb[CAP];
     __shared__ __attribute__((aligned(16))) unsigned lm[MC];
     __shared__ float yl[ROWS];
+    __shared__ float al[CHK ? ROWS : 1];
+    __shared__ unsigned char fl[CHK ? ROWS : 1];    // 1: the row failed its bound (y written by the FP64 pass)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int bi[8];
     ls_block_info(blocks, blockIdx.x, bi);
@@ -427,11 +526,33 @@ __global__ __launch_bounds__(256) void lslab_combine_kernel(const int* __restric
     ls_copy_lds(map + base, lm, np, wave, lane);
     ls_copy_lds((const unsigned*)P + (long)base * PD, (unsigned*)pb, np * PD, wave, lane);
     const int r0 = wave == 0 ? 0 : bi[3 + wave] >> 16, r1 = wave == 3 ? bi[1] : bi[4 + wave] >> 16;
-    for (int i = r0 + lane; i < r1; i += 64) yl[i] = 0.f;  // this wavefront's rows
+    for (int i = r0 + lane; i < r1; i += 64) {  // this wavefront's rows
+        yl[i] = 0.f;
+        if constexpr (CHK) {
+            al[i] = 0.f;
+            fl[i] = 0;
+        }
+    }
     __builtin_amdgcn_s_waitcnt(0);                      // this wavefront's direct loads have landed
     __syncthreads();
-    ls_rows<COMP>(bi, pb, lm, yl, wave, lane);
-    for (int i = r0 + lane; i < r1; i += 64) store_y(y, (long)bi[0] + i, yl[i], alpha, beta);
+    ls_rows<K>(bi, pb, lm, yl, al, wave, lane);
+    if constexpr (CHK) {
+        for (int i0 = r0; i0 < r1; i0 += 64) {
+            const int i = i0 + lane;
+            bool bad = false;
+            if (i < r1) {
+                const long row = (long)bi[0] + i;
+                bad = ls_row_fails<K>(rp[row + 1] - rp[row], yl[i], al[i], rtol);
+            }
+            if (bad) {
+                fl[i] = 1;
+                const int slot = atomicAdd(chk, 1);  // at most one slot per row: the list holds numRows ids
+                chk[1 + slot] = (int)((long)bi[0] + i);
+            }
+        }
+    }
+    for (int i = r0 + lane; i < r1; i += 64)
+        if (!(CHK && fl[i])) store_y(y, (long)bi[0] + i, yl[i], alpha, beta);
 }
 
 // values into stream order: out[i] = v[src[i]], 0 where src[i] < 0 (tile padding)
@@ -454,7 +575,12 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
         set_error("lslab: nonzeros per lane must be 4 or 8");
         return SPMV_ERROR_INTERNAL;
     }
-    const int W = lslab_slab_cols(param0, p->comp);  // columns per slab
+    if (p->check && p->comp && epl != 4) {
+        set_error("lslab: a checked Dot2 plan takes 4 nonzeros per lane (its 12-byte pair sums fill the LDS)");
+        return SPMV_ERROR_INTERNAL;
+    }
+    const size_t ps = sizeof(float) * (1 + (p->comp ? 1 : 0) + (p->check ? 1 : 0));  // bytes per pair sum
+    const int W = lslab_slab_cols(param0, ps > 4);  // columns per slab
     const int pair_bytes = kLsPairBytes, rows_max = kLsRowsMax;
     const int S = C > 0 ? (int)(((long)C + W - 1) / W) : 0;
     if (S > kLsMaxSlabs) {
@@ -528,7 +654,7 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
 
     // ---- combine blocks: rows cut so that a block holds at most kLsRowsMax rows and its pair sums (and map
     // entries) fit the LDS; a block's pairs take P[rstart[first row] ..) in (slab, row) order
-    const int cap = pair_bytes / (p->comp ? 8 : 4);
+    const int cap = pair_bytes / (int)ps;
     std::vector<int> row_pairs((size_t)R + 1, 0);
     for (long j = 0; j < npairs; ++j) ++row_pairs[pair_row[j]];
     std::vector<int> rstart((size_t)R + 1, 0);
@@ -725,7 +851,8 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
     p->ls_chunks = (int)order.size();
     p->ls_blocks = B;
     p->ls_pairs = npairs;
-    const size_t vs = vsize(vt), ps = p->comp ? 8 : 4;
+    const size_t vs = vsize(vt);
+    if (p->check) MUXI_HIP(hipMalloc(&p->d_ls_chk, sizeof(int) * ((size_t)R + 1)));
     MUXI_HIP(hipMalloc(&p->d_ls_lc, sizeof(uint16_t) * hlc.size()));
     MUXI_HIP(hipMalloc(&p->d_ls_lv, vs * hlc.size()));
     MUXI_HIP(hipMalloc(&p->d_ls_tp, sizeof(int) * htile.size()));
@@ -769,15 +896,53 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
     (void)hipFree(d_src);
     p->scratch_bytes = (sizeof(uint16_t) + vs) * hlc.size() +
                        sizeof(int) * (htile.size() + hch.size() + hblk.size() + hrun.size()) +
-                       sizeof(unsigned) * hmap.size() + ps * (size_t)std::max(npairs, 1L);
+                       sizeof(unsigned) * hmap.size() + ps * (size_t)std::max(npairs, 1L) +
+                       (p->check ? sizeof(int) * ((size_t)R + 1) : 0);
     return st;
 }
 
-template <typename VT, bool COMP, int EPL, int W>
+template <typename VT, int K, int EPL, int W>
 static void launch_slab(const muxi_plan* p, const float* x, hipStream_t s) {
-    hipLaunchKernelGGL((lslab_slab_kernel<VT, COMP, EPL, W>), dim3(p->ls_chunks), dim3(kLsThreads), 0, s,
+    hipLaunchKernelGGL((lslab_slab_kernel<VT, K, EPL, W>), dim3(p->ls_chunks), dim3(kLsThreads), 0, s,
                        (const int4*)p->d_ls_ch, p->d_ls_lc, (const VT*)p->d_ls_lv, (const int2*)p->d_ls_tp,
-                       (const int2*)p->d_ls_F, x, p->A.C, (LsPair<COMP>*)p->d_ls_P);
+                       (const int2*)p->d_ls_F, x, p->A.C, (LsPair<K>*)p->d_ls_P, p->d_ls_chk);
+}
+
+__global__ void lslab_clear_count_kernel(int* c) { *c = 0; }
+
+template <typename VT, int K>
+static spmv_status_t launch_lslab_k(const muxi_plan* p, const float* x, float* y, float alpha, float beta,
+                                    hipStream_t s, hipEvent_t ev_end) {
+    if (p->ls_chunks > 0) {
+        // the slab width follows lslab_slab_cols: full slabs only with four nonzeros a lane and 4-byte pair sums
+        const bool four = (p->param0 & 0xff) == 4;
+        if (lslab_slab_cols(p->param0, K != 0) == kLsSlabCols) {
+            if constexpr (K == 0) launch_slab<VT, 0, 4, kLsSlabCols>(p, x, s);
+        } else if (four) {
+            launch_slab<VT, K, 4, kLsSlabCols / 2>(p, x, s);
+        } else {
+            if constexpr (K != 3) launch_slab<VT, K, 8, kLsSlabCols / 2>(p, x, s);  // (K = 3 is built with 4)
+        }
+        MUXI_LAUNCH_CHECK();
+    } else if (K & 2) {
+        hipLaunchKernelGGL(lslab_clear_count_kernel, dim3(1), dim3(1), 0, s, p->d_ls_chk);  // no slab kernel ran
+        MUXI_LAUNCH_CHECK();
+    }
+    if (ev_end) MUXI_HIP(hipEventRecord(ev_end, s));
+    if (p->ls_blocks > 0) {
+        hipLaunchKernelGGL((lslab_combine_kernel<VT, K, kLsPairBytes, kLsRowsMax>), dim3(p->ls_blocks), dim3(256),
+                           0, s, p->d_ls_blk, p->d_ls_map, (const LsPair<K>*)p->d_ls_P, y, alpha, beta, p->A.rp,
+                           p->rtol, p->d_ls_chk);
+        MUXI_LAUNCH_CHECK();
+    }
+    if constexpr ((K & 2) != 0) {
+        // a fixed grid: the count is on the device; workgroups past it exit at once
+        const unsigned g = (unsigned)std::max(1, std::min(p->A.R, 512));
+        hipLaunchKernelGGL(lslab_fp64_rows_kernel<VT>, dim3(g), dim3(1024), 0, s, p->d_ls_chk, p->A.rp, p->A.ci,
+                           (const VT*)p->A.v, x, y, alpha, beta);
+        MUXI_LAUNCH_CHECK();
+    }
+    return SPMV_SUCCESS;
 }
 
 template <typename VT, bool COMP>
@@ -787,26 +952,9 @@ spmv_status_t launch_lslab(const muxi_plan* p, const float* x, float* y, float a
         if (ev_end) MUXI_HIP(hipEventRecord(ev_end, s));
         return SPMV_SUCCESS;
     }
-    using PT = LsPair<COMP>;
-    if (p->ls_chunks > 0) {
-        // the slab width follows lslab_slab_cols: full slabs only with four nonzeros a lane and plain sums
-        const bool four = (p->param0 & 0xff) == 4;
-        if (lslab_slab_cols(p->param0, COMP) == kLsSlabCols) {
-            if constexpr (!COMP) launch_slab<VT, false, 4, kLsSlabCols>(p, x, s);
-        } else if (four) {
-            launch_slab<VT, COMP, 4, kLsSlabCols / 2>(p, x, s);
-        } else {
-            launch_slab<VT, COMP, 8, kLsSlabCols / 2>(p, x, s);
-        }
-        MUXI_LAUNCH_CHECK();
-    }
-    if (ev_end) MUXI_HIP(hipEventRecord(ev_end, s));
-    if (p->ls_blocks > 0) {
-        hipLaunchKernelGGL((lslab_combine_kernel<COMP, kLsPairBytes, kLsRowsMax>), dim3(p->ls_blocks), dim3(256), 0,
-                           s, p->d_ls_blk, p->d_ls_map, (const PT*)p->d_ls_P, y, alpha, beta);
-        MUXI_LAUNCH_CHECK();
-    }
-    return SPMV_SUCCESS;
+    constexpr int K0 = COMP ? 1 : 0;
+    return p->check ? launch_lslab_k<VT, K0 | 2>(p, x, y, alpha, beta, s, ev_end)
+                    : launch_lslab_k<VT, K0>(p, x, y, alpha, beta, s, ev_end);
 }
 
 template spmv_status_t launch_lslab<float, false>(const muxi_plan*, const float*, float*, float, float, hipStream_t,

@@ -30,7 +30,7 @@ static void plan_free(muxi_plan* p) {
     (void)hipFree(p->d_slab_ct);
     (void)hipFree(p->d_slab_heavy);
     for (void* q : {(void*)p->d_ls_lc, p->d_ls_lv, (void*)p->d_ls_tp, (void*)p->d_ls_ch, (void*)p->d_ls_blk,
-                    (void*)p->d_ls_F, (void*)p->d_ls_map, p->d_ls_P})
+                    (void*)p->d_ls_F, (void*)p->d_ls_map, p->d_ls_P, (void*)p->d_ls_chk})
         (void)hipFree(q);
     delete p;
 }
@@ -58,7 +58,7 @@ static bool pow2_in(int v, int lo, int hi) { return v >= lo && v <= hi && (v & (
 
 static spmv_status_t plan_create_impl(const muxi_csr_t* Ain, spmv_type_t vt, spmv_type_t xt, muxi_kernel_t kernel,
                                           int param0, int compensated, void* stream, muxi_plan** out,
-                                          long row_base = 0, long rows_full = -1) {
+                                          long row_base = 0, long rows_full = -1, double check_rtol = 0.0) {
     if (!out) {
         set_error("plan output pointer is NULL");
         return SPMV_ERROR_INVALID_HANDLE;
@@ -92,6 +92,8 @@ static spmv_status_t plan_create_impl(const muxi_csr_t* Ain, spmv_type_t vt, spm
     p->vtype = vt;
     p->xtype = xt;
     p->comp = compensated != 0;
+    p->check = check_rtol > 0.0;
+    p->rtol = (float)check_rtol;
     if (rows_full < 0) rows_full = Ain->numRows;
     if (row_base < 0 || row_base + Ain->numRows > rows_full || rows_full > 2147483647L) {
         set_error("invalid row block");
@@ -508,6 +510,44 @@ extern "C" spmv_status_t muxi_plan_create_rows(const muxi_csr_t* A, spmv_type_t 
                                                muxi_kernel_t kernel, int param0, int compensated, void* stream,
                                                long long row_base, long long rows_full, muxi_plan** out) {
     return plan_create_rows(A, vt, xt, kernel, param0, compensated, stream, (long)row_base, (long)rows_full, out);
+}
+
+extern "C" spmv_status_t muxi_plan_create_checked(const muxi_csr_t* A, spmv_type_t vt, int compensated, double rtol,
+                                                  void* stream, muxi_plan** out) {
+    return no_throw("muxi_plan_create_checked", [&] {
+        if (!(rtol > 0.0 && rtol < 1.0)) {
+            if (out) *out = nullptr;
+            set_error("checked plan: rtol must lie in (0, 1)");
+            return SPMV_ERROR_INTERNAL;
+        }
+        return plan_create_impl(A, vt, SPMV_TYPE_FLOAT, MUXI_KERNEL_LSLAB, 4, compensated, stream, out, 0, -1, rtol);
+    });
+}
+
+extern "C" spmv_status_t muxi_plan_checked_rows(const muxi_plan* p, int* count, int* rows, int cap, void* stream) {
+    if (!p) {
+        set_error("plan is NULL");
+        return SPMV_ERROR_INVALID_HANDLE;
+    }
+    if (!p->check) {
+        set_error("plan is not a checked plan (muxi_plan_create_checked)");
+        return SPMV_ERROR_INVALID_HANDLE;
+    }
+    if (!count || (cap > 0 && !rows)) {
+        set_error("count or rows is NULL");
+        return SPMV_ERROR_INVALID_VECTOR;
+    }
+    *count = 0;
+    if (p->A.R <= 0 || !p->d_ls_chk) return SPMV_SUCCESS;
+    hipStream_t s = (hipStream_t)stream;
+    MUXI_HIP(hipMemcpyAsync(count, p->d_ls_chk, sizeof(int), hipMemcpyDeviceToHost, s));
+    MUXI_HIP(hipStreamSynchronize(s));
+    const int n = std::min(std::min(*count, cap), p->A.R);
+    if (n > 0) {
+        MUXI_HIP(hipMemcpyAsync(rows, p->d_ls_chk + 1, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s));
+        MUXI_HIP(hipStreamSynchronize(s));
+    }
+    return SPMV_SUCCESS;
 }
 
 extern "C" spmv_status_t muxi_plan_create_spmm(const muxi_csr_t* Ain, spmv_type_t vt, spmv_type_t xt, int nrhs, int ipt,

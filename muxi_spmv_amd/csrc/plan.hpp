@@ -17,9 +17,9 @@ constexpr int kLsRowsMax = 1024;     // combine: rows per block
 constexpr int kLsMaxRuns = 1024;     // combine blocks one slab-kernel chunk may write to
 constexpr int kLsHalfSlabs = 0x100;  // param0 bit: slabs of kLsSlabCols / 2 columns
 // Columns per slab: full slabs (128 KiB of x) leave room in LDS for the staged pair sums only with four nonzeros
-// a lane and plain FP32 sums; otherwise, or when asked, half slabs.
-inline int lslab_slab_cols(int param0, bool comp) {
-    return ((param0 & 0xff) == 4 && !comp && !(param0 & kLsHalfSlabs)) ? kLsSlabCols : kLsSlabCols / 2;
+// a lane and 4-byte pair sums (plain FP32, unchecked); otherwise, or when asked, half slabs.
+inline int lslab_slab_cols(int param0, bool wide_pairs) {
+    return ((param0 & 0xff) == 4 && !wide_pairs && !(param0 & kLsHalfSlabs)) ? kLsSlabCols : kLsSlabCols / 2;
 }
 struct SlabArgs {
     int S;
@@ -95,6 +95,11 @@ struct muxi_plan {
     int* d_ls_F = nullptr;    // run tables of the chunks: (first pair, P offset) per run, then (INT_MAX, 0)
     unsigned* d_ls_map = nullptr;
     void* d_ls_P = nullptr;
+    // checked plans (muxi_plan_create_checked): each row's a-posteriori error bound is tested against rtol * |y|
+    // and the rows that fail are recomputed in FP64; d_ls_chk = [count of such rows in the last exec, their ids]
+    bool check = false;
+    float rtol = 0.f;
+    int* d_ls_chk = nullptr;
 };
 
 namespace muxi {

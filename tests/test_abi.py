@@ -195,3 +195,15 @@ def _opts_transpose():
     o = _lib.spmv_opts_t()
     o.operation = _lib.SPMV_OP_TRANSPOSE
     return o
+
+
+def test_checked_plan_argument_errors_before_any_device_work():
+    lib = _lib.load()
+    p = C.c_void_p(1)
+    a = _lib.muxi_csr_t()
+    for bad in (0.0, -1e-3, 1.0, float("nan")):
+        assert lib.muxi_plan_create_checked(C.byref(a), 0, 0, bad, None, C.byref(p)) == -9
+        assert b"rtol" in lib.muxi_spmv_last_error()
+        assert p.value is None  # the output is cleared
+    n = C.c_int(7)
+    assert lib.muxi_plan_checked_rows(None, C.byref(n), None, 0, None) == -1

@@ -6,7 +6,8 @@
   from hipGraph replays; with the generated U[-1, 1] values every member is within rel L2 1e-6 of the FP64 oracle.
 * Precision sweep, power-law leg: the 2M x 2M power-law matrix (exponent 2, lengths 1..100,000, rows shuffled)
   with values stored FP64, FP32 and BF16, x FP32, FP32 and Dot2 accumulation: rel L2 <= 1e-6 against the FP64
-  oracle on the stored values, Dot2 no worse than plain, BF16 storage error within its unit roundoff.
+  oracle on the stored values, Dot2 no worse than plain, BF16 storage error within its unit roundoff; AUTO's
+  kernel there, plain and Dot2; and the checked plan (per-row FP64 fallback) with every row within rtol.
 * Learned selector: a seeded handful of the held-out matrices of the committed sweep DB, regenerated on the
   device. Their features equal the recorded ones, the library's AUTO pick equals the pick the shipped network
   makes from the recorded features (the FP64 oracle network), and re-timed cold the pick is within 15 % (or 2 us)
@@ -179,6 +180,30 @@ def test_powerlaw_2m_auto_plan_kernel(powerlaw_2m, comp):
     A = api.CSRMatrix(h.rowPtr, h.colIdx, torch.from_numpy(h.values), h.numCols)
     info = api.Plan(A, "auto", 0, compensated=comp).info()
     assert (info.kernel, info.param0) == ("lslab", 4), info
+
+
+@pytest.mark.parametrize("comp", [False, True])
+def test_powerlaw_2m_checked_plan_every_row_within_rtol(powerlaw_2m, comp):
+    """The checked plan on the full power-law matrix (random signs, rows up to 90,181 nonzeros): every row of y
+    within rtol of the FP64 oracle -- the rows whose bound fails are the ones recomputed in FP64."""
+    h, x, A64 = powerlaw_2m
+    A = api.CSRMatrix(h.rowPtr, h.colIdx, torch.from_numpy(h.values), h.numCols)
+    xd = torch.from_numpy(x).cuda()
+    rtol = 1e-3
+    p = api.Plan.checked(A, rtol, compensated=comp)
+    y = torch.full((h.numRows,), float("nan"), device="cuda")
+    p.exec(xd, y)
+    y64 = api.oracle_f64(A, xd)
+    torch.cuda.synchronize()
+    rows = p.checked_rows()
+    err = (y.double() - y64).abs()
+    ok = err <= rtol * y64.abs() * 1.0001
+    bad = torch.nonzero(~ok).flatten()[:5].tolist()
+    assert bool(ok.all()), (bad, [float(err[i]) for i in bad], [float(y64[i]) for i in bad])
+    lens = np.diff(h.rowPtr.astype(np.int64))
+    print(f"powerlaw_2m checked {'dot2' if comp else 'fp32'} rtol {rtol}: {rows.size} rows recomputed in FP64 "
+          f"({int(lens[rows].sum()) if rows.size else 0} nonzeros)", flush=True)
+    assert rows.size < h.numRows // 10
 
 
 # ---------------------------------------------------------------- learned selector spot-check
