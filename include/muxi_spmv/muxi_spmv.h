@@ -247,7 +247,9 @@ typedef struct muxi_plan muxi_plan;
 
 /* SpMM plan: Y = alpha*A*X + beta*Y with X (numCols x nrhs) and Y (numRows x
  * nrhs) row-major, nrhs in [1, 64], on merge-path partitions; ipt = items
- * per lane group (4, 8, 16 or 32; <= 0 picks 8). Executed by muxi_plan_exec
+ * per lane group (4, 8, 16 or 32; <= 0 takes it from the schedule table the
+ * project's autotuner fitted, by nrhs and numRows + nnz;
+ * muxi_spmv_amd/sched_tune.py). Executed by muxi_plan_exec
  * with the same nrhs. Value types FLOAT/BF16/DOUBLE with x FLOAT, or
  * DOUBLE with x DOUBLE; plain accumulation. */
 spmv_status_t muxi_plan_create_spmm(const muxi_csr_t* A, spmv_type_t vtype, spmv_type_t xtype, int nrhs, int ipt,

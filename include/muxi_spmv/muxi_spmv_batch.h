@@ -24,7 +24,10 @@ typedef struct muxi_batch_plan muxi_batch_plan;
 
 /* mats[count] with device arrays; xs[count] / ys[count] device vectors of
  * numCols / numRows elements. vtype FLOAT (xtype FLOAT) or DOUBLE (xtype
- * DOUBLE). ipt: merge items per lane (4, 8 or 16; <= 0 picks 16). */
+ * DOUBLE). ipt: merge items per lane (4, 8 or 16) | workgroups per CU << 8
+ * (1 to 32; 0 = 8); ipt <= 0 takes both from the schedule table the
+ * project's autotuner fitted (by the batch's mean nonzeros per row and mean
+ * rows per matrix; muxi_spmv_amd/sched_tune.py). */
 spmv_status_t muxi_batch_plan_create(const muxi_csr_t* mats, int count, spmv_type_t vtype, spmv_type_t xtype,
                                      const void* const* xs, void* const* ys, int ipt, void* stream,
                                      muxi_batch_plan** plan);
@@ -41,6 +44,8 @@ spmv_status_t muxi_batch_exec_per_matrix(const muxi_batch_plan* plan, muxi_kerne
                                          void* stream);
 /* Tiles, grid size and scratch bytes of the plan. */
 spmv_status_t muxi_batch_info(const muxi_batch_plan* plan, int* tiles, int* grid, size_t* scratch_bytes);
+/* The schedule the plan runs: merge items per lane and workgroups per CU. */
+spmv_status_t muxi_batch_schedule(const muxi_batch_plan* plan, int* ipt, int* wg_per_cu);
 
 /* hipGraph of one muxi_batch_exec (per_matrix_kernel = 0) or of one launch
  * per matrix with that kernel id (SCALAR, VECTOR or VWAVE with param0 lanes;

@@ -106,6 +106,13 @@ int muxi_selector_state(void);
  * rows average under 64 nonzeros. */
 spmv_status_t muxi_selector_pick(const muxi_csr_t* A, spmv_type_t xtype, void* stream, int* kernel, int* param0);
 
+/* The schedule tables the autotuner fitted (muxi_spmv_amd/sched_tune.py), compiled into the library: what
+ * muxi_batch_plan_create (ipt <= 0) picks for a batch with the given mean nonzeros per row and mean rows per
+ * matrix, and what muxi_plan_create_spmm (ipt <= 0) picks for k right-hand sides and the given merge-path
+ * length numRows + nnz. Host only. No counterpart in the reference, which hard-codes these (spmv_csr5.cu:953-967). */
+spmv_status_t muxi_batch_schedule_for(double mean_row_len, double mean_rows, int* ipt, int* wg_per_cu);
+int muxi_spmm_schedule_for(int k, double path_len);
+
 #ifdef __cplusplus
 }
 #endif

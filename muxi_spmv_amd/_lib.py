@@ -188,6 +188,9 @@ SIGNATURES: dict[str, tuple] = {
     "muxi_batch_exec_ev": (_I, [_VP, _D, _D, _VP, _VP, _VP]),
     "muxi_batch_exec_per_matrix": (_I, [_VP, _I, _I, _VP]),
     "muxi_batch_info": (_I, [_VP, C.POINTER(_I), C.POINTER(_I), C.POINTER(_Z)]),
+    "muxi_batch_schedule": (_I, [_VP, C.POINTER(_I), C.POINTER(_I)]),
+    "muxi_batch_schedule_for": (_I, [_D, _D, C.POINTER(_I), C.POINTER(_I)]),
+    "muxi_spmm_schedule_for": (_I, [_I, _D]),
     "muxi_batch_graph_create": (_I, [_VP, _D, _D, _I, _I, _VP, C.POINTER(_VP)]),
     "muxi_graph_launch": (_I, [_VP, _VP]),
     "muxi_graph_destroy": (_I, [_VP]),

@@ -356,8 +356,9 @@ class Plan:
                                                C.byref(self._p)), "plan_create_rows")
 
     @classmethod
-    def spmm(cls, A: CSRMatrix, nrhs: int, ipt: int = 8, xtype=torch.float32, stream=None) -> "Plan":
-        """SpMM plan: Y = alpha*A*X + beta*Y, X (numCols x nrhs) and Y (numRows x nrhs) row-major."""
+    def spmm(cls, A: CSRMatrix, nrhs: int, ipt: int = 0, xtype=torch.float32, stream=None) -> "Plan":
+        """SpMM plan: Y = alpha*A*X + beta*Y, X (numCols x nrhs) and Y (numRows x nrhs) row-major; ipt 0 takes
+        the items per lane group from the tuned schedule table."""
         self = cls.__new__(cls)
         self.A = A
         self.xtype = xtype

@@ -13,7 +13,8 @@ from ._lib import check, load
 class BatchPlan:
     """Binds matrices (objects with .c and .dtype) and their x / y tensors."""
 
-    def __init__(self, mats, xs, ys, ipt: int = 16, stream=None):
+    def __init__(self, mats, xs, ys, ipt: int = 0, stream=None):
+        """ipt: merge items per lane (4, 8, 16) | workgroups per CU << 8; 0 = the tuned schedule."""
         n = len(mats)
         self.mats, self.xs, self.ys = mats, xs, ys  # keep alive
         arr = (_lib.muxi_csr_t * n)(*[m.c for m in mats])
@@ -34,7 +35,9 @@ class BatchPlan:
     def info(self) -> dict:
         t, g, s = C.c_int(), C.c_int(), C.c_size_t()
         check(load().muxi_batch_info(self._p, C.byref(t), C.byref(g), C.byref(s)), "batch_info")
-        return {"tiles": t.value, "grid": g.value, "scratch_bytes": s.value}
+        i, w = C.c_int(), C.c_int()
+        check(load().muxi_batch_schedule(self._p, C.byref(i), C.byref(w)), "batch_schedule")
+        return {"tiles": t.value, "grid": g.value, "scratch_bytes": s.value, "ipt": i.value, "wg_per_cu": w.value}
 
     def graph(self, per_matrix_kernel: str | None = None, param0: int = 0, alpha=1.0, beta=0.0, stream=None):
         """A captured hipGraph of one exec (or of the per-matrix loop)."""

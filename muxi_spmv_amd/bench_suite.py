@@ -209,16 +209,19 @@ def batched(args) -> list[dict]:
     tm = Timer()
     s = torch.cuda.Stream()
     out = []
+    from .sched_tune import BATCH_CANDIDATES
+
     ipt_ms = {}
-    for ipt in (4, 8, 16):  # items per lane of the merge tiles
-        bp = BatchPlan(mats, xs, ys, ipt)
+    for ipt, wg in BATCH_CANDIDATES:  # items per lane of the merge tiles x workgroups per CU
+        bp = BatchPlan(mats, xs, ys, ipt | (wg << 8))
         with torch.cuda.stream(s):
-            ipt_ms[ipt] = tm.time(lambda: bp.exec())
+            ipt_ms[f"{ipt}x{wg}"] = tm.time(lambda: bp.exec())
         torch.cuda.synchronize()
-        log(f"batched persistent, {ipt} items per lane: {ipt_ms[ipt]:.4f} ms")
+        log(f"batched persistent, {ipt} items per lane x {wg} workgroups per CU: {ipt_ms[f'{ipt}x{wg}']:.4f} ms")
         bp.destroy()
     best_ipt = min(ipt_ms, key=ipt_ms.get)
-    bp = BatchPlan(mats, xs, ys, best_ipt)
+    bp = BatchPlan(mats, xs, ys)  # the tuned schedule
+    tuned = f"{bp.info()['ipt']}x{bp.info()['wg_per_cu']}"
     times = {}
 
     def worst_err():
@@ -262,7 +265,9 @@ def batched(args) -> list[dict]:
                                                path=name, matrices=len(mats), errors_are="worst member"))
     out.append({"workload": "batched_256", "speedup_graph_vs_per_matrix_graph": round(t_vec_g / t_batch_g, 2),
                 "speedup_eager": round(t_vec / t_batch, 2), "worst_rel_l2": worst, "info": bp.info(),
-                "items_per_lane": best_ipt, "ms_by_items_per_lane": {str(k): round(v, 4) for k, v in ipt_ms.items()}})
+                "schedule": tuned, "sweep_best_schedule": best_ipt,
+                "schedule_regret": round(ipt_ms[tuned] / ipt_ms[best_ipt] - 1, 4),
+                "ms_by_schedule": {k: round(v, 4) for k, v in ipt_ms.items()}})
     for r in out:
         log(json.dumps(r))
     return out
@@ -280,12 +285,15 @@ def spmm(args) -> list[dict]:
         y64 = api.oracle_f64(A, X[:, 3].contiguous())
         cpu = cpu_ms(h.rowPtr, h.colIdx, h.values, X.cpu().numpy(), reps=1)
         best = None
-        for ipt in (4, 8, 16):
+        recs_k = []
+        tuned = api.Plan.spmm(A, k).info().param0  # the autotuner's table
+        for ipt in (4, 8, 16, 32):
             p = api.Plan.spmm(A, k, ipt)
             ms = tm.time(lambda: p.exec(X, Y))
             b = algorithmic_bytes(h.numRows, h.numCols, h.nnz, 4, 4, k=k)
             rec = {"workload": "spmm_powerlaw_2m", "k": k, "ipt": ipt, "ms": round(ms, 4),
                    "GBps": round(b / ms / 1e6, 1), "GFLOPs": round(2 * h.nnz * k / ms / 1e6, 1)}
+            recs_k.append(rec)
             log(json.dumps(rec))
             e = errors(Y[:, 3], y64)
             args.results.append(performance_result(h.numRows, h.numCols, h.nnz, tm.last, b, k=k, rel_l2=e["rel_l2"],
@@ -294,6 +302,8 @@ def spmm(args) -> list[dict]:
                                                    errors_are="column 3"))
             if best is None or ms < best["ms"]:
                 best = rec
+        best["tuned_ipt"] = tuned
+        best["tuned_regret"] = round(next(r["ms"] for r in recs_k if r["ipt"] == tuned) / best["ms"] - 1, 4)
         p = api.Plan.spmm(A, k, best["ipt"])
         p.exec(X, Y)
         y64 = api.oracle_f64(A, X[:, 3].contiguous())

@@ -77,6 +77,40 @@ def write_selector_header() -> Path | None:
     return inc
 
 
+SCHED = PKG / "sched_tables.json"
+
+
+def write_sched_header() -> Path | None:
+    """Writes _build/sched_tables.inc from sched_tables.json (or removes a stale one)."""
+    inc = BUILD / "sched_tables.inc"
+    if not SCHED.exists():
+        inc.unlink(missing_ok=True)
+        return None
+    t = json.loads(SCHED.read_text())
+    b, m = t["batch"], t["spmm"]
+    nb_len, nb_rows = len(b["row_len_edges"]) + 1, len(b["rows_edges"]) + 1
+    ns_len = len(m["path_edges"]) + 1
+    if len(b["table"]) != nb_len * nb_rows or len(m["table"]) != len(m["k"]) * ns_len:
+        raise RuntimeError(f"{SCHED}: table sizes do not match the bins")
+
+    def arr(ctype, name, vals):
+        return f"static const {ctype} {name}[] = {{{','.join(repr(v) for v in vals)}}};\n"
+
+    txt = ("// Generated by muxi_spmv_amd/build.py from sched_tables.json; do not edit.\n"
+           f"static const int kBatchRowLenBins = {nb_len}, kBatchRowsBins = {nb_rows};\n"
+           + arr("double", "kBatchRowLenEdges", [float(v) for v in b["row_len_edges"]])
+           + arr("double", "kBatchRowsEdges", [float(v) for v in b["rows_edges"]])
+           + arr("int", "kBatchIpt", [int(e[0]) for e in b["table"]])
+           + arr("int", "kBatchWgPerCu", [int(e[1]) for e in b["table"]])
+           + f"static const int kSpmmKCount = {len(m['k'])}, kSpmmPathBins = {ns_len};\n"
+           + arr("int", "kSpmmK", [int(v) for v in m["k"]])
+           + arr("double", "kSpmmPathEdges", [float(v) for v in m["path_edges"]])
+           + arr("int", "kSpmmIpt", [int(v) for v in m["table"]]))
+    if not inc.exists() or inc.read_text() != txt:
+        inc.write_text(txt)
+    return inc
+
+
 def sources() -> list[Path]:
     return sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
 
@@ -93,6 +127,7 @@ def _compile(src: Path) -> Path:
 def build(jobs: int | None = None, verbose: bool = True) -> Path:
     BUILD.mkdir(exist_ok=True)
     write_selector_header()
+    write_sched_header()
     srcs = sources()
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -7,11 +7,13 @@
 // their own contiguous eighth, running the shared tile body. Carries are
 // keyed by a virtual row (each matrix's rows plus one dummy row, laid end to
 // end) so that runs never cross matrices, and one fix-up kernel folds them.
+#include <algorithm>
 #include <climits>
 #include <vector>
 
 #include "merge_tile.hpp"
 #include "muxi_spmv/muxi_spmv_batch.h"
+#include "sched.hpp"
 
 namespace muxi {
 
@@ -123,7 +125,7 @@ __global__ __launch_bounds__(256) void batch_fixup_kernel(const int* __restrict_
 using namespace muxi;
 
 struct muxi_batch_plan {
-    int count = 0, T = 0, grid = 0, ipt = 8;
+    int count = 0, T = 0, grid = 0, ipt = 8, wg_per_cu = 8;
     spmv_type_t vtype = SPMV_TYPE_FLOAT, xtype = SPMV_TYPE_FLOAT;
     std::vector<BatchDesc> host;
     BatchDesc* d_desc = nullptr;
@@ -173,9 +175,21 @@ static spmv_status_t batch_plan_create_impl(const muxi_csr_t* mats, int count, s
         set_error("batch: FLOAT/FLOAT or DOUBLE/DOUBLE only");
         return SPMV_ERROR_UNSUPPORTED_TYPE;
     }
-    if (ipt <= 0) ipt = 16;  // the fastest of 4, 8, 16 on 256 matrices of 1e3 to 2e5 rows
-    if (!(ipt == 4 || ipt == 8 || ipt == 16)) {
-        set_error("batch: items per lane must be 4, 8 or 16");
+    // ipt: merge items per lane (bits 0-7: 4, 8 or 16) | workgroups per CU << 8 (1 to 32; 0 = 8); ipt <= 0
+    // takes both from the tuned schedule table
+    int wg_per_cu = ipt > 0 ? (ipt >> 8) & 0xff : 0;
+    if (ipt > 0) ipt &= 0xff;
+    if (ipt <= 0) {
+        long rows = 0, nnz = 0;
+        for (int m = 0; m < count; ++m) {
+            rows += std::max(0, mats[m].numRows);
+            nnz += std::max(0, mats[m].nnz);
+        }
+        batch_schedule(rows > 0 ? (double)nnz / (double)rows : 0.0, (double)rows / count, &ipt, &wg_per_cu);
+    }
+    if (wg_per_cu == 0) wg_per_cu = 8;
+    if (!(ipt == 4 || ipt == 8 || ipt == 16) || wg_per_cu < 1 || wg_per_cu > 32) {
+        set_error("batch: items per lane must be 4, 8 or 16 and workgroups per CU 1 to 32");
         return SPMV_ERROR_INTERNAL;
     }
     hipStream_t s = (hipStream_t)stream;
@@ -184,6 +198,7 @@ static spmv_status_t batch_plan_create_impl(const muxi_csr_t* mats, int count, s
     p->vtype = vt;
     p->xtype = xt;
     p->ipt = ipt;
+    p->wg_per_cu = wg_per_cu;
     p->host.resize(count);
     const long CH = 256L * ipt;
     long T = 0, ncoord = 0, vrows = 0;
@@ -215,7 +230,7 @@ static spmv_status_t batch_plan_create_impl(const muxi_csr_t* mats, int count, s
     BT_HIP(hipGetDevice(&dev));
     BT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     // a multiple of 8 workgroups: an equal share for each XCD's part of the tile table
-    p->grid = (int)std::max<long>(8, std::min<long>(T, (long)cus * 8) / 8 * 8);
+    p->grid = (int)std::max<long>(8, std::min<long>(T, (long)cus * wg_per_cu) / 8 * 8);
     const size_t acc = xt == SPMV_TYPE_DOUBLE ? 8 : 4;
     BT_HIP(hipMalloc(&p->d_desc, sizeof(BatchDesc) * count));
     BT_HIP(hipMalloc(&p->d_tile_mat, sizeof(int) * (size_t)(T > 0 ? T : 1)));
@@ -339,6 +354,13 @@ extern "C" spmv_status_t muxi_batch_info(const muxi_batch_plan* p, int* tiles, i
     if (tiles) *tiles = p->T;
     if (grid) *grid = p->grid;
     if (scratch) *scratch = p->scratch;
+    return SPMV_SUCCESS;
+}
+
+extern "C" spmv_status_t muxi_batch_schedule(const muxi_batch_plan* p, int* ipt, int* wg_per_cu) {
+    if (!p) return SPMV_ERROR_INVALID_HANDLE;
+    if (ipt) *ipt = p->ipt;
+    if (wg_per_cu) *wg_per_cu = p->wg_per_cu;
     return SPMV_SUCCESS;
 }
 

@@ -10,6 +10,7 @@
 
 #include "muxi_spmv/muxi_spmv_tune.h"
 #include "plan.hpp"
+#include "sched.hpp"
 
 using namespace muxi;
 
@@ -311,7 +312,8 @@ static spmv_status_t plan_create_spmm_impl(const muxi_csr_t* Ain, spmv_type_t vt
         set_error("spmm: nrhs must be in [1, 64]");
         return SPMV_ERROR_INVALID_VECTOR;
     }
-    if (ipt <= 0) ipt = 8;
+    if (ipt <= 0)  // the tuned schedule table: by k and the merge-path length
+        ipt = spmm_schedule(nrhs, (double)Ain->numRows + (double)Ain->nnz);
     if (!(ipt == 4 || ipt == 8 || ipt == 16 || ipt == 32)) {
         set_error("spmm: items per group must be 4, 8, 16 or 32");
         return SPMV_ERROR_INTERNAL;
@@ -511,6 +513,14 @@ extern "C" spmv_status_t muxi_plan_create_rows(const muxi_csr_t* A, spmv_type_t 
                                                long long row_base, long long rows_full, muxi_plan** out) {
     return plan_create_rows(A, vt, xt, kernel, param0, compensated, stream, (long)row_base, (long)rows_full, out);
 }
+
+extern "C" spmv_status_t muxi_batch_schedule_for(double mean_row_len, double mean_rows, int* ipt, int* wg_per_cu) {
+    if (!ipt || !wg_per_cu) return SPMV_ERROR_INVALID_HANDLE;
+    batch_schedule(mean_row_len, mean_rows, ipt, wg_per_cu);
+    return SPMV_SUCCESS;
+}
+
+extern "C" int muxi_spmm_schedule_for(int k, double path_len) { return spmm_schedule(k, path_len); }
 
 extern "C" spmv_status_t muxi_plan_create_checked(const muxi_csr_t* A, spmv_type_t vt, int compensated, double rtol,
                                                   void* stream, muxi_plan** out) {

@@ -8,6 +8,8 @@
   with values stored FP64, FP32 and BF16, x FP32, FP32 and Dot2 accumulation: rel L2 <= 1e-6 against the FP64
   oracle on the stored values, Dot2 no worse than plain, BF16 storage error within its unit roundoff; AUTO's
   kernel there, plain and Dot2; and the checked plan (per-row FP64 fallback) with every row within rtol.
+* Tuned schedules: the batched plan's default schedule and SpMM's default items per lane group (the autotuner's
+  tables) re-timed against every candidate on the 256-matrix batch and the power-law matrix at k = 8 and 32.
 * Learned selector: a seeded handful of the held-out matrices of the committed sweep DB, regenerated on the
   device. Their features equal the recorded ones, the library's AUTO pick equals the pick the shipped network
   makes from the recorded features (the FP64 oracle network), and re-timed cold the pick is within 15 % (or 2 us)
@@ -132,6 +134,50 @@ def test_batch_per_matrix_graph_alpha_beta_and_bad_kernel(batch_256):
             bp.graph("merge", 4, stream=s)
     with pytest.raises(api.SpmvError, match="INTERNAL"):
         bp.exec_per_matrix("csr5", 256)
+
+
+def test_batch_256_tuned_schedule_within_5pct_of_sweep(batch_256):
+    """The schedule a batch plan takes by default (the autotuner's table) against every candidate, re-timed
+    cold: within 5 % of the best."""
+    from muxi_spmv_amd import sched_tune
+
+    mats, xs, ys = batch_256
+    tm = sched_tune._Timer()
+    bp = batch.BatchPlan(mats, xs, ys)
+    info = bp.info()
+    pick = (info["ipt"], info["wg_per_cu"])
+    t_pick = tm.cold(lambda: bp.exec(), 7)
+    bp.destroy()
+    times = {}
+    for ipt, wg in sched_tune.BATCH_CANDIDATES:
+        b = batch.BatchPlan(mats, xs, ys, ipt | (wg << 8))
+        times[(ipt, wg)] = tm.cold(lambda: b.exec(), 7)
+        b.destroy()
+    best = min(times, key=times.get)
+    print(f"batch_256 schedule: tuned {pick} {t_pick:.4f} ms, sweep best {best} {times[best]:.4f} ms", flush=True)
+    assert t_pick <= 1.05 * times[best], (pick, t_pick, best, times)
+
+
+@pytest.mark.parametrize("k", [8, 32])
+def test_spmm_powerlaw_tuned_ipt_within_5pct_of_sweep(k):
+    from muxi_spmv_amd import sched_tune
+
+    h = G.cfg_powerlaw_2m()
+    A = api.CSRMatrix(h.rowPtr, h.colIdx, torch.from_numpy(h.values), h.numCols)
+    X = torch.rand((h.numCols, k), device="cuda") * 2 - 1
+    Y = torch.empty((h.numRows, k), device="cuda")
+    tm = sched_tune._Timer()
+    p = api.Plan.spmm(A, k)
+    pick = p.info().param0
+    t_pick = tm.cold(lambda: p.exec(X, Y), 7)
+    times = {}
+    for ipt in sched_tune.SPMM_IPT:
+        q = api.Plan.spmm(A, k, ipt)
+        times[ipt] = tm.cold(lambda: q.exec(X, Y), 7)
+        q.destroy()
+    best = min(times, key=times.get)
+    print(f"spmm k={k}: tuned {pick} {t_pick:.4f} ms, sweep best {best} {times[best]:.4f} ms", flush=True)
+    assert t_pick <= 1.05 * times[best], (pick, t_pick, best, times)
 
 
 # ---------------------------------------------------------------- precision sweep, power-law 2M
