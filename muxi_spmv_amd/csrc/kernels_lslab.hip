@@ -188,14 +188,13 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
     using PT = LsPair<K>;
     constexpr int TILE = 64 * EPL;
     constexpr int NWAVE = kLsThreads / 64;
-    static_assert(W * 4 + kLsThreads * EPL * (int)sizeof(PT) + kLsMaxTiles * (2 * (int)sizeof(PT) + 1) <=
-                      kLsLdsBytes,
-                  "slab kernel LDS");
+    static_assert(lslab_fits(W, EPL, (int)sizeof(PT)), "slab kernel LDS");
     __shared__ float xs[W];
     __shared__ PT stage[kLsThreads * EPL];  // per wavefront: one tile's pair sums in pair order
     __shared__ PT head[kLsMaxTiles];
     __shared__ PT tail[kLsMaxTiles];
     __shared__ unsigned char tflag[kLsMaxTiles];  // bit 0: the tile has a row end; bit 1: starts mid-pair
+    __shared__ int2 tds[kLsMaxTiles];             // the chunk's tile descriptors
     if (LsPair<K>::kAbs && blockIdx.x == 0 && threadIdx.x == 0) *chk_count = 0;
     const int4 ch = chunks[blockIdx.x];
     const int ntiles = ch.y;
@@ -216,6 +215,7 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
     // the slab of x into LDS: a whole slab with 16-byte loads, all in flight together (then the first tile's
     // loads join them), the last, partial slab one element at a time
     const float* xg = x + x0;
+    for (int i = tid; i < ntiles; i += kLsThreads) tds[i] = tiles[(long)ch.x + i];  // with the slab of x
     if (width == W && (((uintptr_t)xg) & 15) == 0) {
         constexpr int NV = W / (4 * kLsThreads);
         float4 v[NV];
@@ -234,8 +234,7 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
     // one tile: products from LDS x, the lane's segmented sums, a DPP scan across lanes, the tile's pair sums
     // through the LDS stage to P
     auto process = [&](int lt, const uint16_t* c, const VT* v) {
-        const long t = (long)ch.x + lt;
-        const int2 td = tiles[t];  // t is uniform: scalar loads
+        const int2 td = tds[lt];  // lt is uniform: one LDS broadcast read (a scalar load here waited on memory)
         const bool mid = td.x < 0;
         const int pair0 = td.x & 0x7fffffff;
         PT p[EPL];
@@ -269,7 +268,9 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
         }
         // the tile's pairs to P with whole-wavefront stores of consecutive pairs; the tile's first pair, when it
         // started in an earlier tile, is finished after the barrier. Runs are taken eight at a time, read with
-        // scalar loads (the run index is uniform), and a lane picks its pair's offset by comparisons.
+        // scalar loads (the run index is uniform), and a lane picks its pair's offset by comparisons. (One fixed
+        // store per lane and round, with idle lanes writing a spare slot so that the wait counts for the next
+        // tile's loads could leave the stores in flight, measured 3.5 us slower on the 2M power-law matrix.)
         int r = td.y, j = pair0 + (mid ? 1 : 0);
         const int jend = pair0 + total;
         while (j < jend) {
@@ -297,16 +298,17 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
             tflag[lt] = (unsigned char)((total > 0 ? 1 : 0) | (mid ? 2 : 0));
         }
     };
-    for (int lt = wave; lt < ntiles; lt += NWAVE) {
-        uint16_t c[EPL];
-        VT v[EPL];
-#pragma unroll
-        for (int j = 0; j < EPL; ++j) {
-            c[j] = cn[j];
-            v[j] = vn[j];
-        }
-        if (lt + NWAVE < ntiles) load_tile(lt + NWAVE, cn, vn);  // the next tile's loads fly meanwhile
-        process(lt, c, v);
+    // two register sets in turn, so that no in-flight load result is moved (a copy of the prefetched registers
+    // made the compiler wait for the prefetch at the loop's back edge): the next tile's loads fly while a tile is
+    // processed; past the chunk's last tile the loads repeat its last tile
+    uint16_t cb[EPL];
+    VT vb[EPL];
+    for (int lt = wave; lt < ntiles; lt += 2 * NWAVE) {
+        load_tile(min(lt + NWAVE, ntiles - 1), cb, vb);
+        process(lt, cn, vn);
+        if (lt + NWAVE >= ntiles) break;
+        load_tile(min(lt + 2 * NWAVE, ntiles - 1), cn, vn);
+        process(lt + NWAVE, cb, vb);
     }
     __syncthreads();
     // pairs that started in an earlier tile of the chunk: head + tails of the tiles before, newest first (a
@@ -318,38 +320,40 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
             s.add(tail[q]);
             if ((tflag[q] & 1) || q == 0) break;
         }
-        const int2 td = tiles[(long)ch.x + lt];
+        const int2 td = tds[lt];
         P[(td.x & 0x7fffffff) + runs[ch.w + td.y].y] = s;
     }
 }
 
 // ---------------------------------------------------------------- the combine kernel
-// blocks[8b ..] = (first row, rows, first pair, pairs, wave split 1, 2, 3, 0); wave split w = (first pair |
-// first row << 16) of wavefront w, at a row boundary (split 0 = (0, 0), split 4 = (pairs, rows)). The block's
-// pair sums are P[first .. first + pairs) (slab by slab, rows ascending inside a slab) and its map entries
-// map[first .. first + pairs) list them in row-major order: LDS index (bits 0-15), row offset in the block
-// (bits 16-30), bit 31 on the row's last pair. Wavefront w sums the rows of its map entries and writes their y.
+// blocks[8b ..] = (first row, rows, first pair, pairs | entries << 16, wave split 1, 2, 3, first map dword);
+// wave split w = (first entry | first row << 16) of wavefront w, at a row boundary (split 0 = (0, 0), split 4 =
+// (entries, rows)). The block's pair sums are P[first .. first + pairs) (slab by slab, rows ascending inside a
+// slab) and its 16-bit map entries list them in row-major order: LDS index (bits 0-14), bit 15 on the row's last
+// entry; an empty row has one entry, the LDS slot kept at zero. A row's offset in the block is the number of
+// row ends before its entries. Wavefront w sums the rows of its entries and writes their y.
 
-// A wavefront sums its rows of one block from LDS (its map range starts and ends on row boundaries): rounds of
-// 64 * E entries start on a multiple of E, a lane takes E consecutive entries (two 16-byte LDS reads), and
-// entries outside [a, b) count as zero and end nothing. Each row's last pair writes the row's sum to yl (and,
-// in a checked plan, its sum of |products| to al).
+// A wavefront sums its rows of one block from LDS (its entry range starts and ends on row boundaries): rounds
+// of 64 * E entries start on a multiple of E, a lane takes E consecutive entries (one 16-byte LDS read), and
+// entries outside [a, b) count as zero and end nothing. Each row's last entry writes the row's sum to yl (and,
+// in a checked plan, its sum of |products| to al), the row being the wavefront's first plus the row ends before.
 template <int K>
 __device__ __forceinline__ void ls_rows(const int* bi, const LsPair<K>* __restrict__ pb,
-                                        const unsigned* __restrict__ lmap, float* __restrict__ yl,
+                                        const uint16_t* __restrict__ lmap, float* __restrict__ yl,
                                         float* __restrict__ al, int wave, int lane) {
     using PT = LsPair<K>;
     constexpr int E = 8;
-    const int np = bi[3];
-    const int a = wave == 0 ? 0 : bi[3 + wave] & 0xffff, b = wave == 3 ? np : bi[4 + wave] & 0xffff;
+    const int ne = bi[3] >> 16;
+    const int a = wave == 0 ? 0 : bi[3 + wave] & 0xffff, b = wave == 3 ? ne : bi[4 + wave] & 0xffff;
+    int row = wave == 0 ? 0 : bi[3 + wave] >> 16;  // the row of the next row end
     PT carry = PT::zero();
     for (int i0 = a & ~(E - 1); i0 < b; i0 += 64 * E) {
         const int ib = i0 + lane * E;
         unsigned m[E];
         {
-            const uint4 u0 = *(const uint4*)(lmap + ib), u1 = *(const uint4*)(lmap + ib + 4);
-            m[0] = u0.x; m[1] = u0.y; m[2] = u0.z; m[3] = u0.w;
-            m[4] = u1.x; m[5] = u1.y; m[6] = u1.z; m[7] = u1.w;
+            const uint4 u = *(const uint4*)(lmap + ib);
+            m[0] = u.x & 0xffffu; m[1] = u.x >> 16; m[2] = u.y & 0xffffu; m[3] = u.y >> 16;
+            m[4] = u.z & 0xffffu; m[5] = u.z >> 16; m[6] = u.w & 0xffffu; m[7] = u.w >> 16;
         }
         PT p[E];
         unsigned f = 0;
@@ -357,8 +361,8 @@ __device__ __forceinline__ void ls_rows(const int* bi, const LsPair<K>* __restri
         for (int j = 0; j < E; ++j) {
             const int i = ib + j;
             const bool in = i >= a && i < b;
-            p[j] = in ? pb[m[j] & 0xffffu] : PT::zero();
-            f |= (in ? (m[j] >> 31) : 0u) << j;
+            p[j] = in ? pb[m[j] & 0x7fffu] : PT::zero();
+            f |= (in ? (m[j] >> 15) : 0u) << j;
         }
         PT seg = PT::zero();
 #pragma unroll
@@ -378,15 +382,19 @@ __device__ __forceinline__ void ls_rows(const int* bi, const LsPair<K>* __restri
         ls_seg_scan<K>(sc, fl);
         PT run = ls_dpp<0x138, 0xf>(sc);
         if (lane == 0) run = carry;
+        int ends;
+        int rr = row + ls_count_prefix(__popc(f), ends);  // the row of this lane's first row end
 #pragma unroll
         for (int j = 0; j < E; ++j) {
             run.add(p[j]);
             if ((f >> j) & 1u) {
-                yl[(m[j] >> 16) & 0x7fffu] = run.value();
-                if constexpr (PT::kAbs) al[(m[j] >> 16) & 0x7fffu] = run.abs_sum();
+                yl[rr] = run.value();
+                if constexpr (PT::kAbs) al[rr] = run.abs_sum();
+                ++rr;
                 run = PT::zero();
             }
         }
+        row += ends;
         carry = ls_readlane63<K>(sc);
     }
 }
@@ -505,7 +513,7 @@ __global__ __launch_bounds__(1024) void lslab_fp64_rows_kernel(const int* __rest
 // value for beta meanwhile).
 template <typename VT, int K, int PAIR_BYTES, int ROWS>
 __global__ __launch_bounds__(256) void lslab_combine_kernel(const int* __restrict__ blocks,
-                                                            const unsigned* __restrict__ map,
+                                                            const uint16_t* __restrict__ map,
                                                             const LsPair<K>* __restrict__ P, float* __restrict__ y,
                                                             float alpha, float beta, const int* __restrict__ rp,
                                                             float rtol, int* __restrict__ chk) {
@@ -513,18 +521,19 @@ __global__ __launch_bounds__(256) void lslab_combine_kernel(const int* __restric
     constexpr bool CHK = PT::kAbs;
     constexpr int PD = (int)sizeof(PT) / 4;
     constexpr int CAP = PAIR_BYTES / (int)sizeof(PT);
-    constexpr int MC = PAIR_BYTES / 4 + 8;  // map entries (+ the tail of a lane's last 16-byte reads)
+    constexpr int MC = CAP + ROWS + 8;  // map entries: pairs and empty rows (+ the tail of a lane's last read)
     __shared__ __attribute__((aligned(16))) PT pb[CAP];
-    __shared__ __attribute__((aligned(16))) unsigned lm[MC];
+    __shared__ __attribute__((aligned(16))) uint16_t lm[MC];
     __shared__ float yl[ROWS];
     __shared__ float al[CHK ? ROWS : 1];
     __shared__ unsigned char fl[CHK ? ROWS : 1];    // 1: the row failed its bound (y written by the FP64 pass)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int bi[8];
     ls_block_info(blocks, blockIdx.x, bi);
-    const int base = bi[2], np = bi[3];
-    ls_copy_lds(map + base, lm, np, wave, lane);
+    const int base = bi[2], np = bi[3] & 0xffff, ne = bi[3] >> 16;
+    ls_copy_lds((const unsigned*)map + bi[7], (unsigned*)lm, (ne + 1) / 2, wave, lane);
     ls_copy_lds((const unsigned*)P + (long)base * PD, (unsigned*)pb, np * PD, wave, lane);
+    if (tid == 0) pb[CAP - 1] = PT::zero();  // the slot of empty rows (a block holds at most CAP - 1 pairs)
     const int r0 = wave == 0 ? 0 : bi[3 + wave] >> 16, r1 = wave == 3 ? bi[1] : bi[4 + wave] >> 16;
     for (int i = r0 + lane; i < r1; i += 64) {  // this wavefront's rows
         yl[i] = 0.f;
@@ -580,7 +589,7 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
         return SPMV_ERROR_INTERNAL;
     }
     const size_t ps = sizeof(float) * (1 + (p->comp ? 1 : 0) + (p->check ? 1 : 0));  // bytes per pair sum
-    const int W = lslab_slab_cols(param0, ps > 4);  // columns per slab
+    const int W = lslab_slab_cols(param0, (int)ps);  // columns per slab
     const int pair_bytes = kLsPairBytes, rows_max = kLsRowsMax;
     const int S = C > 0 ? (int)(((long)C + W - 1) / W) : 0;
     if (S > kLsMaxSlabs) {
@@ -669,7 +678,9 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
     for (int r = 0; r < R;) {
         brow.push_back(r);
         int np = 0, nr = 0;
-        while (r < R && nr < rows_max && (nr == 0 || np + row_pairs[r] <= cap)) {  // a row has <= S <= cap
+        // a block keeps one LDS slot for a zero (the map entry of an empty row points there); a row has
+        // <= S <= cap - 1 pairs
+        while (r < R && nr < rows_max && (nr == 0 || np + row_pairs[r] <= cap - 1)) {
             np += row_pairs[r];
             row_block[r] = (int)brow.size() - 1;
             ++nr;
@@ -696,7 +707,8 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
     for (int i = 0, xc = 0; xc < 8; ++xc)
         for (int q = xc; q < S; q += 8) xorder[i++] = q;
     std::vector<int> delta((size_t)std::max(B, 1) * std::max(S, 1), 0);
-    std::vector<unsigned> hmap((size_t)std::max(npairs, 1L));
+    std::vector<uint16_t> hmap;  // per block, from a dword boundary: its entries in row-major order
+    hmap.reserve((size_t)npairs + (size_t)R + 2 * (size_t)B + 8);
     std::vector<int> hblk((size_t)8 * std::max(B, 1), 0);
     {
         std::vector<int> L((size_t)S + 1);  // L[q]: offset of slab q's run in the block
@@ -710,25 +722,36 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
                 acc += F[(size_t)(b + 1) * S + q] - F[(size_t)b * S + q];
                 delta[(size_t)b * S + q] = m0 + L[q] - F[(size_t)b * S + q];
             }
-            for (int r = r0; r < r1; ++r)
+            const size_t mb = hmap.size();  // the block's first entry (even: a dword boundary)
+            std::vector<int> rent((size_t)(r1 - r0) + 1);  // entry index of each row's first entry in the block
+            for (int r = r0; r < r1; ++r) {
+                rent[(size_t)(r - r0)] = (int)(hmap.size() - mb);
+                if (rstart[r + 1] == rstart[r]) {  // an empty row: one entry, the zero slot, ending the row
+                    hmap.push_back((uint16_t)((cap - 1) | 0x8000));
+                    continue;
+                }
                 for (int k = rstart[r]; k < rstart[r + 1]; ++k) {
                     const int j = rlist[k], q = pair_slab[j];
                     const unsigned idx = (unsigned)(L[q] + j - F[(size_t)b * S + q]);
-                    hmap[k] = idx | ((unsigned)(r - r0) << 16) | (k + 1 == rstart[r + 1] ? 0x80000000u : 0u);
+                    hmap.push_back((uint16_t)(idx | (k + 1 == rstart[r + 1] ? 0x8000u : 0u)));
                 }
+            }
+            const int ne = (int)(hmap.size() - mb);
+            rent[(size_t)(r1 - r0)] = ne;
+            if (hmap.size() & 1) hmap.push_back(0);
             int* d = &hblk[(size_t)8 * b];
             d[0] = r0;
             d[1] = r1 - r0;
             d[2] = m0;
-            d[3] = np;
-            // wave splits at row boundaries, near w * np / 4: (first pair | first row << 16) of wavefront w
+            d[3] = np | (ne << 16);
+            // wave splits at row boundaries, near w * ne / 4: (first entry | first row << 16) of wavefront w
             int r = r0;
             for (int w = 1; w < 4; ++w) {
-                const long want = (long)np * w / 4;
-                while (r < r1 && rstart[r] - m0 < want) ++r;
-                d[3 + w] = (rstart[r] - m0) | ((r - r0) << 16);
+                const long want = (long)ne * w / 4;
+                while (r < r1 && rent[(size_t)(r - r0)] < want) ++r;
+                d[3 + w] = rent[(size_t)(r - r0)] | ((r - r0) << 16);
             }
-            d[7] = 0;
+            d[7] = (int)(mb / 2);  // the block's map, in dwords
         }
     }
 
@@ -859,14 +882,15 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
     MUXI_HIP(hipMalloc(&p->d_ls_ch, sizeof(int) * hch.size()));
     MUXI_HIP(hipMalloc(&p->d_ls_blk, sizeof(int) * hblk.size()));
     MUXI_HIP(hipMalloc(&p->d_ls_F, sizeof(int) * hrun.size()));
-    MUXI_HIP(hipMalloc(&p->d_ls_map, sizeof(unsigned) * hmap.size()));
+    if (hmap.empty()) hmap.assign(2, 0);
+    MUXI_HIP(hipMalloc(&p->d_ls_map, sizeof(uint16_t) * hmap.size()));
     MUXI_HIP(hipMalloc(&p->d_ls_P, ps * (size_t)std::max(npairs, 1L)));
     MUXI_HIP(hipMemcpyAsync(p->d_ls_lc, hlc.data(), sizeof(uint16_t) * hlc.size(), hipMemcpyHostToDevice, s));
     MUXI_HIP(hipMemcpyAsync(p->d_ls_tp, htile.data(), sizeof(int) * htile.size(), hipMemcpyHostToDevice, s));
     MUXI_HIP(hipMemcpyAsync(p->d_ls_ch, hch.data(), sizeof(int) * hch.size(), hipMemcpyHostToDevice, s));
     MUXI_HIP(hipMemcpyAsync(p->d_ls_blk, hblk.data(), sizeof(int) * hblk.size(), hipMemcpyHostToDevice, s));
     MUXI_HIP(hipMemcpyAsync(p->d_ls_F, hrun.data(), sizeof(int) * hrun.size(), hipMemcpyHostToDevice, s));
-    MUXI_HIP(hipMemcpyAsync(p->d_ls_map, hmap.data(), sizeof(unsigned) * hmap.size(), hipMemcpyHostToDevice, s));
+    MUXI_HIP(hipMemcpyAsync(p->d_ls_map, hmap.data(), sizeof(uint16_t) * hmap.size(), hipMemcpyHostToDevice, s));
     int* d_src = nullptr;
     MUXI_HIP(hipMalloc(&d_src, sizeof(int) * hsrc.size()));
     hipError_t e = hipMemcpyAsync(d_src, hsrc.data(), sizeof(int) * hsrc.size(), hipMemcpyHostToDevice, s);
@@ -896,7 +920,7 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
     (void)hipFree(d_src);
     p->scratch_bytes = (sizeof(uint16_t) + vs) * hlc.size() +
                        sizeof(int) * (htile.size() + hch.size() + hblk.size() + hrun.size()) +
-                       sizeof(unsigned) * hmap.size() + ps * (size_t)std::max(npairs, 1L) +
+                       sizeof(uint16_t) * hmap.size() + ps * (size_t)std::max(npairs, 1L) +
                        (p->check ? sizeof(int) * ((size_t)R + 1) : 0);
     return st;
 }
@@ -908,21 +932,35 @@ static void launch_slab(const muxi_plan* p, const float* x, hipStream_t s) {
                        (const int2*)p->d_ls_F, x, p->A.C, (LsPair<K>*)p->d_ls_P, p->d_ls_chk);
 }
 
+template <typename VT, int K, int EPL>
+static void launch_slab_w(const muxi_plan* p, const float* x, hipStream_t s) {
+    constexpr int PB = (int)sizeof(LsPair<K>);
+    switch (lslab_slab_cols(p->param0, PB)) {
+        case kLsSlabCols:
+            if constexpr (lslab_fits(kLsSlabCols, EPL, PB)) launch_slab<VT, K, EPL, kLsSlabCols>(p, x, s);
+            break;
+        case 28672:
+            if constexpr (lslab_fits(28672, EPL, PB)) launch_slab<VT, K, EPL, 28672>(p, x, s);
+            break;
+        case 24576:
+            if constexpr (lslab_fits(24576, EPL, PB)) launch_slab<VT, K, EPL, 24576>(p, x, s);
+            break;
+        default:
+            launch_slab<VT, K, EPL, kLsSlabCols / 2>(p, x, s);
+    }
+}
+
 __global__ void lslab_clear_count_kernel(int* c) { *c = 0; }
 
 template <typename VT, int K>
 static spmv_status_t launch_lslab_k(const muxi_plan* p, const float* x, float* y, float alpha, float beta,
                                     hipStream_t s, hipEvent_t ev_end) {
     if (p->ls_chunks > 0) {
-        // the slab width follows lslab_slab_cols: full slabs only with four nonzeros a lane and 4-byte pair sums
-        const bool four = (p->param0 & 0xff) == 4;
-        if (lslab_slab_cols(p->param0, K != 0) == kLsSlabCols) {
-            if constexpr (K == 0) launch_slab<VT, 0, 4, kLsSlabCols>(p, x, s);
-        } else if (four) {
-            launch_slab<VT, K, 4, kLsSlabCols / 2>(p, x, s);
-        } else {
-            if constexpr (K != 3) launch_slab<VT, K, 8, kLsSlabCols / 2>(p, x, s);  // (K = 3 is built with 4)
-        }
+        // the slab width the plan was built with (lslab_slab_cols)
+        if ((p->param0 & 0xff) == 4)
+            launch_slab_w<VT, K, 4>(p, x, s);
+        else if constexpr (K != 3)  // (a checked Dot2 plan is built with 4)
+            launch_slab_w<VT, K, 8>(p, x, s);
         MUXI_LAUNCH_CHECK();
     } else if (K & 2) {
         hipLaunchKernelGGL(lslab_clear_count_kernel, dim3(1), dim3(1), 0, s, p->d_ls_chk);  // no slab kernel ran

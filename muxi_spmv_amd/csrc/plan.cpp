@@ -109,9 +109,11 @@ static spmv_status_t plan_create_impl(const muxi_csr_t* Ain, spmv_type_t vt, spm
         int k = MUXI_KERNEL_MERGE, q = 4;
         PLAN_TRY(selector_pick_rows(Ain, xt, stream, row_base, rows_full, &k, &q));
         // Compensated sums: the L2 column-slab kernel has no Dot2 variant; its place goes to the LDS column slabs
-        // (Dot2 in half slabs, so at most kLsMaxSlabs * kLsSlabCols / 2 columns with FP32 x), else to merge-path.
+        // (Dot2 with 8-byte pair sums in narrower slabs, at most kLsMaxSlabs of them, with FP32 x), else to
+        // merge-path.
         if (p->comp && (k == MUXI_KERNEL_XSLAB || k == MUXI_KERNEL_LSLAB)) {
-            const bool ls_ok = xt == SPMV_TYPE_FLOAT && (long)Ain->numCols <= (long)kLsMaxSlabs * (kLsSlabCols / 2);
+            const bool ls_ok = xt == SPMV_TYPE_FLOAT &&
+                               (long)Ain->numCols <= (long)kLsMaxSlabs * lslab_slab_cols(4, 8);
             k = ls_ok ? MUXI_KERNEL_LSLAB : MUXI_KERNEL_MERGE;
             q = 4;
         }

@@ -16,10 +16,19 @@ constexpr int kLsPairBytes = 8192;   // combine: LDS for a row block's pair sums
 constexpr int kLsRowsMax = 1024;     // combine: rows per block
 constexpr int kLsMaxRuns = 1024;     // combine blocks one slab-kernel chunk may write to
 constexpr int kLsHalfSlabs = 0x100;  // param0 bit: slabs of kLsSlabCols / 2 columns
-// Columns per slab: full slabs (128 KiB of x) leave room in LDS for the staged pair sums only with four nonzeros
-// a lane and 4-byte pair sums (plain FP32, unchecked); otherwise, or when asked, half slabs.
-inline int lslab_slab_cols(int param0, bool wide_pairs) {
-    return ((param0 & 0xff) == 4 && !wide_pairs && !(param0 & kLsHalfSlabs)) ? kLsSlabCols : kLsSlabCols / 2;
+// Columns per slab: the widest of 32768, 28672, 24576 and 16384 whose FP32 x fits one CU's LDS beside the slab
+// kernel's staged pair sums (1024 lanes x EPL pair sums of pair_bytes), its per-tile head and tail sums and flags
+// and the chunk's tile descriptors (8 bytes a tile); half slabs (16384) when asked. Full slabs take four nonzeros
+// a lane and 4-byte pair sums.
+constexpr bool lslab_fits(int W, int epl, int pair_bytes) {
+    return W * 4 + kLsThreads * epl * pair_bytes + kLsMaxTiles * (2 * pair_bytes + 1 + 8) <= kLsLdsBytes;
+}
+inline int lslab_slab_cols(int param0, int pair_bytes) {
+    const int epl = param0 & 0xff;
+    if (param0 & kLsHalfSlabs) return kLsSlabCols / 2;
+    for (int W : {kLsSlabCols, 28672, 24576})
+        if (lslab_fits(W, epl, pair_bytes)) return W;
+    return kLsSlabCols / 2;
 }
 struct SlabArgs {
     int S;
@@ -93,7 +102,7 @@ struct muxi_plan {
     int* d_ls_ch = nullptr;   // per launch slot: tile0, tiles, first column, run table offset
     int* d_ls_blk = nullptr;  // per combine block: first row, rows, first pair, pairs, wave splits
     int* d_ls_F = nullptr;    // run tables of the chunks: (first pair, P offset) per run, then (INT_MAX, 0)
-    unsigned* d_ls_map = nullptr;
+    uint16_t* d_ls_map = nullptr;  // per combine block, from a dword boundary
     void* d_ls_P = nullptr;
     // checked plans (muxi_plan_create_checked): each row's a-posteriori error bound is tested against rtol * |y|
     // and the rows that fail are recomputed in FP64; d_ls_chk = [count of such rows in the last exec, their ids]

@@ -500,7 +500,9 @@ def _dense_rows_matrix(seed=17):
     for _ in range(3000):
         rows.append(np.sort(rng.choice(C, size=int(rng.integers(0, 9)), replace=False)))
     rows += [np.zeros(0, np.int64)] * 50
-    rows.append(np.array([32767, 32768, 65535, 65536]))
+    # slab boundary columns of every slab width (16384, 24576, 28672, 32768)
+    rows.append(np.array([16383, 16384, 24575, 24576, 28671, 28672, 32767, 32768, 49151, 49152, 57343, 57344, 65535,
+                          65536]))
     rp = np.zeros(len(rows) + 1, np.int32)
     rp[1:] = np.cumsum([r.size for r in rows])
     ci = np.concatenate(rows).astype(np.int32)
@@ -520,7 +522,7 @@ def test_lslab_dense_rows_and_slab_edges(param, comp):
 def test_lslab_integer_exact_many_slabs():
     """A 300 x 8,000,000 matrix (245 slabs of 32768 columns, the last one partial) with integer values and x: y
     equals the host's integer product, so every slab offset, chunk, pair and combine map entry lands in the right
-    place; the same with 16384-column slabs on the first 4,000,000 columns."""
+    place; the same with 28672- and 16384-column slabs on the first 4,000,000 columns."""
     rng = np.random.default_rng(23)
     R, C = 300, 8_000_000
     lens = rng.integers(0, 4000, R)
@@ -535,10 +537,10 @@ def test_lslab_integer_exact_many_slabs():
     np.add.at(exp, rows, h.values.astype(np.int64) * xi[h.colIdx])
     y, _, _, _ = run_plan(h, "lslab", 4, x=xi.astype(np.float32))  # full slabs (half ones would need 489)
     assert np.array_equal(y.cpu().numpy(), exp.astype(np.float32))
-    for param in (8, 0x104):  # 16384-column slabs: over the 256 a plan takes
+    for param in (8, 0x104):  # 28672- and 16384-column slabs: over the 256 a plan takes
         with pytest.raises(api.SpmvError, match="INVALID_MATRIX"):
             run_plan(h, "lslab", param, x=xi.astype(np.float32))
-    # the same pattern on its first 4,000,000 columns (245 half slabs) with 16384-column slabs
+    # the same pattern on its first 4,000,000 columns (140 slabs of 28672, 245 of 16384)
     keep = h.colIdx < 4_000_000
     rows = np.repeat(np.arange(R), np.diff(h.rowPtr))[keep]
     rp2 = np.zeros(R + 1, np.int32)
