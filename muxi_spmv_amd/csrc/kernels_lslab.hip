@@ -404,14 +404,15 @@ __device__ __forceinline__ void ls_block_info(const int* __restrict__ blocks, in
     for (int k = 0; k < 8; ++k) bi[k] = blocks[8 * (long)b + k];  // uniform: scalar loads
 }
 
-// Copies n dwords from global src to LDS dst with the wavefront's direct global->LDS loads (no register round
-// trip; nothing waits here). The workgroup's four wavefronts take every fourth 64-dword piece.
-__device__ __forceinline__ void ls_copy_lds(const unsigned* __restrict__ src, unsigned* __restrict__ dst, int n,
-                                            int wave, int lane) {
+// Copies n 16-byte units from global src to LDS dst with the wavefront's direct global->LDS loads, 16 bytes a
+// lane (no register round trip; nothing waits here). The workgroup's four wavefronts take every fourth 64-unit
+// piece.
+__device__ __forceinline__ void ls_copy_lds16(const uint4* __restrict__ src, uint4* __restrict__ dst, int n,
+                                              int wave, int lane) {
     for (int e0 = wave * 64; e0 < n; e0 += 256)
         if (e0 + lane < n)
             __builtin_amdgcn_global_load_lds((const void*)(src + e0 + lane),
-                                             (__attribute__((address_space(3))) void*)(dst + e0), 4, 0, 0);
+                                             (__attribute__((address_space(3))) void*)(dst + e0), 16, 0, 0);
 }
 
 // ---- checked plans: a row's a-posteriori error bound, and its FP64 recomputation
@@ -519,10 +520,9 @@ __global__ __launch_bounds__(256) void lslab_combine_kernel(const int* __restric
                                                             float rtol, int* __restrict__ chk) {
     using PT = LsPair<K>;
     constexpr bool CHK = PT::kAbs;
-    constexpr int PD = (int)sizeof(PT) / 4;
     constexpr int CAP = PAIR_BYTES / (int)sizeof(PT);
     constexpr int MC = CAP + ROWS + 8;  // map entries: pairs and empty rows (+ the tail of a lane's last read)
-    __shared__ __attribute__((aligned(16))) PT pb[CAP];
+    __shared__ __attribute__((aligned(16))) PT pb[CAP + 4];  // CAP + 3: a zero, out of the copies' reach
     __shared__ __attribute__((aligned(16))) uint16_t lm[MC];
     __shared__ float yl[ROWS];
     __shared__ float al[CHK ? ROWS : 1];
@@ -531,9 +531,10 @@ __global__ __launch_bounds__(256) void lslab_combine_kernel(const int* __restric
     int bi[8];
     ls_block_info(blocks, blockIdx.x, bi);
     const int base = bi[2], np = bi[3] & 0xffff, ne = bi[3] >> 16;
-    ls_copy_lds((const unsigned*)map + bi[7], (unsigned*)lm, (ne + 1) / 2, wave, lane);
-    ls_copy_lds((const unsigned*)P + (long)base * PD, (unsigned*)pb, np * PD, wave, lane);
-    if (tid == 0) pb[CAP - 1] = PT::zero();  // the slot of empty rows (a block holds at most CAP - 1 pairs)
+    ls_copy_lds16((const uint4*)map + bi[7], (uint4*)lm, (ne + 7) / 8, wave, lane);
+    ls_copy_lds16((const uint4*)((const char*)P + (long)base * (long)sizeof(PT)), (uint4*)pb,
+                  (np * (int)sizeof(PT) + 15) / 16, wave, lane);
+    if (tid == 0) pb[CAP + 3] = PT::zero();  // the slot of empty rows
     const int r0 = wave == 0 ? 0 : bi[3 + wave] >> 16, r1 = wave == 3 ? bi[1] : bi[4 + wave] >> 16;
     for (int i = r0 + lane; i < r1; i += 64) {  // this wavefront's rows
         yl[i] = 0.f;
@@ -678,9 +679,7 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
     for (int r = 0; r < R;) {
         brow.push_back(r);
         int np = 0, nr = 0;
-        // a block keeps one LDS slot for a zero (the map entry of an empty row points there); a row has
-        // <= S <= cap - 1 pairs
-        while (r < R && nr < rows_max && (nr == 0 || np + row_pairs[r] <= cap - 1)) {
+        while (r < R && nr < rows_max && (nr == 0 || np + row_pairs[r] <= cap)) {  // a row has <= S <= cap
             np += row_pairs[r];
             row_block[r] = (int)brow.size() - 1;
             ++nr;
@@ -707,6 +706,9 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
     for (int i = 0, xc = 0; xc < 8; ++xc)
         for (int q = xc; q < S; q += 8) xorder[i++] = q;
     std::vector<int> delta((size_t)std::max(B, 1) * std::max(S, 1), 0);
+    // a block's pair sums start on a 16-byte boundary of P (the combine copies them to LDS 16 bytes a lane)
+    const long palign = ps == 8 ? 2 : 4;
+    long pnext = 0;
     std::vector<uint16_t> hmap;  // per block, from a dword boundary: its entries in row-major order
     hmap.reserve((size_t)npairs + (size_t)R + 2 * (size_t)B + 8);
     std::vector<int> hblk((size_t)8 * std::max(B, 1), 0);
@@ -714,7 +716,9 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
         std::vector<int> L((size_t)S + 1);  // L[q]: offset of slab q's run in the block
         for (int b = 0; b < B; ++b) {
             const int r0 = brow[b], r1 = brow[b + 1];
-            const int m0 = rstart[r0], np = rstart[r1] - m0;
+            const int np = rstart[r1] - rstart[r0];
+            const int m0 = (int)pnext;  // the block's first pair sum in P
+            pnext = (pnext + np + palign - 1) / palign * palign;
             int acc = 0;
             for (int i = 0; i < S; ++i) {
                 const int q = xorder[i];
@@ -722,12 +726,12 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
                 acc += F[(size_t)(b + 1) * S + q] - F[(size_t)b * S + q];
                 delta[(size_t)b * S + q] = m0 + L[q] - F[(size_t)b * S + q];
             }
-            const size_t mb = hmap.size();  // the block's first entry (even: a dword boundary)
+            const size_t mb = hmap.size();  // the block's first entry (a multiple of 8: a 16-byte boundary)
             std::vector<int> rent((size_t)(r1 - r0) + 1);  // entry index of each row's first entry in the block
             for (int r = r0; r < r1; ++r) {
                 rent[(size_t)(r - r0)] = (int)(hmap.size() - mb);
                 if (rstart[r + 1] == rstart[r]) {  // an empty row: one entry, the zero slot, ending the row
-                    hmap.push_back((uint16_t)((cap - 1) | 0x8000));
+                    hmap.push_back((uint16_t)((cap + 3) | 0x8000));
                     continue;
                 }
                 for (int k = rstart[r]; k < rstart[r + 1]; ++k) {
@@ -738,7 +742,7 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
             }
             const int ne = (int)(hmap.size() - mb);
             rent[(size_t)(r1 - r0)] = ne;
-            if (hmap.size() & 1) hmap.push_back(0);
+            while (hmap.size() & 7) hmap.push_back(0);  // the next block's map on a 16-byte boundary
             int* d = &hblk[(size_t)8 * b];
             d[0] = r0;
             d[1] = r1 - r0;
@@ -751,7 +755,7 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
                 while (r < r1 && rent[(size_t)(r - r0)] < want) ++r;
                 d[3 + w] = rent[(size_t)(r - r0)] | ((r - r0) << 16);
             }
-            d[7] = (int)(mb / 2);  // the block's map, in dwords
+            d[7] = (int)(mb / 8);  // the block's map, in 16-byte units
         }
     }
 
@@ -882,9 +886,10 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
     MUXI_HIP(hipMalloc(&p->d_ls_ch, sizeof(int) * hch.size()));
     MUXI_HIP(hipMalloc(&p->d_ls_blk, sizeof(int) * hblk.size()));
     MUXI_HIP(hipMalloc(&p->d_ls_F, sizeof(int) * hrun.size()));
-    if (hmap.empty()) hmap.assign(2, 0);
+    if (hmap.empty()) hmap.assign(8, 0);
+    p->ls_pspan = pnext;
     MUXI_HIP(hipMalloc(&p->d_ls_map, sizeof(uint16_t) * hmap.size()));
-    MUXI_HIP(hipMalloc(&p->d_ls_P, ps * (size_t)std::max(npairs, 1L)));
+    MUXI_HIP(hipMalloc(&p->d_ls_P, ps * (size_t)pnext + 16));  // (+16: the last 16-byte copy may run over)
     MUXI_HIP(hipMemcpyAsync(p->d_ls_lc, hlc.data(), sizeof(uint16_t) * hlc.size(), hipMemcpyHostToDevice, s));
     MUXI_HIP(hipMemcpyAsync(p->d_ls_tp, htile.data(), sizeof(int) * htile.size(), hipMemcpyHostToDevice, s));
     MUXI_HIP(hipMemcpyAsync(p->d_ls_ch, hch.data(), sizeof(int) * hch.size(), hipMemcpyHostToDevice, s));
@@ -920,7 +925,7 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
     (void)hipFree(d_src);
     p->scratch_bytes = (sizeof(uint16_t) + vs) * hlc.size() +
                        sizeof(int) * (htile.size() + hch.size() + hblk.size() + hrun.size()) +
-                       sizeof(uint16_t) * hmap.size() + ps * (size_t)std::max(npairs, 1L) +
+                       sizeof(uint16_t) * hmap.size() + ps * (size_t)pnext + 16 +
                        (p->check ? sizeof(int) * ((size_t)R + 1) : 0);
     return st;
 }

@@ -95,7 +95,7 @@ struct muxi_plan {
     // LDS column slabs (lslab): slab streams (16-bit column offsets with row-end bits, values), tile
     // descriptors, chunk descriptors (launch order), combine blocks, run tables, row-major pair map, pair sums
     int ls_S = 0, ls_chunks = 0, ls_blocks = 0;
-    long ls_pairs = 0;
+    long ls_pairs = 0, ls_pspan = 0;  // pair sums, and P's length in pair sums (blocks start 16-byte aligned)
     uint16_t* d_ls_lc = nullptr;
     void* d_ls_lv = nullptr;
     int* d_ls_tp = nullptr;   // per tile: (first pair | mid-pair bit, run in its chunk)
