@@ -662,6 +662,86 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
     std::vector<int> pslab((size_t)S + 1, 0);  // first pair of each slab
     for (int q = 0; q < S; ++q) pslab[q + 1] = ends_before[off[q + 1]];
 
+    // ---- chunks (slab-kernel workgroups): each slab's stream cut at row ends into pieces. All pieces run at once
+    // when their count is the number of CUs (one 1024-lane workgroup fills a CU), so the pieces are apportioned
+    // to the slabs by their nonzeros (largest remainders, at least one a slab) for exactly `cus` of them -- more,
+    // in whole multiples of `cus`, when a piece would pass kLsMaxTiles tiles -- and cut at the first row end at or
+    // past each even share. Workgroup b runs on XCD b mod 8: the pieces go to the XCDs in slab order, an equal
+    // count each, so that every XCD gets the same work and reads few slabs of x into its L2.
+    int dev = 0, cus = 256;
+    MUXI_HIP(hipGetDevice(&dev));
+    MUXI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    cus = std::max(8, cus / 8 * 8);
+    // a (row, slab) pair without repeated columns holds at most W nonzeros, so a piece cut at the first row end
+    // past a share of at most max_piece stays within kLsMaxTiles tiles
+    const long max_piece = (long)kLsMaxTiles * TILE - W;
+    long nchunks = (long)cus * std::max(1L, ((long)N + (long)cus * max_piece - 1) / ((long)cus * max_piece));
+    long nslabs_used = 0;
+    for (int q = 0; q < S; ++q) nslabs_used += off[q + 1] > off[q] ? 1 : 0;
+    nchunks = std::max(nchunks, nslabs_used);
+    std::vector<long> kq((size_t)S, 0);
+    {
+        std::vector<std::pair<double, int>> rem;
+        long used = 0;
+        for (int q = 0; q < S; ++q) {
+            const long nq = off[q + 1] - off[q];
+            if (nq == 0) continue;
+            const double xq = (double)nchunks * (double)nq / (double)std::max(N, 1);
+            kq[q] = std::max(1L, (long)xq);
+            used += kq[q];
+            rem.push_back({xq - (double)(long)xq, q});
+        }
+        std::sort(rem.begin(), rem.end(), [](const std::pair<double, int>& u, const std::pair<double, int>& v) {
+            return u.first > v.first || (u.first == v.first && u.second < v.second);
+        });
+        for (size_t i = 0; used < nchunks && i < rem.size(); ++i, ++used) ++kq[rem[i].second];
+        // past the count (the one-a-slab floor): take pieces back from the slabs with the largest shares
+        while (used > nchunks) {
+            int best = -1;
+            for (int q = 0; q < S; ++q)
+                if (kq[q] > 1 && (best < 0 || (off[q + 1] - off[q]) * kq[best] < (off[best + 1] - off[best]) * kq[q]))
+                    best = q;
+            if (best < 0) break;
+            --kq[best];
+            --used;
+        }
+    }
+    struct Piece {
+        int slab;
+        long a, n;  // stream positions [a, a + n)
+    };
+    std::vector<Piece> pieces;
+    for (int q = 0; q < S; ++q) {
+        const long e = off[q + 1], nq = e - off[q];
+        if (nq == 0) continue;
+        long a = off[q];
+        for (long c = 1; a < e; ++c) {
+            // the c-th cut: the first row end at or past c / kq of the slab (a share an earlier cut already passed
+            // is skipped), within max_piece of the last cut
+            long want = std::min(off[q] + (nq * c + kq[q] - 1) / kq[q], e);
+            if (c < kq[q] && want <= a) continue;
+            want = std::min(std::max(want, a + 1), a + max_piece);
+            long i = want - 1;
+            while (i + 1 < e && !end[i]) ++i;
+            if (c >= kq[q] && e - a <= max_piece) i = e - 1;
+            if (i + 1 - a > (long)kLsMaxTiles * TILE) {
+                set_error("lslab: a row holds more than " + std::to_string(W) +
+                          " nonzeros in one column slab (repeated columns)");
+                return SPMV_ERROR_INVALID_MATRIX;
+            }
+            pieces.push_back({q, a, i + 1 - a});
+            a = i + 1;
+        }
+    }
+    // XCD of each piece: equal counts in slab order; a slab's XCD is that of its first piece
+    const int np_ = (int)pieces.size();
+    std::vector<int> pxcd((size_t)np_);
+    std::vector<int> slab_xcd((size_t)S, 0);
+    for (int i = np_ - 1; i >= 0; --i) {
+        pxcd[(size_t)i] = (int)((long)i * 8 / std::max(np_, 1));
+        slab_xcd[(size_t)pieces[(size_t)i].slab] = pxcd[(size_t)i];
+    }
+
     // ---- combine blocks: rows cut so that a block holds at most kLsRowsMax rows and its pair sums (and map
     // entries) fit the LDS; a block's pairs take P[rstart[first row] ..) in (slab, row) order
     const int cap = pair_bytes / (int)ps;
@@ -699,12 +779,11 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
             F[(size_t)b * S + q] = j;
         }
     }
-    // Inside a block the slabs' runs are ordered by (slab mod 8, slab): the chunks of slab q run on XCD q mod 8
-    // (see the launch order below), so each XCD writes one contiguous part of every block and no cache line of
-    // P is written from two L2s.
+    // Inside a block the slabs' runs are ordered by the XCD their chunks run on (then by slab), so each XCD writes
+    // one contiguous part of every block and few cache lines of P are written from two L2s.
     std::vector<int> xorder((size_t)S);
-    for (int i = 0, xc = 0; xc < 8; ++xc)
-        for (int q = xc; q < S; q += 8) xorder[i++] = q;
+    for (int q = 0; q < S; ++q) xorder[q] = q;
+    std::stable_sort(xorder.begin(), xorder.end(), [&](int u, int v) { return slab_xcd[u] < slab_xcd[v]; });
     std::vector<int> delta((size_t)std::max(B, 1) * std::max(S, 1), 0);
     // a block's pair sums start on a 16-byte boundary of P (the combine copies them to LDS 16 bytes a lane)
     const long palign = ps == 8 ? 2 : 4;
@@ -759,52 +838,9 @@ spmv_status_t lslab_build(muxi_plan* p, int param0, spmv_type_t vt, hipStream_t 
         }
     }
 
-    // ---- chunks: each slab's stream cut at row ends into about equal parts, about one per CU in all; a chunk
-    // also ends before its pairs would touch more than kLsMaxRuns combine blocks (a block change is a row end)
-    int dev = 0, cus = 256;
-    MUXI_HIP(hipGetDevice(&dev));
-    MUXI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const long target_chunks = std::max(1, cus);
-    // a cut goes at the first row end at or past the wanted size; a (row, slab) pair without repeated columns
-    // holds at most W nonzeros, so a chunk stays within kLsMaxTiles tiles
-    const long max_piece = (long)kLsMaxTiles * TILE - W;
-    struct Piece {
-        int slab;
-        long a, n;  // stream positions [a, a + n)
-    };
-    std::vector<Piece> pieces;
-    for (int q = 0; q < S; ++q) {
-        const long e = off[q + 1], nq = e - off[q];
-        if (nq == 0) continue;
-        const long k = std::max(1L, (long)std::llround((double)target_chunks * (double)nq / (double)std::max(N, 1)));
-        const long want = std::min((nq + k - 1) / k, max_piece);
-        long a = off[q];
-        while (a < e) {
-            long i = a;
-            int runs = 1, blk = row_block[srow[a]];
-            while (i + 1 < e) {
-                if (end[i] && i + 1 - a >= want) break;
-                const int nb = row_block[srow[i + 1]];
-                if (nb != blk) {
-                    if (runs == kLsMaxRuns) break;  // i ends a row: the next element is in another block
-                    ++runs;
-                    blk = nb;
-                }
-                ++i;
-            }
-            if (i + 1 - a > (long)kLsMaxTiles * TILE) {
-                set_error("lslab: a row holds more than " + std::to_string(W) +
-                          " nonzeros in one column slab (repeated columns)");
-                return SPMV_ERROR_INVALID_MATRIX;
-            }
-            pieces.push_back({q, a, i + 1 - a});
-            a = i + 1;
-        }
-    }
-    // launch order: slab q's chunks on XCD q mod 8 (workgroup b runs on XCD b mod 8), so a slab of x is read
-    // into one L2 and shared there
+    // launch order: workgroup b runs on XCD b mod 8, so XCD x's i-th piece takes slot 8 i + x
     std::vector<std::vector<int>> per_xcd(8);
-    for (int i = 0; i < (int)pieces.size(); ++i) per_xcd[pieces[i].slab % 8].push_back(i);
+    for (int i = 0; i < np_; ++i) per_xcd[(size_t)pxcd[(size_t)i]].push_back(i);
     size_t rounds = 0;
     for (auto& v : per_xcd) rounds = std::max(rounds, v.size());
     std::vector<int> order;
