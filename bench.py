@@ -46,7 +46,7 @@ CANDIDATES = [("merge", 4), ("merge", 8), ("merge", 16), ("csr5", 256), ("csr5",
 # profiles/xslab_spmv_powerlaw_2m_fp32.md, profiles/ell_spmv_laplacian_4096_fp64.md and
 # profiles/ell_spmv_uniform_100k_fp32.md. Configurations not measured report null.
 MEASURED_TRAFFIC = {
-    ("powerlaw_2m_fp32", "lslab", 4): int(((48184.8 + 25049.7) * 2.0 + (26488.3 + 9748.6) * 1.0) * 1024),
+    ("powerlaw_2m_fp32", "lslab", 4): int(((48557.1 + 18917.7) * 2.0 + (28110.1 + 9772.2) * 1.0) * 1024),
     ("powerlaw_2m_fp32", "xslab", 1028): int((66336.6 * 2.0 + 15477.4 * 1.0) * 1024),
     ("powerlaw_2m_fp32", "xslab", 1032): int((68168.4 * 2.0 + 15791.2 * 1.0) * 1024),
     ("laplacian_4096_fp64", "ell", 5): int((483392.3 * 2.0 + 131072.0 * 1.0) * 1024),
