@@ -335,9 +335,10 @@ __global__ __launch_bounds__(kLsThreads) void lslab_slab_kernel(const int4* __re
 
 // A wavefront sums its rows of one block from LDS (its entry range starts and ends on row boundaries): rounds
 // of 64 * E entries start on a multiple of E, a lane takes E consecutive entries (one 16-byte LDS read), and
-// entries outside [a, b) count as zero and end nothing. Each row's last entry writes the row's sum to yl (and,
-// in a checked plan, its sum of |products| to al), the row being the wavefront's first plus the row ends before.
-template <int K>
+// entries outside [a, b) read the zero slot pb[ZERO] and end nothing. Each row's last entry writes the row's sum
+// to yl (and, in a checked plan, its sum of |products| to al), the row being the wavefront's first plus the row
+// ends before.
+template <int K, int ZERO>
 __device__ __forceinline__ void ls_rows(const int* bi, const LsPair<K>* __restrict__ pb,
                                         const uint16_t* __restrict__ lmap, float* __restrict__ yl,
                                         float* __restrict__ al, int wave, int lane) {
@@ -357,11 +358,13 @@ __device__ __forceinline__ void ls_rows(const int* bi, const LsPair<K>* __restri
         }
         PT p[E];
         unsigned f = 0;
+        // an entry outside [a, b) reads the zero slot: every lane issues its E reads at once (a guarded read
+        // became a branch that waited for the reads before it)
 #pragma unroll
         for (int j = 0; j < E; ++j) {
             const int i = ib + j;
             const bool in = i >= a && i < b;
-            p[j] = in ? pb[m[j] & 0x7fffu] : PT::zero();
+            p[j] = pb[in ? (m[j] & 0x7fffu) : (unsigned)ZERO];
             f |= (in ? (m[j] >> 15) : 0u) << j;
         }
         PT seg = PT::zero();
@@ -545,7 +548,7 @@ __global__ __launch_bounds__(256) void lslab_combine_kernel(const int* __restric
     }
     __builtin_amdgcn_s_waitcnt(0);                      // this wavefront's direct loads have landed
     __syncthreads();
-    ls_rows<K>(bi, pb, lm, yl, al, wave, lane);
+    ls_rows<K, CAP + 3>(bi, pb, lm, yl, al, wave, lane);
     if constexpr (CHK) {
         for (int i0 = r0; i0 < r1; i0 += 64) {
             const int i = i0 + lane;
