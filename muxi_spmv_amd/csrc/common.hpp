@@ -61,6 +61,25 @@ __host__ __device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
 }
 
 // ---------------------------------------------------------------- accumulators
+// A cross-lane DPP move of one value (GFX9 controls: row_shr:n = 0x110 | n inside a row of 16 lanes,
+// row_bcast:15 = 0x142, row_bcast:31 = 0x143, wave_shr:1 = 0x138; ROWS masks the rows written); a lane without a
+// source reads 0. Runs in the ALUs, without the LDS round trip of a bpermute.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_move(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(dpp_move<CTRL, ROWS>(__float_as_int(v)));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)dpp_move<CTRL, ROWS>((int)(unsigned)b);
+    const unsigned hi = (unsigned)dpp_move<CTRL, ROWS>((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 template <typename AT, bool COMP> struct Acc;
 
 template <typename AT> struct Acc<AT, false> {
@@ -73,6 +92,8 @@ template <typename AT> struct Acc<AT, false> {
     __device__ __forceinline__ Acc shfl_xor(int m) const { Acc r; r.s = __shfl_xor(s, m); return r; }
     __device__ __forceinline__ Acc shfl_up(int d) const { Acc r; r.s = __shfl_up(s, d); return r; }
     __device__ __forceinline__ Acc shfl(int src) const { Acc r; r.s = __shfl(s, src); return r; }
+    template <int CTRL, int ROWS>
+    __device__ __forceinline__ Acc dpp() const { Acc r; r.s = dpp_move<CTRL, ROWS>(s); return r; }
 };
 
 // Dot2: s + c carries the sum to about twice the working precision. The arithmetic is written out here with
@@ -110,6 +131,13 @@ template <> struct Acc<float, true> {
     __device__ __forceinline__ Acc shfl_xor(int m) const { Acc r; r.s = __shfl_xor(s, m); r.c = __shfl_xor(c, m); return r; }
     __device__ __forceinline__ Acc shfl_up(int d) const { Acc r; r.s = __shfl_up(s, d); r.c = __shfl_up(c, d); return r; }
     __device__ __forceinline__ Acc shfl(int src) const { Acc r; r.s = __shfl(s, src); r.c = __shfl(c, src); return r; }
+    template <int CTRL, int ROWS>
+    __device__ __forceinline__ Acc dpp() const {
+        Acc r;
+        r.s = dpp_move<CTRL, ROWS>(s);
+        r.c = dpp_move<CTRL, ROWS>(c);
+        return r;
+    }
 };
 
 // Product term stored between phases (LDS staging): one AT, or (p, err) when compensated.

@@ -6,24 +6,34 @@
 
 namespace muxi {
 
+// One step of a wavefront's segmented scan by a DPP move: (f1,v1) then (f2,v2) -> (f1|f2, f2 ? v2 : v1+v2); a
+// lane without a source takes (0, 0).
+template <int CTRL, int ROWS, typename A>
+__device__ __forceinline__ void seg_scan_step(A& v, int& f) {
+    A u = v.template dpp<CTRL, ROWS>();
+    const int fu = dpp_move<CTRL, ROWS>(f);
+    if (!f) {
+        u.add(v);
+        v = u;
+    }
+    f |= fu;
+}
+
 // Workgroup-wide inclusive segmented scan over (flag, value) in lane order:
 // (f1,v1) then (f2,v2) -> (f1|f2, f2 ? v2 : v1+v2). Returns the exclusive
-// value (sum carried into this lane) and the workgroup total in *tot.
+// value (sum carried into this lane) and the workgroup total in *tot. Inside
+// a wavefront: shifts of 1, 2, 4, 8 within each row of 16 lanes, then row 0's
+// total into rows 1 and 3 and rows 0-1's total into rows 2 and 3 (DPP, no
+// LDS); across wavefronts through LDS.
 template <typename A, int NW>
 __device__ __forceinline__ A block_seg_scan_excl(A v, int f, A* s_agg, int* s_flag, A* tot) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        A vu = v.shfl_up(d);
-        int fu = __shfl_up(f, d);
-        if (lane >= d) {
-            if (!f) {
-                vu.add(v);
-                v = vu;
-            }
-            f |= fu;
-        }
-    }
+    seg_scan_step<0x111, 0xf>(v, f);
+    seg_scan_step<0x112, 0xf>(v, f);
+    seg_scan_step<0x114, 0xf>(v, f);
+    seg_scan_step<0x118, 0xf>(v, f);
+    seg_scan_step<0x142, 0xa>(v, f);
+    seg_scan_step<0x143, 0xc>(v, f);
     if (lane == 63) {
         s_agg[w] = v;
         s_flag[w] = f;
@@ -43,7 +53,7 @@ __device__ __forceinline__ A block_seg_scan_excl(A v, int f, A* s_agg, int* s_fl
         t.add(v);
         inc = t;
     }
-    A ex = inc.shfl_up(1);
+    A ex = inc.template dpp<0x138, 0xf>();  // wave_shr:1
     if (lane == 0) ex = pre;
     A t;
     t.zero();
