@@ -260,6 +260,7 @@ def main(argv=None) -> int:
                      "kernels": "slab kernel + combine" if info.kernel == "lslab" else f"{info.kernel} main kernel"
                      + (" + carry fix-up" if info.kernel in ("merge", "csr5", "adaptive", "xslab") else ""),
                      "launch_us_mean": round(ems * 1e3, 3), "main_kernel_us_mean": round(kms * 1e3, 3),
+                     "launch_us_median": round(meas.exec_ms_median * 1e3, 3), "launch_samples": len(meas.exec_ms),
                      "bytes_per_launch": bytes_step},
         "sweep_ms": sweep,
         "selector": selector,

@@ -14,6 +14,9 @@
 //   spmv_csr_transpose (src/spmv/csr/spmv_csr.cuh:1107-1114)
 //   io::MTXHeader, io::readMTXHeader, io::readMTXFile, io::writeMTXFile,
 //   io::readMTXToCOO, io::writeCOOToMTX (src/generators/mtx_io.h)
+//   benchmark::PerformanceResult, benchmark::BenchmarkConfig, calculateGFlops, calculateBandwidth,
+//   measureCSRPerformance<T>, measureCSR5Performance<T>, compareWithCusparse<T> (no counterpart: unsupported),
+//   writeResultsJSON, printPerformanceResult (src/benchmark/performance_benchmark.h)
 // Instantiated for float and double, as the reference is.
 #ifndef MUXI_SPMV_HPP_
 #define MUXI_SPMV_HPP_
@@ -24,7 +27,10 @@
 #include <cstdlib>
 #include <type_traits>
 
+#include <vector>
+
 #include "muxi_spmv.h"
+#include "muxi_spmv_bench.h"
 #include "muxi_spmv_formats.h"
 #include "muxi_spmv_io.h"
 #include "muxi_spmv_prep.h"
@@ -547,6 +553,82 @@ inline spmv_status_t writeCOOToMTX(const char* filename, const COOMatrix<T>& mat
                                          : ::muxi_mtx_write_coo_f64(filename, &c);
 }
 }  // namespace io
+
+// ---- the benchmark layer (src/benchmark/performance_benchmark.h), over muxi_spmv_bench.h
+namespace benchmark {
+
+using PerformanceResult = ::muxi_perf_result_t;  // the reference's fields in its order
+
+// BenchmarkConfig with the reference's defaults (performance_benchmark.h:58-71), laid out as muxi_bench_config_t
+struct BenchmarkConfig {
+    int warmupIterations = 3;
+    int measureIterations = 10;
+    bool checkCorrectness = true;
+    bool syncAfterEach = true;
+    hipStream_t stream = 0;
+    spmv_type_t floatType = SPMV_TYPE_FLOAT;
+    bool compareCusparse = false;
+
+    const ::muxi_bench_config_t* abi() const { return reinterpret_cast<const ::muxi_bench_config_t*>(this); }
+};
+static_assert(std::is_standard_layout<BenchmarkConfig>::value && sizeof(BenchmarkConfig) == sizeof(::muxi_bench_config_t) &&
+                  offsetof(BenchmarkConfig, warmupIterations) == offsetof(::muxi_bench_config_t, warmupIterations) &&
+                  offsetof(BenchmarkConfig, measureIterations) == offsetof(::muxi_bench_config_t, measureIterations) &&
+                  offsetof(BenchmarkConfig, checkCorrectness) == offsetof(::muxi_bench_config_t, checkCorrectness) &&
+                  offsetof(BenchmarkConfig, syncAfterEach) == offsetof(::muxi_bench_config_t, syncAfterEach) &&
+                  offsetof(BenchmarkConfig, stream) == offsetof(::muxi_bench_config_t, stream) &&
+                  offsetof(BenchmarkConfig, floatType) == offsetof(::muxi_bench_config_t, floatType) &&
+                  offsetof(BenchmarkConfig, compareCusparse) == offsetof(::muxi_bench_config_t, compareCusparse),
+              "BenchmarkConfig must be laid out as muxi_bench_config_t");
+
+inline double calculateGFlops(int nnz, double timeMs) { return ::muxi_calculate_gflops(nnz, timeMs); }
+inline double calculateBandwidth(int nnz, int numRows, int numCols, double timeMs, int floatBytes = 4) {
+    return ::muxi_calculate_bandwidth(nnz, numRows, numCols, timeMs, floatBytes);
+}
+
+namespace detail {
+template <typename T>
+inline BenchmarkConfig typed(const BenchmarkConfig& c) {
+    BenchmarkConfig t = c;
+    t.floatType = std::is_same<T, double>::value ? SPMV_TYPE_DOUBLE : SPMV_TYPE_FLOAT;
+    return t;
+}
+}  // namespace detail
+
+template <typename FloatType>
+inline spmv_status_t measureCSRPerformance(const CSRMatrix<FloatType>& matrix, const FloatType* d_x, FloatType* d_y,
+                                           const BenchmarkConfig& config, spmv_handle_t* handle,
+                                           PerformanceResult& result) {
+    static_assert(muxi_spmv::detail::is_fp<FloatType>::value, "float or double");
+    const ::muxi_csr_t c = matrix.abi();
+    const BenchmarkConfig t = detail::typed<FloatType>(config);
+    return ::muxi_measure_csr_performance(&c, d_x, d_y, t.abi(), handle, &result);
+}
+
+template <typename FloatType>
+inline spmv_status_t measureCSR5Performance(CSRMatrix<FloatType>& csr, CSR5Matrix<FloatType>& csr5,
+                                            const FloatType* d_x, FloatType* d_y, const BenchmarkConfig& config,
+                                            spmv_handle_t* handle, PerformanceResult& result, int sigma) {
+    static_assert(muxi_spmv::detail::is_fp<FloatType>::value, "float or double");
+    const ::muxi_csr_t c = csr.abi();
+    const BenchmarkConfig t = detail::typed<FloatType>(config);
+    return ::muxi_measure_csr5_performance(&c, csr5.abi(), d_x, d_y, t.abi(), handle, &result, sigma);
+}
+
+// The reference compiles its cuSPARSE comparison out (ENABLE_CUSPARSE is never defined) and returns this status.
+template <typename FloatType>
+inline spmv_status_t compareWithCusparse(const CSRMatrix<FloatType>&, const FloatType*, FloatType*,
+                                         const BenchmarkConfig&, double&, double&) {
+    return SPMV_ERROR_UNSUPPORTED_FORMAT;
+}
+
+inline void printPerformanceResult(const PerformanceResult& result) { ::muxi_print_performance_result(&result); }
+
+inline spmv_status_t writeResultsJSON(const std::vector<PerformanceResult>& results, const char* filename) {
+    return ::muxi_write_results_json(results.data(), (int)results.size(), filename);
+}
+
+}  // namespace benchmark
 
 }  // namespace muxi_spmv
 

@@ -84,6 +84,25 @@ class muxi_spmv_handle_t(C.Structure):
                 ("smCount", C.c_int), ("maxThreadsPerBlock", C.c_int), ("sharedMemPerSM", C.c_size_t)]
 
 
+class muxi_perf_result_t(C.Structure):
+    """PerformanceResult (performance_benchmark.h:21-53), as muxi_spmv_bench.h lays it out."""
+    _fields_ = [("timeMs", C.c_double), ("gflops", C.c_double), ("bandwidthGBs", C.c_double),
+                ("bandwidthUtilization", C.c_double), ("iterations", C.c_int), ("correctnessPassed", C.c_bool),
+                ("minTimeMs", C.c_double), ("maxTimeMs", C.c_double), ("avgTimeMs", C.c_double),
+                ("stdDevMs", C.c_double), ("numRows", C.c_int), ("numCols", C.c_int), ("nnz", C.c_int),
+                ("sparsity", C.c_double), ("deviceId", C.c_int), ("warpSize", C.c_int),
+                ("peakBandwidthGBs", C.c_double), ("useCSR5", C.c_bool), ("conversionTimeMs", C.c_double),
+                ("totalTimeMs", C.c_double), ("numTiles", C.c_int), ("tileSize", C.c_int),
+                ("conversionOverheadPercent", C.c_double)]
+
+
+class muxi_bench_config_t(C.Structure):
+    """BenchmarkConfig (performance_benchmark.h:58-71), as muxi_spmv_bench.h lays it out."""
+    _fields_ = [("warmupIterations", C.c_int), ("measureIterations", C.c_int), ("checkCorrectness", C.c_bool),
+                ("syncAfterEach", C.c_bool), ("stream", C.c_void_p), ("floatType", C.c_int),
+                ("compareCusparse", C.c_bool)]
+
+
 _VP, _I, _D, _F, _Z = C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_size_t
 _P_CSR, _P_CSR5, _P_ELL = C.POINTER(muxi_csr_t), C.POINTER(muxi_csr5_t), C.POINTER(muxi_ell_t)
 _P_OPTS = C.POINTER(spmv_opts_t)
@@ -211,6 +230,16 @@ SIGNATURES: dict[str, tuple] = {
     "muxi_selector_reset": (_I, []),
     "muxi_selector_state": (_I, []),
     "muxi_selector_pick": (_I, [_P_CSR, _I, _VP, C.POINTER(_I), C.POINTER(_I)]),
+    # muxi_spmv_bench.h
+    "muxi_bench_default_config": (muxi_bench_config_t, []),
+    "muxi_calculate_gflops": (_D, [_I, _D]),
+    "muxi_calculate_bandwidth": (_D, [_I, _I, _I, _D, _I]),
+    "muxi_measure_csr_performance": (_I, [_P_CSR, _VP, _VP, C.POINTER(muxi_bench_config_t), _P_H,
+                                          C.POINTER(muxi_perf_result_t)]),
+    "muxi_measure_csr5_performance": (_I, [_P_CSR, _P_CSR5, _VP, _VP, C.POINTER(muxi_bench_config_t), _P_H,
+                                           C.POINTER(muxi_perf_result_t), _I]),
+    "muxi_write_results_json": (_I, [C.POINTER(muxi_perf_result_t), _I, C.c_char_p]),
+    "muxi_print_performance_result": (None, [C.POINTER(muxi_perf_result_t)]),
 }
 
 

@@ -92,6 +92,10 @@ class Measurement:
     def exec_ms_mean(self) -> float:
         return statistics.fmean(self.exec_ms) if self.exec_ms else float("nan")
 
+    @property
+    def exec_ms_median(self) -> float:
+        return statistics.median(self.exec_ms) if self.exec_ms else float("nan")
+
 
 def copies_for_cold(bytes_per_copy: int, floor: int = 3) -> int:
     """Enough rotating copies that the others evict a copy from the MALL."""
@@ -227,6 +231,56 @@ def write_results_json(path: str, results: list[dict]) -> None:
     validate_results(doc)
     with open(path, "w") as f:
         json.dump(doc, f, indent=2)
+
+
+def _perf_dict(r) -> dict:
+    return {name: getattr(r, name) for name, _ in r._fields_}
+
+
+def _bench_config(warmup: int, iterations: int, check_correctness: bool, sync_after_each: bool, stream,
+                  float_type: int):
+    c = load().muxi_bench_default_config()
+    c.warmupIterations, c.measureIterations = int(warmup), int(iterations)
+    c.checkCorrectness, c.syncAfterEach = bool(check_correctness), bool(sync_after_each)
+    c.stream = _stream(stream)
+    c.floatType = float_type
+    return c
+
+
+def measure_csr_performance(A, x: torch.Tensor, y: torch.Tensor, *, warmup: int = 3, iterations: int = 10,
+                            check_correctness: bool = True, sync_after_each: bool = True, stream=None) -> dict:
+    """measureCSRPerformance<T> (performance_benchmark.cu:56-169) through muxi_measure_csr_performance: the
+    dispatching SpMV (spmv_csr) timed with HIP events, warmup then `iterations` measured calls; returns the
+    PerformanceResult fields (timeMs is the median) as a dict. A is an api.CSRMatrix of float or double values."""
+    from .api import SpmvError, _check_xy
+    if A.dtype not in (torch.float32, torch.float64):
+        raise SpmvError(-8, "measure_csr_performance", "float or double values, as the reference's template")
+    _check_xy(x, y, A.dtype, A.numCols, A.numRows, "measure_csr_performance")
+    ft = _lib.SPMV_TYPE_DOUBLE if A.dtype == torch.float64 else _lib.SPMV_TYPE_FLOAT
+    cfg = _bench_config(warmup, iterations, check_correctness, sync_after_each, stream, ft)
+    r = _lib.muxi_perf_result_t()
+    check(load().muxi_measure_csr_performance(C.byref(A.c), _ptr(x), _ptr(y), C.byref(cfg), None, C.byref(r)),
+          "measure_csr_performance")
+    return _perf_dict(r)
+
+
+def measure_csr5_performance(A, x: torch.Tensor, y: torch.Tensor, sigma: int = 0, *, warmup: int = 3,
+                             iterations: int = 10, check_correctness: bool = True, sync_after_each: bool = True,
+                             stream=None) -> tuple[dict, object]:
+    """measureCSR5Performance<T> (performance_benchmark.cu:348-479): the CSR -> CSR5 conversion timed with HIP
+    events, then the CSR5 SpMV as measure_csr_performance times spmv_csr; returns the PerformanceResult fields
+    and the api.CSR5Matrix the conversion built."""
+    from .api import CSR5Matrix, SpmvError, _check_xy
+    if A.dtype not in (torch.float32, torch.float64):
+        raise SpmvError(-8, "measure_csr5_performance", "float or double values, as the reference's template")
+    _check_xy(x, y, A.dtype, A.numCols, A.numRows, "measure_csr5_performance")
+    ft = _lib.SPMV_TYPE_DOUBLE if A.dtype == torch.float64 else _lib.SPMV_TYPE_FLOAT
+    cfg = _bench_config(warmup, iterations, check_correctness, sync_after_each, stream, ft)
+    m = CSR5Matrix(A)
+    r = _lib.muxi_perf_result_t()
+    check(load().muxi_measure_csr5_performance(C.byref(A.c), C.byref(m.c), _ptr(x), _ptr(y), C.byref(cfg), None,
+                                               C.byref(r), int(sigma)), "measure_csr5_performance")
+    return _perf_dict(r), m
 
 
 def device_record() -> dict:

@@ -144,7 +144,7 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
 
 
 CPP_TESTS = [ROOT / "tests" / "cpp" / "test_spmv_known_answer.cpp", ROOT / "tests" / "cpp" / "test_formats_io.cpp",
-             ROOT / "tests" / "cpp" / "test_csr5_fields.cpp"]
+             ROOT / "tests" / "cpp" / "test_csr5_fields.cpp", ROOT / "tests" / "cpp" / "test_benchmark_layer.cpp"]
 
 
 def build_cpp_tests(verbose: bool = True) -> list[Path]:
