@@ -15,7 +15,8 @@
 //   io::MTXHeader, io::readMTXHeader, io::readMTXFile, io::writeMTXFile,
 //   io::readMTXToCOO, io::writeCOOToMTX (src/generators/mtx_io.h)
 //   benchmark::PerformanceResult, benchmark::BenchmarkConfig, calculateGFlops, calculateBandwidth,
-//   measureCSRPerformance<T>, measureCSR5Performance<T>, compareWithCusparse<T> (no counterpart: unsupported),
+//   measureCSRPerformance<T>, measureCSR5Performance<T>, compareWithCusparse<T> (no counterpart: unsupported)
+//   and compareWithRocsparse<T> (the vendor library's SpMV timed the same way),
 //   writeResultsJSON, printPerformanceResult (src/benchmark/performance_benchmark.h)
 // Instantiated for float and double, as the reference is.
 #ifndef MUXI_SPMV_HPP_
@@ -620,6 +621,18 @@ template <typename FloatType>
 inline spmv_status_t compareWithCusparse(const CSRMatrix<FloatType>&, const FloatType*, FloatType*,
                                          const BenchmarkConfig&, double&, double&) {
     return SPMV_ERROR_UNSUPPORTED_FORMAT;
+}
+
+// The vendor library's CSR SpMV timed the same way (muxi_compare_with_rocsparse; alg 0 adaptive, 1 rowsplit,
+// 2 LRB, 3 nnzsplit), in the place of compareWithCusparse's outputs.
+template <typename FloatType>
+inline spmv_status_t compareWithRocsparse(const CSRMatrix<FloatType>& matrix, const FloatType* d_x, FloatType* d_y,
+                                          const BenchmarkConfig& config, double& rocsparseTime,
+                                          double& rocsparseGflops, int alg = 0) {
+    static_assert(muxi_spmv::detail::is_fp<FloatType>::value, "float or double");
+    const ::muxi_csr_t c = matrix.abi();
+    const BenchmarkConfig t = detail::typed<FloatType>(config);
+    return ::muxi_compare_with_rocsparse(&c, d_x, d_y, t.abi(), alg, &rocsparseTime, &rocsparseGflops, nullptr);
 }
 
 inline void printPerformanceResult(const PerformanceResult& result) { ::muxi_print_performance_result(&result); }

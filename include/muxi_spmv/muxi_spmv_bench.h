@@ -79,6 +79,17 @@ spmv_status_t muxi_measure_csr5_performance(const muxi_csr_t* csr, muxi_csr5_t* 
                                             const muxi_bench_config_t* config, const muxi_spmv_handle_t* handle,
                                             muxi_perf_result_t* result, int sigma);
 
+/* compareWithCusparse (performance_benchmark.h:124-151, performance_benchmark.cu:173-270, compiled out in the
+ * reference) on this device: the vendor library's CSR SpMV (rocSPARSE rocsparse_spmv by stages, loaded at run time
+ * from librocsparse.so.1; alg 0 adaptive, 1 rowsplit, 2 LRB, 3 nnzsplit) on the same matrix and vectors, y cleared
+ * first, its preprocess stage timed once (*analysisMs, may be NULL) and its compute stage timed as
+ * muxi_measure_csr_performance times the dispatching SpMV (warmup, then the median of config->measureIterations,
+ * HIP events on config->stream): *timeMs and *gflops (2 * nnz / timeMs / 1e6). A speed reference and a cross-check
+ * of y; the library's own SpMV never calls it. SPMV_ERROR_UNSUPPORTED_FORMAT when rocSPARSE cannot be loaded. */
+spmv_status_t muxi_compare_with_rocsparse(const muxi_csr_t* A, const void* d_x, void* d_y,
+                                          const muxi_bench_config_t* config, int alg, double* timeMs,
+                                          double* gflops, double* analysisMs);
+
 /* writeResultsJSON (performance_benchmark.cu:298-335): {"results": [{matrix, device, performance,
  * correctnessPassed}, ...]} with the reference's keys and number formats. */
 spmv_status_t muxi_write_results_json(const muxi_perf_result_t* results, int count, const char* filename);

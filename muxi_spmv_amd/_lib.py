@@ -239,6 +239,8 @@ SIGNATURES: dict[str, tuple] = {
     "muxi_measure_csr5_performance": (_I, [_P_CSR, _P_CSR5, _VP, _VP, C.POINTER(muxi_bench_config_t), _P_H,
                                            C.POINTER(muxi_perf_result_t), _I]),
     "muxi_write_results_json": (_I, [C.POINTER(muxi_perf_result_t), _I, C.c_char_p]),
+    "muxi_compare_with_rocsparse": (_I, [_P_CSR, _VP, _VP, C.POINTER(muxi_bench_config_t), _I,
+                                         C.POINTER(_D), C.POINTER(_D), C.POINTER(_D)]),
     "muxi_print_performance_result": (None, [C.POINTER(muxi_perf_result_t)]),
 }
 

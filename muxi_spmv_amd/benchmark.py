@@ -283,6 +283,27 @@ def measure_csr5_performance(A, x: torch.Tensor, y: torch.Tensor, sigma: int = 0
     return _perf_dict(r), m
 
 
+ROCSPARSE_ALGS = {"adaptive": 0, "rowsplit": 1, "lrb": 2, "nnzsplit": 3}
+
+
+def compare_with_rocsparse(A, x: torch.Tensor, y: torch.Tensor, alg: str = "adaptive", *, warmup: int = 3,
+                           iterations: int = 10, sync_after_each: bool = True, stream=None) -> dict:
+    """The vendor library's CSR SpMV (rocSPARSE rocsparse_spmv by stages, algorithm `alg`) on the same matrix
+    and vectors, timed as measure_csr_performance times ours (muxi_compare_with_rocsparse, the counterpart of the
+    reference's compiled-out compareWithCusparse, performance_benchmark.cu:173-270): {"timeMs", "gflops",
+    "analysisMs"}; y is cleared first and holds its result. SpmvError -7 when rocSPARSE cannot be loaded."""
+    from .api import SpmvError, _check_xy
+    if A.dtype not in (torch.float32, torch.float64):
+        raise SpmvError(-8, "compare_with_rocsparse", "float or double values")
+    _check_xy(x, y, A.dtype, A.numCols, A.numRows, "compare_with_rocsparse")
+    ft = _lib.SPMV_TYPE_DOUBLE if A.dtype == torch.float64 else _lib.SPMV_TYPE_FLOAT
+    cfg = _bench_config(warmup, iterations, False, sync_after_each, stream, ft)
+    t, g, a = C.c_double(), C.c_double(), C.c_double()
+    check(load().muxi_compare_with_rocsparse(C.byref(A.c), _ptr(x), _ptr(y), C.byref(cfg), ROCSPARSE_ALGS[alg],
+                                             C.byref(t), C.byref(g), C.byref(a)), "compare_with_rocsparse")
+    return {"timeMs": t.value, "gflops": g.value, "analysisMs": a.value}
+
+
 def device_record() -> dict:
     p = torch.cuda.get_device_properties(0)
     return {"id": 0, "name": p.name, "warpSize": 64, "computeUnits": p.multi_processor_count,

@@ -3,11 +3,20 @@
 // results file (:298-335) and printout (:272-296), timed with HIP events on the configured stream.
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <mutex>
 #include <string>
 #include <vector>
+
+// rocSPARSE's types and prototypes only: the library is opened at run time by the comparison that uses it (a
+// process that has loaded torch already holds a copy of it, under the same name, which dlopen then returns), so
+// libmuxi_spmv_amd.so itself depends on nothing beyond the HIP runtime.
+#include <rocsparse/rocsparse.h>
 
 #include "kernels.hpp"
 #include "muxi_spmv/muxi_spmv_bench.h"
@@ -127,6 +136,92 @@ spmv_status_t check_finite(const muxi_bench_config_t* cfg, const void* d_y, int 
     return SPMV_SUCCESS;
 }
 
+// The rocSPARSE entry points the comparison calls, looked up once. The SpMV is the stage-based rocsparse_spmv
+// (buffer size, preprocess, compute), which every rocSPARSE of ROCm 6 and 7 exports (ROCm 7 marks it deprecated
+// in favour of rocsparse_v2_spmv).
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wdeprecated-declarations"
+using RsSpmv = decltype(&rocsparse_spmv);
+#pragma clang diagnostic pop
+struct RsApi {
+    decltype(&rocsparse_create_handle) create_handle = nullptr;
+    decltype(&rocsparse_destroy_handle) destroy_handle = nullptr;
+    decltype(&rocsparse_set_stream) set_stream = nullptr;
+    decltype(&rocsparse_create_csr_descr) create_csr_descr = nullptr;
+    decltype(&rocsparse_destroy_spmat_descr) destroy_spmat_descr = nullptr;
+    decltype(&rocsparse_create_dnvec_descr) create_dnvec_descr = nullptr;
+    decltype(&rocsparse_destroy_dnvec_descr) destroy_dnvec_descr = nullptr;
+    RsSpmv spmv = nullptr;
+    bool ok = false;
+    std::string why;
+    std::string path;  // the file the symbols came from
+};
+
+const RsApi& rs_api() {
+    static RsApi api;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        // the copy already in the process (or on the loader's path), else the one in the ROCm installation
+        void* h = dlopen("librocsparse.so.1", RTLD_NOW | RTLD_LOCAL);
+        if (!h) {
+            const char* root = std::getenv("ROCM_PATH");
+            const std::string path = std::string(root && *root ? root : "/opt/rocm") + "/lib/librocsparse.so.1";
+            h = dlopen(path.c_str(), RTLD_NOW | RTLD_LOCAL);
+            if (!h) {
+                const char* e = dlerror();
+                api.why = "cannot open librocsparse.so.1 (" + path + "): " + (e ? e : "");
+                return;
+            }
+        }
+#define MUXI_RS_SYM(field, name)                                  \
+    api.field = reinterpret_cast<decltype(api.field)>(dlsym(h, name)); \
+    if (!api.field) {                                               \
+        api.why = std::string("librocsparse.so.1 lacks ") + name;   \
+        return;                                                     \
+    }
+        MUXI_RS_SYM(create_handle, "rocsparse_create_handle")
+        MUXI_RS_SYM(destroy_handle, "rocsparse_destroy_handle")
+        MUXI_RS_SYM(set_stream, "rocsparse_set_stream")
+        MUXI_RS_SYM(create_csr_descr, "rocsparse_create_csr_descr")
+        MUXI_RS_SYM(destroy_spmat_descr, "rocsparse_destroy_spmat_descr")
+        MUXI_RS_SYM(create_dnvec_descr, "rocsparse_create_dnvec_descr")
+        MUXI_RS_SYM(destroy_dnvec_descr, "rocsparse_destroy_dnvec_descr")
+        MUXI_RS_SYM(spmv, "rocsparse_spmv")
+#undef MUXI_RS_SYM
+        Dl_info info{};
+        api.path = dladdr(reinterpret_cast<void*>(api.spmv), &info) && info.dli_fname ? info.dli_fname : "?";
+        api.ok = true;
+    });
+    return api;
+}
+
+// rocSPARSE objects of one comparison, released on every return path.
+struct Rocsparse {
+    const RsApi& f;
+    explicit Rocsparse(const RsApi& api) : f(api) {}
+    rocsparse_handle h = nullptr;
+    rocsparse_spmat_descr mat = nullptr;
+    rocsparse_dnvec_descr x = nullptr, y = nullptr;
+    void* buf = nullptr;
+    ~Rocsparse() {
+        (void)hipFree(buf);
+        if (x) f.destroy_dnvec_descr(x);
+        if (y) f.destroy_dnvec_descr(y);
+        if (mat) f.destroy_spmat_descr(mat);
+        if (h) f.destroy_handle(h);
+    }
+};
+
+#define MUXI_RS(call)                                                                          \
+    do {                                                                                       \
+        const rocsparse_status rs_ = (call);                                                   \
+        if (rs_ != rocsparse_status_success) {                                                 \
+            set_error(std::string(#call) + ": rocsparse status " + std::to_string((int)rs_)); \
+            return SPMV_ERROR_INTERNAL;                                                        \
+        }                                                                                      \
+    } while (0)
+
+
 spmv_opts_t opts_for(const muxi_bench_config_t* cfg) {
     spmv_opts_t o = muxi_spmv_default_opts();
     o.sync = cfg->syncAfterEach ? 1 : 0;
@@ -234,6 +329,89 @@ extern "C" spmv_status_t muxi_measure_csr5_performance(const muxi_csr_t* csr, mu
         result->totalTimeMs = result->timeMs + result->conversionTimeMs;
         result->conversionOverheadPercent = result->conversionTimeMs / result->timeMs * 100.0;
         return check_finite(config, d_y, csr->numRows, result);
+    });
+}
+
+extern "C" spmv_status_t muxi_compare_with_rocsparse(const muxi_csr_t* A, const void* d_x, void* d_y,
+                                                     const muxi_bench_config_t* config, int alg, double* timeMs,
+                                                     double* gflops, double* analysisMs) {
+    return no_throw("muxi_compare_with_rocsparse", [&]() -> spmv_status_t {
+        if (!A || A->numRows < 0 || A->numCols < 0 || A->nnz < 0 || (A->numRows > 0 && !A->d_rowPtr) ||
+            (A->nnz > 0 && (!A->d_colIdx || !A->d_values))) {
+            set_error("compare_with_rocsparse: invalid matrix");
+            return SPMV_ERROR_INVALID_MATRIX;
+        }
+        if (!d_x || !d_y || !timeMs || !gflops) {
+            set_error("compare_with_rocsparse: x, y, timeMs or gflops is NULL");
+            return SPMV_ERROR_INVALID_VECTOR;
+        }
+        spmv_status_t st = check_config(config);
+        if (st != SPMV_SUCCESS) return st;
+        static const rocsparse_spmv_alg algs[] = {rocsparse_spmv_alg_csr_adaptive, rocsparse_spmv_alg_csr_rowsplit,
+                                                  rocsparse_spmv_alg_csr_lrb, rocsparse_spmv_alg_csr_nnzsplit};
+        if (alg < 0 || alg > 3) {
+            set_error("compare_with_rocsparse: alg must be 0 (adaptive), 1 (rowsplit), 2 (LRB) or 3 (nnzsplit)");
+            return SPMV_ERROR_INTERNAL;
+        }
+        const RsApi& f = rs_api();
+        if (!f.ok) {
+            set_error("compare_with_rocsparse: " + f.why);
+            return SPMV_ERROR_UNSUPPORTED_FORMAT;
+        }
+        const bool f64 = config->floatType == SPMV_TYPE_DOUBLE;
+        const rocsparse_datatype dt = f64 ? rocsparse_datatype_f64_r : rocsparse_datatype_f32_r;
+        hipStream_t s = (hipStream_t)config->stream;
+        Rocsparse rs(f);
+        MUXI_RS(f.create_handle(&rs.h));
+        MUXI_RS(f.set_stream(rs.h, s));
+        MUXI_RS(f.create_csr_descr(&rs.mat, A->numRows, A->numCols, A->nnz, A->d_rowPtr, A->d_colIdx, A->d_values,
+                                   rocsparse_indextype_i32, rocsparse_indextype_i32, rocsparse_index_base_zero, dt));
+        MUXI_RS(f.create_dnvec_descr(&rs.x, A->numCols, const_cast<void*>(d_x), dt));
+        MUXI_RS(f.create_dnvec_descr(&rs.y, A->numRows, d_y, dt));
+        const rocsparse_spmv_alg a = algs[alg];
+        const float a32 = 1.f, b32 = 0.f;
+        const double a64 = 1.0, b64 = 0.0;
+        const void* alpha = f64 ? (const void*)&a64 : (const void*)&a32;
+        const void* beta = f64 ? (const void*)&b64 : (const void*)&b32;
+        auto stage = [&](rocsparse_spmv_stage sg, size_t* bytes, void* buf) {
+            return f.spmv(rs.h, rocsparse_operation_none, alpha, rs.mat, rs.x, beta, rs.y, dt, a, sg, bytes, buf);
+        };
+        size_t bytes = 0;
+        const rocsparse_status bs = stage(rocsparse_spmv_stage_buffer_size, &bytes, nullptr);
+        if (bs == rocsparse_status_invalid_value && alg != 0) {
+            // an algorithm newer than the rocSPARSE in the process (a framework may bring an older copy)
+            set_error("compare_with_rocsparse: " + f.path + " rejects algorithm " + std::to_string(alg) +
+                      " (rocsparse status 7, invalid value)");
+            return SPMV_ERROR_UNSUPPORTED_FORMAT;
+        }
+        MUXI_RS(bs);
+        MUXI_HIP(hipMalloc(&rs.buf, std::max<size_t>(bytes, 16)));
+        // The library's atomic-accumulating algorithms (adaptive on long rows, LRB, nnzsplit) scale y by beta and
+        // then add into it, so with beta 0 a NaN already in y survives: y starts at zero, as the first call's
+        // result would leave it for every later one.
+        if (A->numRows > 0) MUXI_HIP(hipMemsetAsync(d_y, 0, (size_t)A->numRows * (f64 ? 8 : 4), s));
+        EventPair ev;
+        MUXI_HIP(hipEventCreate(&ev.a));
+        MUXI_HIP(hipEventCreate(&ev.b));
+        MUXI_HIP(hipEventRecord(ev.a, s));
+        MUXI_RS(stage(rocsparse_spmv_stage_preprocess, &bytes, rs.buf));
+        MUXI_HIP(hipEventRecord(ev.b, s));
+        MUXI_HIP(hipEventSynchronize(ev.b));
+        float ams = 0.f;
+        MUXI_HIP(hipEventElapsedTime(&ams, ev.a, ev.b));
+        if (analysisMs) *analysisMs = (double)ams;
+        muxi_perf_result_t r{};
+        r.iterations = config->measureIterations;
+        auto run = [&]() -> spmv_status_t {
+            MUXI_RS(stage(rocsparse_spmv_stage_compute, &bytes, rs.buf));
+            if (config->syncAfterEach) MUXI_HIP(hipStreamSynchronize(s));
+            return SPMV_SUCCESS;
+        };
+        st = time_iterations(config, s, A, &r, run);
+        if (st != SPMV_SUCCESS) return st;
+        *timeMs = r.timeMs;
+        *gflops = r.gflops;
+        return SPMV_SUCCESS;
     });
 }
 

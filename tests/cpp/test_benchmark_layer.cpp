@@ -3,7 +3,8 @@
 // <double> through a handle, measureCSR5Performance<float> on a fresh CSR5Matrix, printPerformanceResult and
 // writeResultsJSON, with hipStream_t in place of cudaStream_t. It checks the fields a reference caller reads:
 // the median between min and max, GFLOP/s and bandwidth at the median by the reference's formulas, the CSR5
-// conversion time and tile count, and that y equals a host CSR loop (integer-valued data, exact).
+// conversion time and tile count, and that y equals a host CSR loop (integer-valued data, exact). compareWithRocsparse<T>
+// runs rocSPARSE's four CSR algorithms on the same data (y NaN beforehand), each y exact as well.
 // argv[1]: the JSON file to write. Exit code 0 on success.
 #include <cmath>
 #include <cstdio>
@@ -83,6 +84,14 @@ static PerformanceResult run_csr(int n, spmv_handle_t* h) {
     CHECK(y_exact(A, hx, d_y));
     double cs = 0.0, cg = 0.0;
     CHECK(compareWithCusparse<T>(A, d_x, d_y, config, cs, cg) == SPMV_ERROR_UNSUPPORTED_FORMAT);
+    for (int alg = 0; alg < 4; ++alg) {  // the vendor library: adaptive, rowsplit, LRB, nnzsplit
+        (void)hipMemset(d_y, 0xff, sizeof(T) * n);  // NaN: the comparison clears y itself
+        const spmv_status_t rs = compareWithRocsparse<T>(A, d_x, d_y, config, cs, cg, alg);
+        if (rs != SPMV_SUCCESS) std::printf("compareWithRocsparse(%d): %d %s\n", alg, (int)rs, ::muxi_spmv_last_error());
+        CHECK(rs == SPMV_SUCCESS);
+        CHECK(cs > 0.0 && std::fabs(cg - 2.0 * A.nnz / cs / 1e6) <= 1e-9 * cg);
+        CHECK(y_exact(A, hx, d_y));
+    }
     (void)hipFree(d_x);
     (void)hipFree(d_y);
     return r;

@@ -82,3 +82,40 @@ def test_cpp_benchmark_caller(tmp_path):
     assert "PASSED" in r.stdout and "=== Performance Results ===" in r.stdout
     doc = json.loads(out.read_text())
     assert len(doc["results"]) == 3 and all(e["correctnessPassed"] for e in doc["results"])
+
+
+@pytest.mark.parametrize("alg", ["adaptive", "rowsplit", "lrb", "nnzsplit"])
+@pytest.mark.parametrize("mname", ["powerlaw", "uniform", "empty_rows", "long_row"])
+def test_rocsparse_cross_check(mname, alg):
+    """The vendor library's y (rocSPARSE, every CSR algorithm) within the FP32 bound of the FP64 oracle, as ours
+    is: the comparison times a correct product. y starts as NaN: the library's atomic algorithms add into y, so
+    the comparison must clear it first."""
+    h = ZOO[mname]
+    A = dev_csr(h)
+    x = torch.from_numpy(np.random.default_rng(5).uniform(-1, 1, h.numCols).astype(np.float32)).cuda()
+    y = torch.full((h.numRows,), float("nan"), device="cuda")
+    try:
+        r = benchmark.compare_with_rocsparse(A, x, y, alg, iterations=3)
+    except api.SpmvError as e:
+        if e.status == -7 and "rejects algorithm" in str(e):  # the rocSPARSE loaded with torch predates it
+            pytest.skip(str(e))
+        raise
+    assert r["timeMs"] > 0 and r["gflops"] == pytest.approx(2.0 * h.nnz / r["timeMs"] / 1e6, rel=1e-12)
+    assert r["analysisMs"] >= 0
+    y64, ab = oracle.spmv_f64(h.rowPtr, h.colIdx, h.values.astype(np.float64), x.double().cpu().numpy())
+    yh = y.double().cpu().numpy()
+    err = np.abs(yh - y64)
+    bound = fp32_bound(h.rowPtr, 1.0) * ab + 1e-37
+    bad = ~(err <= bound)
+    assert not bad.any(), (int(bad.sum()), int(np.isnan(yh).sum()), float(np.nanmax(err / bound)),
+                           np.flatnonzero(bad)[:5].tolist())
+
+
+def test_rocsparse_fp64():
+    h = ZOO["powerlaw"]
+    A = dev_csr(h, torch.float64)
+    x = torch.from_numpy(np.random.default_rng(6).uniform(-1, 1, h.numCols)).cuda()
+    y = torch.full((h.numRows,), float("nan"), dtype=torch.float64, device="cuda")
+    benchmark.compare_with_rocsparse(A, x, y, "adaptive", iterations=3)
+    y64, ab = oracle.spmv_f64(h.rowPtr, h.colIdx, h.values.astype(np.float64), x.cpu().numpy())
+    assert np.all(np.abs(y.cpu().numpy() - y64) <= fp64_bound(h.rowPtr, 1.0) * ab + 1e-300)
