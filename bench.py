@@ -19,6 +19,8 @@ fourth timed step (a sample spread over the timed region; markers on every
 step would add their own cost to a short step). The window holds every
 kernel of the launch: for the two-kernel LDS slab path the slab kernel and
 the combine after it, whose shares are reported beside it;
+vendor = rocSPARSE's CSR SpMV algorithms on the same matrix, warm, beside the
+timed plan timed the same way (--no-vendor skips it);
 cpu_baseline = the oracle's FP32 CSR loop on the host cores.
 """
 from __future__ import annotations
@@ -112,6 +114,79 @@ def cpu_baseline(h, x, budget_s: float = 12.0) -> dict:
             "single_thread": {"value": round(b / ms1 / 1e6, 3), "ms": round(ms1, 3), "reps": reps1}}
 
 
+def vendor_leg(A0, x0, y0, plan, y64, steps: int = 200) -> dict:
+    """rocSPARSE's CSR SpMV (adaptive, rowsplit, LRB: muxi_compare_with_rocsparse) on the same matrix and x,
+    beside the timed plan, both warm on one copy and timed alike: every launch between two HIP events, back to
+    back, the median taken. Each of rocSPARSE's y is checked against the FP64 oracle's."""
+    import torch
+
+    from muxi_spmv_amd.api import SpmvError
+    from muxi_spmv_amd.benchmark import compare_with_rocsparse, measure_plans
+
+    m = measure_plans([plan], [x0], [y0], steps, 10, event_every=1)
+    out = {"timing": f"warm (one copy, {steps} launches back to back), median of HIP events around each launch",
+           "ours_us": round(m.exec_ms_median * 1e3, 3), "rocsparse_us": {}, "rocsparse_analysis_ms": {},
+           "rocsparse_rel_l2_vs_fp64": {}}
+    for alg in ("adaptive", "rowsplit", "lrb"):
+        try:
+            r = compare_with_rocsparse(A0, x0, y0, alg, warmup=10, iterations=steps, sync_after_each=False)
+        except SpmvError as e:
+            out["rocsparse_us"][alg] = None
+            log(f"rocSPARSE {alg}: {e}")
+            continue
+        torch.cuda.synchronize()
+        out["rocsparse_us"][alg] = round(r["timeMs"] * 1e3, 3)
+        out["rocsparse_analysis_ms"][alg] = round(r["analysisMs"], 3)
+        out["rocsparse_rel_l2_vs_fp64"][alg] = float(torch.linalg.vector_norm(y0.double() - y64)
+                                                     / torch.linalg.vector_norm(y64))
+        log(f"rocSPARSE {alg}: {r['timeMs'] * 1e3:.1f} us (ours {out['ours_us']:.1f} us warm), "
+            f"analysis {r['analysisMs']:.1f} ms")
+    with open("/proc/self/maps") as f:
+        libs = sorted({ln.split()[-1] for ln in f if "librocsparse" in ln})
+    out["library"] = libs[0] if libs else None
+    out["system_library"] = system_rocsparse(A0, x0, steps)
+    times = list(out["rocsparse_us"].values()) + list((out["system_library"] or {}).get("us", {}).values())
+    best = min((v for v in times if v), default=None)
+    out["speedup_vs_best_rocsparse"] = round(best / out["ours_us"], 3) if best else None
+    return out
+
+
+def system_rocsparse(A0, x0, steps: int) -> dict | None:
+    """The ROCm installation's rocSPARSE (all four CSR algorithms) timed the same way by
+    muxi_spmv_amd/rocsparse_timing, a C++ caller run as a child process: a PyTorch process holds torch's own,
+    older librocsparse under the same soname. The matrix and x go over as raw arrays in a temporary folder."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = ROOT / "muxi_spmv_amd" / "rocsparse_timing"
+    if A0.dtype != __import__("torch").float32 or not exe.exists():
+        return None
+    d = tempfile.mkdtemp(prefix="muxi_rs_")
+    try:
+        A0.d_rowPtr.cpu().numpy().tofile(f"{d}/rowPtr.i32")
+        A0.d_colIdx.cpu().numpy().tofile(f"{d}/colIdx.i32")
+        A0.d_values.cpu().numpy().tofile(f"{d}/values.f32")
+        x0.cpu().numpy().tofile(f"{d}/x.f32")
+        r = subprocess.run([str(exe), d, str(A0.numRows), str(A0.numCols), str(A0.nnz), str(steps)],
+                           capture_output=True, text=True, timeout=300)
+    except subprocess.TimeoutExpired:
+        log("system rocSPARSE: timed out")
+        return None
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    if r.stderr.strip():
+        log(f"system rocSPARSE: {r.stderr.strip()}")
+    try:
+        res = json.loads(r.stdout.strip().splitlines()[-1])
+    except (IndexError, ValueError):
+        log(f"system rocSPARSE: exit {r.returncode}, no result")
+        return None
+    res["exit"] = r.returncode
+    log(f"system rocSPARSE ({res['library']}): {res['us']} us, analysis {res['analysis_ms']} ms")
+    return res
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +196,7 @@ def main(argv=None) -> int:
     ap.add_argument("--kernel", default="auto",
                     help="auto (the library's pick, sweep timed beside it), sweep (the sweep's best), or kernel:param")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE comparison")
     ap.add_argument("--sweep-steps", type=int, default=30)
     a = ap.parse_args(argv)
 
@@ -265,6 +341,7 @@ def main(argv=None) -> int:
         "sweep_ms": sweep,
         "selector": selector,
     }
+    out["vendor"] = None if a.no_vendor else vendor_leg(A0, xs[0], ys[0], plans[0], y64)
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(h, x)

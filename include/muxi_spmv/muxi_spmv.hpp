@@ -624,15 +624,16 @@ inline spmv_status_t compareWithCusparse(const CSRMatrix<FloatType>&, const Floa
 }
 
 // The vendor library's CSR SpMV timed the same way (muxi_compare_with_rocsparse; alg 0 adaptive, 1 rowsplit,
-// 2 LRB, 3 nnzsplit), in the place of compareWithCusparse's outputs.
+// 2 LRB, 3 nnzsplit), in the place of compareWithCusparse's outputs; *analysisMs, when given, receives the
+// library's preprocess time.
 template <typename FloatType>
 inline spmv_status_t compareWithRocsparse(const CSRMatrix<FloatType>& matrix, const FloatType* d_x, FloatType* d_y,
                                           const BenchmarkConfig& config, double& rocsparseTime,
-                                          double& rocsparseGflops, int alg = 0) {
+                                          double& rocsparseGflops, int alg = 0, double* analysisMs = nullptr) {
     static_assert(muxi_spmv::detail::is_fp<FloatType>::value, "float or double");
     const ::muxi_csr_t c = matrix.abi();
     const BenchmarkConfig t = detail::typed<FloatType>(config);
-    return ::muxi_compare_with_rocsparse(&c, d_x, d_y, t.abi(), alg, &rocsparseTime, &rocsparseGflops, nullptr);
+    return ::muxi_compare_with_rocsparse(&c, d_x, d_y, t.abi(), alg, &rocsparseTime, &rocsparseGflops, analysisMs);
 }
 
 inline void printPerformanceResult(const PerformanceResult& result) { ::muxi_print_performance_result(&result); }

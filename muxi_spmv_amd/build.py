@@ -147,19 +147,24 @@ CPP_TESTS = [ROOT / "tests" / "cpp" / "test_spmv_known_answer.cpp", ROOT / "test
              ROOT / "tests" / "cpp" / "test_csr5_fields.cpp", ROOT / "tests" / "cpp" / "test_benchmark_layer.cpp"]
 
 
+# C++ programs beside the library: rocSPARSE timed outside a PyTorch process (bench.py runs it)
+CPP_TOOLS = [PKG / "rocsparse_timing.cpp"]
+
+
 def build_cpp_tests(verbose: bool = True) -> list[Path]:
     """Reference-style C++ callers compiled against include/muxi_spmv/muxi_spmv.hpp."""
     out = []
-    for src in CPP_TESTS:
+    for src in CPP_TESTS + CPP_TOOLS:
         exe = src.with_suffix("")
+        rpath = "$ORIGIN" if src.parent == PKG else "$ORIGIN/../../muxi_spmv_amd"
         cmd = [HIPCC, "-std=c++17", "-O2", f"--offload-arch={ARCH}", f"-I{ROOT / 'include'}", str(src),
-               f"-L{PKG}", "-lmuxi_spmv_amd", "-Wl,-rpath,$ORIGIN/../../muxi_spmv_amd", "-o", str(exe)]
+               f"-L{PKG}", "-lmuxi_spmv_amd", f"-Wl,-rpath,{rpath}", "-o", str(exe)]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"compile failed: {src.name}\n{r.stdout}\n{r.stderr}")
         out.append(exe)
     if verbose:
-        print(f"built {len(out)} C++ caller test(s)", flush=True)
+        print(f"built {len(out)} C++ caller(s)", flush=True)
     return out
 
 

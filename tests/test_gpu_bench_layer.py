@@ -119,3 +119,22 @@ def test_rocsparse_fp64():
     benchmark.compare_with_rocsparse(A, x, y, "adaptive", iterations=3)
     y64, ab = oracle.spmv_f64(h.rowPtr, h.colIdx, h.values.astype(np.float64), x.cpu().numpy())
     assert np.all(np.abs(y.cpu().numpy() - y64) <= fp64_bound(h.rowPtr, 1.0) * ab + 1e-300)
+
+
+def test_rocsparse_timing_program(tmp_path):
+    """muxi_spmv_amd/rocsparse_timing (the installation's rocSPARSE, outside a PyTorch process): every CSR
+    algorithm runs, its y within 1e-5 relative L2 of a host FP64 loop, one JSON line out."""
+    exe = Path(__file__).resolve().parents[1] / "muxi_spmv_amd" / "rocsparse_timing"
+    if not exe.exists():
+        pytest.fail("muxi_spmv_amd/rocsparse_timing is not built (python -m muxi_spmv_amd.build)")
+    h = ZOO["powerlaw"]
+    h.rowPtr.astype(np.int32).tofile(tmp_path / "rowPtr.i32")
+    h.colIdx.astype(np.int32).tofile(tmp_path / "colIdx.i32")
+    h.values.astype(np.float32).tofile(tmp_path / "values.f32")
+    np.random.default_rng(7).uniform(-1, 1, h.numCols).astype(np.float32).tofile(tmp_path / "x.f32")
+    r = subprocess.run([str(exe), str(tmp_path), str(h.numRows), str(h.numCols), str(h.nnz), "5"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    doc = json.loads(r.stdout.strip().splitlines()[-1])
+    assert set(doc["us"]) == {"adaptive", "rowsplit", "lrb", "nnzsplit"} and all(v > 0 for v in doc["us"].values())
+    assert all(v <= 1e-5 for v in doc["rel_l2_vs_fp64"].values()) and "librocsparse" in doc["library"]
