@@ -1,6 +1,6 @@
 """Secondary benchmark workloads (one JSON line each on stdout).
 
-python -m muxi_spmv_amd.bench_suite precision|batched|spmm|laplacian|shard|host [--out FILE] [--results FILE]
+python -m muxi_spmv_amd.bench_suite precision|batched|spmm|laplacian|shard|host|kernels [--out FILE] [--results FILE]
 
 precision  values stored FP64 / FP32 / BF16, x FP32, FP32 accumulation plain
            and compensated (Dot2), on the 2M power-law matrix and on a
@@ -19,6 +19,9 @@ shard      one GPU's share of an 8-GPU row split (muxi_shard_rows: equal merge-p
            diagonals moved to row starts, x replicated) of the power-law and
            Laplacian matrices: the heaviest shard run with the AUTO plan, next
            to the whole matrix on one GPU.
+kernels    every configuration of the autotune sweep (and the scalar and vector kernels) on one matrix
+           (--matrix powerlaw: 2M power-law FP32; laplacian: 4096^2 Laplacian FP64), --reps cold launches
+           each: the process to run under rocprofv3 for every kernel's counters side by side.
 host       x and y in host memory (muxi_spmv_host.h): pinned and pageable buffers,
            1 / 2 / 4 / 8 row chunks overlapping PCIe copies with the kernels, on
            the power-law matrix and the FP64 Laplacian; end-to-end time from the
@@ -340,6 +343,46 @@ def laplacian(args) -> list[dict]:
     return out
 
 
+def kernels(args) -> list[dict]:
+    """Every configuration of the autotune sweep, each its own launches, on one matrix: the 2M power-law matrix in
+    FP32 (--matrix powerlaw) or the 4096^2 Laplacian in FP64 (--matrix laplacian). Each configuration runs cold
+    (cache flush before each of --reps launches), so that one rocprofv3 counter pass over this process gives every
+    kernel's counters side by side, told apart by kernel name. The reference's one-lane and one-wavefront-per-row
+    kernels (scalar, vector) run too, where the sweep leaves them out."""
+    fp64 = args.matrix == "laplacian"
+    h = G.cfg_laplacian_4096() if fp64 else G.cfg_powerlaw_2m()
+    dt, esz = (torch.float64, 8) if fp64 else (torch.float32, 4)
+    wname = "laplacian_4096_fp64" if fp64 else "powerlaw_2m_fp32"
+    A = api.CSRMatrix(h.rowPtr, h.colIdx, torch.from_numpy(h.values).to(dt), h.numCols)
+    x = torch.from_numpy(G.uniform_vector(h.numCols, 3, np.float64 if fp64 else np.float32)).cuda()
+    y = torch.empty(h.numRows, dtype=dt, device="cuda")
+    f = tune.features(A)
+    b = algorithmic_bytes(h.numRows, h.numCols, h.nnz, esz, esz)
+    tm = Timer()
+    y64 = api.oracle_f64(A, x)
+    dev = device_record()
+    cpu = cpu_ms(h.rowPtr, h.colIdx, h.values, x.cpu().numpy(), reps=1)
+    tol = TOL_FP64 if fp64 else TOL_FP32
+    out = []
+    for k, p in tune.CANDIDATES:
+        if not (k in ("scalar", "vector") or tune.feasible(k, f, fp64)):
+            continue
+        pl = api.Plan(A, k, p, xtype=dt)
+        ms = tm.time(lambda: pl.exec(x, y), reps=args.reps)
+        info = pl.info()
+        e = errors(y, y64)
+        rec = {"workload": wname, "kernel": f"{k}:{p}", "runs": f"{info.kernel}:{info.param0}", "ms": round(ms, 4),
+               "GBps": round(b / ms / 1e6, 1), "frac_hbm": round(b / ms / 1e6 / HBM_PEAK_GBS, 4),
+               "rel_l2": e["rel_l2"]}
+        log(json.dumps(rec))
+        out.append(rec)
+        args.results.append(performance_result(h.numRows, h.numCols, h.nnz, tm.last, b, rel_l2=e["rel_l2"],
+                                               max_abs=e["max_abs"], tolerance=tol, device=dev, cpu=cpu,
+                                               workload=wname, kernel=rec["runs"]))
+        pl.destroy()
+    return out
+
+
 def shard(args) -> list[dict]:
     G_ = 8
     tm = Timer()
@@ -438,7 +481,9 @@ def host_path(args) -> list[dict]:
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["precision", "batched", "spmm", "laplacian", "shard", "host"])
+    ap.add_argument("what", choices=["precision", "batched", "spmm", "laplacian", "shard", "host", "kernels"])
+    ap.add_argument("--matrix", choices=["powerlaw", "laplacian"], default="powerlaw", help="kernels: the matrix")
+    ap.add_argument("--reps", type=int, default=3, help="kernels: cold launches per configuration")
     ap.add_argument("--out", default=None, help="the JSON lines printed on stdout, also to this file")
     ap.add_argument("--results", dest="results_file", default=None,
                     help="results file (default spmv_results_<workload>.json)")
@@ -446,7 +491,7 @@ def main(argv=None) -> int:
     a.results = []  # performance_result entries, filled by the workload
     torch.cuda.set_device(0)
     res = {"precision": precision, "batched": batched, "spmm": spmm, "laplacian": laplacian, "shard": shard,
-           "host": host_path}[a.what](a)
+           "host": host_path, "kernels": kernels}[a.what](a)
     txt = "\n".join(json.dumps(r) for r in res)
     print(txt, flush=True)
     if a.out:
