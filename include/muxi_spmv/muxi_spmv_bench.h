@@ -85,7 +85,9 @@ spmv_status_t muxi_measure_csr5_performance(const muxi_csr_t* csr, muxi_csr5_t* 
  * first, its preprocess stage timed once (*analysisMs, may be NULL) and its compute stage timed as
  * muxi_measure_csr_performance times the dispatching SpMV (warmup, then the median of config->measureIterations,
  * HIP events on config->stream): *timeMs and *gflops (2 * nnz / timeMs / 1e6). A speed reference and a cross-check
- * of y; the library's own SpMV never calls it. SPMV_ERROR_UNSUPPORTED_FORMAT when rocSPARSE cannot be loaded. */
+ * of y; the library's own SpMV never calls it. SPMV_ERROR_UNSUPPORTED_FORMAT for an alg outside 0..3, when
+ * rocSPARSE cannot be loaded, or when the loaded rocSPARSE rejects the algorithm; the argument errors of
+ * muxi_measure_csr_performance otherwise (timeMs or gflops NULL: SPMV_ERROR_INVALID_VECTOR). */
 spmv_status_t muxi_compare_with_rocsparse(const muxi_csr_t* A, const void* d_x, void* d_y,
                                           const muxi_bench_config_t* config, int alg, double* timeMs,
                                           double* gflops, double* analysisMs);

@@ -351,7 +351,7 @@ extern "C" spmv_status_t muxi_compare_with_rocsparse(const muxi_csr_t* A, const 
                                                   rocsparse_spmv_alg_csr_lrb, rocsparse_spmv_alg_csr_nnzsplit};
         if (alg < 0 || alg > 3) {
             set_error("compare_with_rocsparse: alg must be 0 (adaptive), 1 (rowsplit), 2 (LRB) or 3 (nnzsplit)");
-            return SPMV_ERROR_INTERNAL;
+            return SPMV_ERROR_UNSUPPORTED_FORMAT;
         }
         const RsApi& f = rs_api();
         if (!f.ok) {

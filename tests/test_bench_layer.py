@@ -115,3 +115,33 @@ def test_measure_argument_errors_before_any_device_work():
                                              256) == -2
     assert lib.muxi_measure_csr5_performance(C.byref(A), C.byref(c5), dummy, None, C.byref(cfg), None,
                                              C.byref(res), 256) == -3
+
+
+def test_rocsparse_comparison_argument_errors_before_any_device_work():
+    """muxi_compare_with_rocsparse rejects bad arguments before it loads rocSPARSE or touches the device."""
+    lib = _lib.load()
+    cfg = lib.muxi_bench_default_config()
+    dummy = C.c_void_p(1)
+    A = _lib.muxi_csr_t(numRows=4, numCols=4, nnz=0)
+    A.d_rowPtr = dummy
+    t, g, a = C.c_double(), C.c_double(), C.c_double()
+
+    def call(Am, x, y, c, alg, tp=C.byref(t)):
+        return lib.muxi_compare_with_rocsparse(Am, x, y, c, alg, tp, C.byref(g), C.byref(a))
+
+    assert call(None, dummy, dummy, C.byref(cfg), 0) == -2
+    assert call(C.byref(_lib.muxi_csr_t(numRows=4, numCols=4, nnz=0)), dummy, dummy, C.byref(cfg), 0) == -2
+    assert call(C.byref(A), None, dummy, C.byref(cfg), 0) == -3
+    assert call(C.byref(A), dummy, dummy, C.byref(cfg), 0, tp=None) == -3
+    bad = lib.muxi_bench_default_config()
+    bad.measureIterations = 0
+    assert call(C.byref(A), dummy, dummy, C.byref(bad), 0) == -3
+    cus = lib.muxi_bench_default_config()
+    cus.compareCusparse = True
+    assert call(C.byref(A), dummy, dummy, C.byref(cus), 0) == -7
+    half = lib.muxi_bench_default_config()
+    half.floatType = _lib.SPMV_TYPE_HALF
+    assert call(C.byref(A), dummy, dummy, C.byref(half), 0) == -8
+    for alg in (-1, 4):
+        assert call(C.byref(A), dummy, dummy, C.byref(cfg), alg) == -7
+        assert b"alg must be" in lib.muxi_spmv_last_error()
