@@ -66,6 +66,14 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "cannot read the arrays under %s\n", dir.c_str());
         return 2;
     }
+    // a well-formed CSR before any device work: rocSPARSE indexes x and y with these arrays unchecked
+    bool valid = A.rowPtr[0] == 0 && A.rowPtr[rows] == nnz;
+    for (int r = 0; valid && r < rows; ++r) valid = A.rowPtr[r] <= A.rowPtr[r + 1];
+    for (int k = 0; valid && k < nnz; ++k) valid = A.colIdx[k] >= 0 && A.colIdx[k] < cols;
+    if (!valid) {
+        std::fprintf(stderr, "not a CSR matrix of %d x %d with %d nonzeros\n", rows, cols, nnz);
+        return 2;
+    }
     A.allocateDevice();
     A.copyToDevice();
     float *d_x = nullptr, *d_y = nullptr;

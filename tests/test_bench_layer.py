@@ -145,3 +145,31 @@ def test_rocsparse_comparison_argument_errors_before_any_device_work():
     for alg in (-1, 4):
         assert call(C.byref(A), dummy, dummy, C.byref(cfg), alg) == -7
         assert b"alg must be" in lib.muxi_spmv_last_error()
+
+
+def test_rocsparse_timing_rejects_a_malformed_matrix_before_any_device_work(tmp_path):
+    """muxi_spmv_amd/rocsparse_timing checks the CSR arrays it is given on the host (row pointers from 0 to nnz,
+    non-decreasing; columns inside the matrix) and exits with 2 before it allocates device memory."""
+    import numpy as np
+
+    exe = ROOT / "muxi_spmv_amd" / "rocsparse_timing"
+    if not exe.exists():
+        pytest.skip("rocsparse_timing is not built (python -m muxi_spmv_amd.build)")
+    rp = np.array([0, 2, 3], np.int32)
+    vals = np.ones(3, np.float32)
+    x = np.ones(4, np.float32)
+    for name, ci in (("column past the end", np.array([0, 4, 1], np.int32)),
+                     ("negative column", np.array([0, -1, 1], np.int32))):
+        rp.tofile(tmp_path / "rowPtr.i32")
+        ci.tofile(tmp_path / "colIdx.i32")
+        vals.tofile(tmp_path / "values.f32")
+        x.tofile(tmp_path / "x.f32")
+        r = subprocess.run([str(exe), str(tmp_path), "2", "4", "3", "3"], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and "not a CSR matrix" in r.stderr, (name, r.returncode, r.stderr)
+    np.array([0, 2, 1, 3], np.int32).tofile(tmp_path / "rowPtr.i32")  # ends at nnz, decreasing in between
+    np.array([0, 1, 1], np.int32).tofile(tmp_path / "colIdx.i32")
+    r = subprocess.run([str(exe), str(tmp_path), "3", "4", "3", "3"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "not a CSR matrix" in r.stderr
+    r = subprocess.run([str(exe), str(tmp_path / "missing"), "2", "4", "3", "3"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 2 and "cannot read" in r.stderr
