@@ -239,7 +239,7 @@ typedef enum {
                                   (slabs of 16384 columns); otherwise slabs of 32768 columns with 4
                                   nonzeros a lane and plain sums, else the widest of 28672, 24576 and
                                   16384 that fits the LDS (8 a lane, compensated or checked sums);
-                                  FP32 x and y; at most 256 slabs (8,388,608 columns with 32768) */
+                                  FP32 x and y; at most 512 slabs (16,777,216 columns with 32768) */
     MUXI_KERNEL_COUNT = 10
 } muxi_kernel_t;
 
@@ -277,7 +277,7 @@ spmv_status_t muxi_plan_create_rows(const muxi_csr_t* A, spmv_type_t vtype, spmv
 /* Checked plan (no counterpart in the reference, whose FP64 path is the whole
  * spmv_csr<double> dispatch, src/spmv/csr/spmv_csr.cu:164-249): FP32 x and y,
  * values FLOAT/BF16/DOUBLE, FP32 (compensated = 0) or Dot2 (1) accumulation on
- * the LDS column-slab kernels (at most 256 slabs: 7,340,032 columns, 4,194,304
+ * the LDS column-slab kernels (at most 512 slabs: 14,680,064 columns, 8,388,608
  * with Dot2). Each
  * exec also sums |a_rj x_j| per row beside y and tests the row's a-posteriori
  * error bound -- (n_r+2)u S_r for FP32 sums, 2u|y_r| + 2((n_r+2)u)^2 S_r for

@@ -10,11 +10,17 @@ from .api import _TORCH2SPMV, _stream
 from ._lib import check, load
 
 
+BATCH_SLABS = 0x10000  # ipt flag: the slab mode (MUXI_BATCH_SLABS)
+TUNED_MERGE = -1  # ipt: the merge mode with the tuned schedule
+
+
 class BatchPlan:
     """Binds matrices (objects with .c and .dtype) and their x / y tensors."""
 
     def __init__(self, mats, xs, ys, ipt: int = 0, stream=None):
-        """ipt: merge items per lane (4, 8, 16) | workgroups per CU << 8; 0 = the tuned schedule."""
+        """ipt: 0 = the slab mode (the block-diagonal matrix through an LDS column-slab plan) when the batch
+        qualifies, else the merge mode with the tuned schedule; TUNED_MERGE (< 0) = that tuned merge schedule;
+        BATCH_SLABS = the slab mode; other values = merge items per lane (4, 8, 16) | workgroups per CU << 8."""
         n = len(mats)
         self.mats, self.xs, self.ys = mats, xs, ys  # keep alive
         arr = (_lib.muxi_csr_t * n)(*[m.c for m in mats])
@@ -37,7 +43,8 @@ class BatchPlan:
         check(load().muxi_batch_info(self._p, C.byref(t), C.byref(g), C.byref(s)), "batch_info")
         i, w = C.c_int(), C.c_int()
         check(load().muxi_batch_schedule(self._p, C.byref(i), C.byref(w)), "batch_schedule")
-        return {"tiles": t.value, "grid": g.value, "scratch_bytes": s.value, "ipt": i.value, "wg_per_cu": w.value}
+        return {"tiles": t.value, "grid": g.value, "scratch_bytes": s.value, "ipt": i.value, "wg_per_cu": w.value,
+                "mode": "slabs" if i.value == 0 else "merge"}
 
     def graph(self, per_matrix_kernel: str | None = None, param0: int = 0, alpha=1.0, beta=0.0, stream=None):
         """A captured hipGraph of one exec (or of the per-matrix loop)."""

@@ -10,8 +10,10 @@ precision  values stored FP64 / FP32 / BF16, x FP32, FP32 accumulation plain
            plans (FP32 values, rtol 1e-3, plain and Dot2): time, rows
            recomputed in FP64 and the largest relative row error.
 batched    256 CSR matrices, rows log-uniform in [1e3, 2e5], mean nnz/row
-           log-uniform in [4, 64], Poisson row lengths: one persistent
-           launch (eager and hipGraph) against one CSR-vector launch per
+           log-uniform in [4, 64], Poisson row lengths: the default batch
+           plan (the slab mode here: the block-diagonal matrix through an
+           LDS column-slab plan; eager and hipGraph), the merge mode at every
+           schedule and at the tuned one, against one CSR-vector launch per
            matrix (eager and hipGraph).
 spmm       Y = A X on the power-law matrix for k = 8 and 32 (row-major X).
 laplacian  4096^2 5-point Laplacian in FP64, CSR kernels, ELLPACK and the AUTO pick.
@@ -223,9 +225,24 @@ def batched(args) -> list[dict]:
         log(f"batched persistent, {ipt} items per lane x {wg} workgroups per CU: {ipt_ms[f'{ipt}x{wg}']:.4f} ms")
         bp.destroy()
     best_ipt = min(ipt_ms, key=ipt_ms.get)
-    bp = BatchPlan(mats, xs, ys)  # the tuned schedule
-    tuned = f"{bp.info()['ipt']}x{bp.info()['wg_per_cu']}"
+    from .batch import TUNED_MERGE
+
     times = {}
+    bm = BatchPlan(mats, xs, ys, TUNED_MERGE)  # the merge mode with the autotuner's schedule
+    tuned = f"{bm.info()['ipt']}x{bm.info()['wg_per_cu']}"
+    with torch.cuda.stream(s):
+        t_merge = tm.time(lambda: bm.exec())
+    times["batched_merge_tuned"] = tm.last
+    torch.cuda.synchronize()
+    bm.exec()
+    torch.cuda.synchronize()
+    worst_m, worst_m_abs = 0.0, 0.0
+    for A, x, y in zip(mats, xs, ys):
+        e = errors(y, api.oracle_f64(A, x))
+        worst_m, worst_m_abs = max(worst_m, e["rel_l2"]), max(worst_m_abs, e["max_abs"])
+    bm.destroy()
+    bp = BatchPlan(mats, xs, ys)  # the default: the slab mode for this batch
+    log(f"default batch plan: {bp.info()}")
 
     def worst_err():
         w, m = 0.0, 0.0
@@ -259,18 +276,21 @@ def batched(args) -> list[dict]:
     dev = device_record()
     R_all, C_all = sum(m.numRows for m in mats), sum(m.numCols for m in mats)
     for name, ms in [("batched_persistent", t_batch), ("batched_persistent_graph", t_batch_g),
-                     ("per_matrix_vector", t_vec), ("per_matrix_vector_graph", t_vec_g)]:
+                     ("batched_merge_tuned", t_merge), ("per_matrix_vector", t_vec),
+                     ("per_matrix_vector_graph", t_vec_g)]:
         out.append({"workload": "batched_256", "path": name, "ms": round(ms, 4), "GBps": round(tot_b / ms / 1e6, 1),
                     "nnz": nnz})
-        w, wa = (worst, worst_abs) if name.startswith("batched") else (worst_v, worst_v_abs)
+        w, wa = ((worst, worst_abs) if name.startswith("batched_persistent") else
+                 (worst_m, worst_m_abs) if name == "batched_merge_tuned" else (worst_v, worst_v_abs))
         args.results.append(performance_result(R_all, C_all, nnz, times[name], tot_b, rel_l2=w, max_abs=wa,
                                                tolerance=TOL_FP32, device=dev, cpu=cpu, workload="batched_256",
                                                path=name, matrices=len(mats), errors_are="worst member"))
     out.append({"workload": "batched_256", "speedup_graph_vs_per_matrix_graph": round(t_vec_g / t_batch_g, 2),
-                "speedup_eager": round(t_vec / t_batch, 2), "worst_rel_l2": worst, "info": bp.info(),
-                "schedule": tuned, "sweep_best_schedule": best_ipt,
-                "schedule_regret": round(ipt_ms[tuned] / ipt_ms[best_ipt] - 1, 4),
-                "ms_by_schedule": {k: round(v, 4) for k, v in ipt_ms.items()}})
+                "speedup_eager": round(t_vec / t_batch, 2), "speedup_vs_merge_tuned": round(t_merge / t_batch, 2),
+                "worst_rel_l2": worst, "info": bp.info(),
+                "merge_schedule": tuned, "merge_sweep_best_schedule": best_ipt,
+                "merge_schedule_regret": round(ipt_ms[tuned] / ipt_ms[best_ipt] - 1, 4),
+                "merge_ms_by_schedule": {k: round(v, 4) for k, v in ipt_ms.items()}})
     for r in out:
         log(json.dumps(r))
     return out

@@ -11,7 +11,9 @@ constexpr int kLsSlabCols = 32768;   // columns per slab: 128 KiB of FP32 x in L
 constexpr int kLsThreads = 1024;     // slab-kernel workgroup
 constexpr int kLsMaxTiles = 512;     // tiles per slab-kernel chunk (LDS head/tail sums)
 constexpr int kLsLdsBytes = 163840;  // LDS of one CU (MI355X)
-constexpr int kLsMaxSlabs = 256;     // slabs (8,388,608 columns)
+constexpr int kLsMaxSlabs = 512;     // slabs (16,777,216 columns of 32768)
+constexpr long kLsSelectorMaxCols = 256L * kLsSlabCols;  // AUTO considers lslab up to 8,388,608 columns (its
+                                                         // training domain; explicit plans go to kLsMaxSlabs)
 constexpr int kLsPairBytes = 8192;   // combine: LDS for a row block's pair sums (and as much for its map)
 constexpr int kLsRowsMax = 1024;     // combine: rows per block
 constexpr int kLsMaxRuns = 1024;     // combine blocks one slab-kernel chunk may write to

@@ -82,7 +82,7 @@ static bool candidate_allowed(int kernel, const double* f, spmv_type_t xtype, lo
     if (kernel == MUXI_KERNEL_ELL) return nnz > 0 && rows * mx <= 4.0 * nnz && rows * mx < 2147483648.0;
     if (kernel == MUXI_KERNEL_SCALAR || kernel == MUXI_KERNEL_VECTOR) return mx <= 20000;
     if (kernel == MUXI_KERNEL_LSLAB)
-        return xtype == SPMV_TYPE_FLOAT && cols >= (1L << 16) && cols <= (long)kLsMaxSlabs * kLsSlabCols;
+        return xtype == SPMV_TYPE_FLOAT && cols >= (1L << 16) && cols <= kLsSelectorMaxCols;
     return true;
 }
 

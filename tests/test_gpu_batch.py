@@ -5,7 +5,8 @@ import torch
 
 from muxi_spmv_amd import batch
 from muxi_spmv_amd import generators as G
-from gpu_util import assert_close_fp32, dev_csr, matrix_zoo
+import oracle
+from gpu_util import assert_close_fp32, dev_csr, fp32_bound, matrix_zoo
 
 pytestmark = pytest.mark.gpu
 
@@ -65,9 +66,14 @@ def test_per_matrix_paths_match():
             assert_close_fp32(y, h, x)
 
 
-def test_graph_replays_match_eager():
+MODES = {"merge": batch.TUNED_MERGE, "slabs": batch.BATCH_SLABS}
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_graph_replays_match_eager(mode):
     hs, mats, xs_h, xs, ys = make_batch(24, seed=7)
-    bp = batch.BatchPlan(mats, xs, ys)
+    bp = batch.BatchPlan(mats, xs, ys, MODES[mode])
+    assert bp.info()["mode"] == mode
     bp.exec()
     torch.cuda.synchronize()
     eager = [y.clone() for y in ys]
@@ -92,9 +98,10 @@ def test_graph_replays_match_eager():
             assert_close_fp32(y, h, x)
 
 
-def test_batch_integers_exact():
+@pytest.mark.parametrize("mode", MODES)
+def test_batch_integers_exact(mode):
     """Integer values in [-3, 3] and x in [-4, 4] on the mixed batch (Poisson matrices, empty rows, a long
-    row, tiny matrices): every matrix's y equals its integer product on the host, bit for bit."""
+    row, tiny matrices): every matrix's y equals its integer product on the host, bit for bit, in both modes."""
     hs, mats, xs_h, xs, ys = make_batch(n=30, seed=8)
     mats_i, xs_i, exps = [], [], []
     for m, h in enumerate(hs):
@@ -108,8 +115,41 @@ def test_batch_integers_exact():
         exps.append(e)
         mats_i.append(dev_csr(G.HostCSR(h.rowPtr, h.colIdx, vi.astype(np.float32), h.numRows, h.numCols)))
         xs_i.append(torch.from_numpy(xi.astype(np.float32)).cuda())
-    bp = batch.BatchPlan(mats_i, xs_i, ys)
+    bp = batch.BatchPlan(mats_i, xs_i, ys, MODES[mode])
+    assert bp.info()["mode"] == mode
     bp.exec()
     torch.cuda.synchronize()
     for e, y in zip(exps, ys):
         assert np.array_equal(y.cpu().numpy(), e.astype(np.float32))
+
+
+def test_batch_slabs_alpha_beta_and_nan_y():
+    """The slab mode scatters y with alpha and beta: y = 2 A x + 0.5 y over y = 1, and beta = 0 over a NaN y
+    (never read); members with empty rows, one long row and tiny matrices included."""
+    hs, mats, xs_h, xs, ys = make_batch(18, seed=11)
+    bp = batch.BatchPlan(mats, xs, ys, batch.BATCH_SLABS)
+    info = bp.info()
+    assert info["mode"] == "slabs" and info["tiles"] == 0 and info["scratch_bytes"] > 0
+    for y in ys:
+        y.fill_(1.0)
+    bp.exec(alpha=2.0, beta=0.5)
+    torch.cuda.synchronize()
+    for h, x, y in zip(hs, xs_h, ys):
+        y64, ab = oracle.spmv_f64(h.rowPtr, h.colIdx, h.values.astype(np.float64), np.asarray(x, np.float64))
+        want = 2.0 * y64 + 0.5
+        # twice the product's FP32 bound, then the rounding of alpha * sum and of the sum with beta * y
+        tol = 2.0 * fp32_bound(h.rowPtr, ab) + 2.0 ** -23 * (np.abs(want) + 2.0 * np.abs(y64) + 1.0)
+        assert np.all(np.abs(y.double().cpu().numpy() - want) <= tol)
+    for y in ys:
+        y.fill_(float("nan"))
+    bp.exec()
+    torch.cuda.synchronize()
+    for h, x, y in zip(hs, xs_h, ys):
+        assert_close_fp32(y, h, x)
+
+
+def test_batch_slabs_needs_fp32():
+    hs, mats, xs_h, xs, ys = make_batch(6, seed=12, dtype=torch.float64)
+    with pytest.raises(Exception, match="UNSUPPORTED_FORMAT"):
+        batch.BatchPlan(mats, xs, ys, batch.BATCH_SLABS)
+    assert batch.BatchPlan(mats, xs, ys).info()["mode"] == "merge"  # the default: merge for FP64

@@ -130,8 +130,9 @@ def test_checked_benign_matrix_recomputes_nothing():
 
 
 def test_checked_plan_limits():
-    # 8,000,000 columns: 279 slabs of 28672 (the checked plain plan's width), over the 256 a plan takes
-    h = G.HostCSR(np.array([0, 1], np.int32), np.array([7_999_999], np.int32), np.ones(1, np.float32), 1, 8_000_000)
+    # 15,000,000 columns: 524 slabs of 28672 (the checked plain plan's width), over the 512 a plan takes
+    h = G.HostCSR(np.array([0, 1], np.int32), np.array([14_999_999], np.int32), np.ones(1, np.float32), 1,
+                  15_000_000)
     with pytest.raises(api.SpmvError, match="INVALID_MATRIX"):
         api.Plan.checked(dev_csr(h), RTOL)
     q = api.Plan(dev_csr(G.HostCSR(np.array([0, 1], np.int32), np.array([0], np.int32), np.ones(1, np.float32), 1,

@@ -520,9 +520,9 @@ def test_lslab_dense_rows_and_slab_edges(param, comp):
 
 
 def test_lslab_integer_exact_many_slabs():
-    """A 300 x 8,000,000 matrix (245 slabs of 32768 columns, the last one partial) with integer values and x: y
-    equals the host's integer product, so every slab offset, chunk, pair and combine map entry lands in the right
-    place; the same with 28672- and 16384-column slabs on the first 4,000,000 columns."""
+    """A 300 x 8,000,000 matrix with integer values and x, in slabs of 32768, 28672 and 16384 columns (245, 279
+    and 489 slabs, the last one partial): y equals the host's integer product, so every slab offset, chunk, pair
+    and combine map entry lands in the right place; the same on the first 4,000,000 columns."""
     rng = np.random.default_rng(23)
     R, C = 300, 8_000_000
     lens = rng.integers(0, 4000, R)
@@ -535,11 +535,9 @@ def test_lslab_integer_exact_many_slabs():
     exp = np.zeros(R, np.int64)
     rows = np.repeat(np.arange(R), np.diff(h.rowPtr))
     np.add.at(exp, rows, h.values.astype(np.int64) * xi[h.colIdx])
-    y, _, _, _ = run_plan(h, "lslab", 4, x=xi.astype(np.float32))  # full slabs (half ones would need 489)
-    assert np.array_equal(y.cpu().numpy(), exp.astype(np.float32))
-    for param in (8, 0x104):  # 28672- and 16384-column slabs: over the 256 a plan takes
-        with pytest.raises(api.SpmvError, match="INVALID_MATRIX"):
-            run_plan(h, "lslab", param, x=xi.astype(np.float32))
+    for param in (4, 8, 0x104):
+        y, _, _, _ = run_plan(h, "lslab", param, x=xi.astype(np.float32))
+        assert np.array_equal(y.cpu().numpy(), exp.astype(np.float32)), param
     # the same pattern on its first 4,000,000 columns (140 slabs of 28672, 245 of 16384)
     keep = h.colIdx < 4_000_000
     rows = np.repeat(np.arange(R), np.diff(h.rowPtr))[keep]
@@ -558,7 +556,9 @@ def test_lslab_rejects_fp64_x_and_too_many_columns():
     A = dev_csr(G.HostCSR(h.rowPtr, h.colIdx, h.values.astype(np.float64), h.numRows, h.numCols), torch.float64)
     with pytest.raises(api.SpmvError, match="UNSUPPORTED_TYPE"):
         api.Plan(A, "lslab", 8, xtype=torch.float64)
-    wide = G.HostCSR(np.array([0, 1], np.int32), np.array([9_000_000], np.int32), np.ones(1, np.float32), 1,
-                     9_000_000)
-    with pytest.raises(api.SpmvError, match="INVALID_MATRIX"):
-        api.Plan(dev_csr(wide), "lslab", 8)
+    # 17,000,000 columns: 593 slabs of 28672 (and 519 of 32768), over the 512 a plan takes
+    wide = G.HostCSR(np.array([0, 1], np.int32), np.array([16_999_999], np.int32), np.ones(1, np.float32), 1,
+                     17_000_000)
+    for param in (4, 8):
+        with pytest.raises(api.SpmvError, match="INVALID_MATRIX"):
+            api.Plan(dev_csr(wide), "lslab", param)

@@ -4,7 +4,9 @@
   [4, 64], Poisson row lengths, uniform sorted columns (seed 5, the batched benchmark's matrices). With integer
   values in [-3, 3] and x in [-4, 4] every member's y equals the host's integer product bit for bit, eagerly and
   from hipGraph replays; with the generated U[-1, 1] values every member is within rel L2 1e-6 of the FP64 oracle,
-  and the persistent launch's hipGraph runs at least 2x faster than one CSR-vector launch per matrix (a graph too).
+  in both modes (merge tiles in one persistent launch; the slab mode, the block-diagonal matrix through an LDS
+  column-slab plan), the default (the slab mode) runs faster than the tuned merge schedule, and its hipGraph at
+  least 2x faster than one CSR-vector launch per matrix (a graph too).
 * Precision sweep, power-law leg: the 2M x 2M power-law matrix (exponent 2, lengths 1..100,000, rows shuffled)
   with values stored FP64, FP32 and BF16, x FP32, FP32 and Dot2 accumulation: rel L2 <= 1e-6 against the FP64
   oracle on the stored values, Dot2 no worse than plain, BF16 storage error within its unit roundoff; AUTO's
@@ -67,12 +69,16 @@ def _int_members(mats):
     return out
 
 
-def test_batch_256_integers_exact_eager_and_graph(batch_256):
+MODES = {"merge": batch.TUNED_MERGE, "slabs": batch.BATCH_SLABS}
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_batch_256_integers_exact_eager_and_graph(batch_256, mode):
     mats, _, ys = batch_256
     members = _int_members(mats)
-    bp = batch.BatchPlan([a for a, _, _ in members], [x for _, x, _ in members], ys)
+    bp = batch.BatchPlan([a for a, _, _ in members], [x for _, x, _ in members], ys, MODES[mode])
     info = bp.info()
-    assert info["tiles"] > 0
+    assert info["mode"] == mode and (info["tiles"] > 0) == (mode == "merge")
     bp.exec()
     torch.cuda.synchronize()
     for (_, _, e), y in zip(members, ys):
@@ -96,11 +102,12 @@ def test_batch_256_integers_exact_eager_and_graph(batch_256):
             assert np.array_equal(y.cpu().numpy(), (2 * e).astype(np.float32)), rep
 
 
-def test_batch_256_rel_l2_vs_fp64_oracle(batch_256):
+@pytest.mark.parametrize("mode", MODES)
+def test_batch_256_rel_l2_vs_fp64_oracle(batch_256, mode):
     mats, xs, ys = batch_256
     for y in ys:
         y.fill_(float("nan"))
-    bp = batch.BatchPlan(mats, xs, ys)
+    bp = batch.BatchPlan(mats, xs, ys, MODES[mode])
     bp.exec()
     torch.cuda.synchronize()
     worst = 0.0
@@ -109,7 +116,7 @@ def test_batch_256_rel_l2_vs_fp64_oracle(batch_256):
         err = float(torch.linalg.vector_norm(y.double() - y64) / torch.linalg.vector_norm(y64))
         worst = max(worst, err)
         assert err <= 1e-6, (m, err)
-    print(f"batch_256 worst rel L2 {worst:.3g}", flush=True)
+    print(f"batch_256 {mode} worst rel L2 {worst:.3g}", flush=True)
 
 
 def test_batch_per_matrix_graph_alpha_beta_and_bad_kernel(batch_256):
@@ -148,13 +155,13 @@ def _interleaved_cold(tm, fns: dict, rounds: int = 3, reps: int = 5) -> dict:
 
 
 def test_batch_256_tuned_schedule_within_5pct_of_sweep(batch_256):
-    """The schedule a batch plan takes by default (the autotuner's table) against every candidate, re-timed
-    cold in the same rounds: within 5 % of the best."""
+    """The merge schedule the autotuner's table picks against every merge candidate, re-timed cold in the same
+    rounds: within 5 % of the best."""
     from muxi_spmv_amd import sched_tune
 
     mats, xs, ys = batch_256
     tm = sched_tune._Timer()
-    bp = batch.BatchPlan(mats, xs, ys)
+    bp = batch.BatchPlan(mats, xs, ys, batch.TUNED_MERGE)
     info = bp.info()
     pick = (info["ipt"], info["wg_per_cu"])
     plans = {(ipt, wg): batch.BatchPlan(mats, xs, ys, ipt | (wg << 8)) for ipt, wg in sched_tune.BATCH_CANDIDATES}
@@ -177,7 +184,8 @@ def test_batch_256_graph_beats_per_matrix_vector_graph_2x(batch_256):
     from muxi_spmv_amd import sched_tune
 
     mats, xs, ys = batch_256
-    bp = batch.BatchPlan(mats, xs, ys)
+    bp = batch.BatchPlan(mats, xs, ys)  # the default: the slab mode
+    assert bp.info()["mode"] == "slabs"
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
         tm = sched_tune._Timer()  # its events and cache flushes on s, with the graphs
@@ -190,6 +198,23 @@ def test_batch_256_graph_beats_per_matrix_vector_graph_2x(batch_256):
     print(f"batch_256 graphs: persistent {t['batched']:.4f} ms, per-matrix vector {t['per_matrix']:.4f} ms "
           f"({t['per_matrix'] / t['batched']:.2f}x)", flush=True)
     assert t["per_matrix"] >= 2.0 * t["batched"], t
+
+
+def test_batch_256_slab_mode_beats_tuned_merge(batch_256):
+    """Why the default takes the slab mode for this batch: it runs faster than the tuned merge schedule, both
+    timed cold round-robin (2.3x measured in the launches themselves)."""
+    from muxi_spmv_amd import sched_tune
+
+    mats, xs, ys = batch_256
+    tm = sched_tune._Timer()
+    slabs = batch.BatchPlan(mats, xs, ys)
+    merge = batch.BatchPlan(mats, xs, ys, batch.TUNED_MERGE)
+    assert slabs.info()["mode"] == "slabs" and merge.info()["mode"] == "merge"
+    t = _interleaved_cold(tm, {"slabs": slabs.exec, "merge": merge.exec}, reps=3)
+    slabs.destroy()
+    merge.destroy()
+    print(f"batch_256: slab mode {t['slabs']:.4f} ms, tuned merge {t['merge']:.4f} ms", flush=True)
+    assert t["slabs"] < t["merge"], t
 
 
 @pytest.mark.parametrize("k", [8, 32])
