@@ -44,7 +44,10 @@ typedef struct muxi_batch_plan muxi_batch_plan;
  *   other > 0: merge items per lane (4, 8 or 16) | workgroups per CU << 8
  *      (1 to 32; 0 = 8).
  * The slab mode holds the block-diagonal matrix's row pointers, its slab plan
- * and one x and one y of the batch's total length. */
+ * and one x and one y of the batch's total length; it copies the members'
+ * column indices and values into that plan when it is created, so later
+ * changes to them are not seen (the merge mode reads the members' arrays at
+ * every exec). Both modes bind the xs / ys pointers at creation. */
 spmv_status_t muxi_batch_plan_create(const muxi_csr_t* mats, int count, spmv_type_t vtype, spmv_type_t xtype,
                                      const void* const* xs, void* const* ys, int ipt, void* stream,
                                      muxi_batch_plan** plan);
