@@ -148,19 +148,30 @@ __global__ __launch_bounds__(kBatchCopyThreads) void batch_gather_x_kernel(const
     int m = d.m;
     long start = d.start, next = d.next;
     const float* x = (const float*)d.v;
+    // every element's member first (a walk only past a member that ends inside the range), then every load at
+    // once, then the stores
+    const float* src[kBatchCopyEpt];
+    const long i0 = (long)blockIdx.x * kBatchCopyPer + threadIdx.x;
 #pragma unroll
     for (int k = 0; k < kBatchCopyEpt; ++k) {
-        const long i = (long)blockIdx.x * kBatchCopyPer + k * kBatchCopyThreads + threadIdx.x;
-        if (i >= n) return;
-        if (i >= next) {  // past the workgroup's first member
+        const long i = i0 + k * kBatchCopyThreads;
+        src[k] = nullptr;
+        if (i >= n) continue;
+        if (i >= next) {
             while (i >= next && m + 1 < count) {
                 start = coff[++m];
                 next = coff[m + 1];
             }
             x = xs[m];
         }
-        xcat[i] = x ? x[i - start] : 0.f;  // a member without rows may come without x: its columns are never read
+        src[k] = x ? x + (i - start) : nullptr;  // a member without rows may come without x: never read
     }
+    float v[kBatchCopyEpt];
+#pragma unroll
+    for (int k = 0; k < kBatchCopyEpt; ++k) v[k] = src[k] ? *src[k] : 0.f;
+#pragma unroll
+    for (int k = 0; k < kBatchCopyEpt; ++k)
+        if (i0 + k * kBatchCopyThreads < n) xcat[i0 + k * kBatchCopyThreads] = v[k];
 }
 
 __global__ __launch_bounds__(kBatchCopyThreads) void batch_scatter_y_kernel(const CopySeg* __restrict__ seg,
@@ -172,10 +183,13 @@ __global__ __launch_bounds__(kBatchCopyThreads) void batch_scatter_y_kernel(cons
     int m = d.m;
     long start = d.start, next = d.next;
     float* y = (float*)d.v;
+    float* dst[kBatchCopyEpt];
+    const long i0 = (long)blockIdx.x * kBatchCopyPer + threadIdx.x;
 #pragma unroll
     for (int k = 0; k < kBatchCopyEpt; ++k) {
-        const long i = (long)blockIdx.x * kBatchCopyPer + k * kBatchCopyThreads + threadIdx.x;
-        if (i >= n) return;
+        const long i = i0 + k * kBatchCopyThreads;
+        dst[k] = nullptr;
+        if (i >= n) continue;
         if (i >= next) {
             while (i >= next && m + 1 < count) {
                 start = roff[++m];
@@ -183,8 +197,14 @@ __global__ __launch_bounds__(kBatchCopyThreads) void batch_scatter_y_kernel(cons
             }
             y = ys[m];
         }
-        store_y(y, i - start, ycat[i], alpha, beta);
+        dst[k] = y + (i - start);
     }
+    float v[kBatchCopyEpt];
+#pragma unroll
+    for (int k = 0; k < kBatchCopyEpt; ++k) v[k] = dst[k] ? ycat[i0 + k * kBatchCopyThreads] : 0.f;
+#pragma unroll
+    for (int k = 0; k < kBatchCopyEpt; ++k)
+        if (dst[k]) store_y(dst[k], 0, v[k], alpha, beta);
 }
 
 // The descriptor of each copy workgroup's first member: the last m with off[m] <= b * kBatchCopyPer.
