@@ -31,7 +31,8 @@ typedef struct muxi_batch_plan muxi_batch_plan;
 #define MUXI_BATCH_SLABS 0x10000 /* ipt flag: the slab mode */
 
 /* mats[count] with device arrays; xs[count] / ys[count] device vectors of
- * numCols / numRows elements. vtype FLOAT (xtype FLOAT) or DOUBLE (xtype
+ * numCols / numRows elements (x may be NULL for a member without nonzeros, y
+ * for a member without rows). vtype FLOAT (xtype FLOAT) or DOUBLE (xtype
  * DOUBLE). ipt selects the mode and schedule:
  *   0: the slab mode when the batch qualifies -- FLOAT, at least 65,536
  *      columns and 8,388,608 nonzeros in all, at most 16,777,216 columns, rows

@@ -396,7 +396,7 @@ static spmv_status_t batch_plan_create_impl(const muxi_csr_t* mats, int count, s
         const muxi_csr_t& A = mats[m];
         if (A.numRows < 0 || A.numCols < 0 || A.nnz < 0 || (long)A.numRows + A.nnz > 2147483647L ||
             (A.numRows > 0 && !A.d_rowPtr) || (A.nnz > 0 && (!A.d_colIdx || !A.d_values)) ||
-            (A.numRows > 0 && (!xs[m] || !ys[m]))) {
+            (A.nnz > 0 && !xs[m]) || (A.numRows > 0 && !ys[m])) {  // x is read only through nonzeros
             set_error("batch: invalid matrix " + std::to_string(m));
             return SPMV_ERROR_INVALID_MATRIX;
         }
@@ -467,15 +467,8 @@ static spmv_status_t batch_plan_create_impl(const muxi_csr_t* mats, int count, s
     const long CH = 256L * ipt;
     long T = 0, ncoord = 0, vrows = 0;
     std::vector<int> P(count);
-    for (int m = 0; m < count; ++m) {
+    for (int m = 0; m < count; ++m) {  // (every member was checked above)
         const muxi_csr_t& A = mats[m];
-        if (A.numRows < 0 || A.nnz < 0 || (long)A.numRows + A.nnz > 2147483647L ||
-            (A.numRows > 0 && !A.d_rowPtr) || (A.nnz > 0 && (!A.d_colIdx || !A.d_values)) ||
-            (A.numRows > 0 && (!xs[m] || !ys[m]))) {
-            set_error("batch: invalid matrix " + std::to_string(m));
-            batch_free(p);
-            return SPMV_ERROR_INVALID_MATRIX;
-        }
         P[m] = A.numRows > 0 ? (int)(((long)A.numRows + A.nnz + CH - 1) / CH) : 0;
         BatchDesc& d = p->host[m];
         d = BatchDesc{A.d_rowPtr, A.d_colIdx, A.d_values, xs[m], ys[m], A.numRows, A.numCols, A.nnz,

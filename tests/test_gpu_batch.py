@@ -153,3 +153,41 @@ def test_batch_slabs_needs_fp32():
     with pytest.raises(Exception, match="UNSUPPORTED_FORMAT"):
         batch.BatchPlan(mats, xs, ys, batch.BATCH_SLABS)
     assert batch.BatchPlan(mats, xs, ys).info()["mode"] == "merge"  # the default: merge for FP64
+
+
+def test_batch_slabs_degenerate_members():
+    """The slab mode with members that have no rows, rows but no columns, rows and columns but no nonzeros, and
+    many tiny members inside one copy workgroup's range (1,024 elements): every y equals its integer product."""
+    rng = np.random.default_rng(21)
+    hs = []
+    for i in range(60):
+        kind = i % 6
+        if kind == 0:
+            hs.append(G.HostCSR(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32), 0, 5))
+        elif kind == 1:
+            hs.append(G.HostCSR(np.zeros(4, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32), 3, 0))
+        elif kind == 2:
+            hs.append(G.HostCSR(np.zeros(8, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32), 7, 9))
+        else:
+            R = int(rng.integers(1, 40))
+            hs.append(G.poisson_rows(R, R, 3.0, seed=3000 + i))
+    hs.append(G.poisson_rows(70_000, 70_000, 6.0, seed=77))  # one member over two slabs
+    mats, xs, ys, exps = [], [], [], []
+    for m, h in enumerate(hs):
+        i = np.arange(h.nnz, dtype=np.int64)
+        vi = ((i * 2654435761 + m) >> 7) % 7 - 3
+        xi = (np.arange(h.numCols, dtype=np.int64) * 40503 + m >> 3) % 9 - 4
+        e = np.zeros(h.numRows, np.int64)
+        if h.nnz:
+            rows = np.repeat(np.arange(h.numRows), np.diff(h.rowPtr))
+            np.add.at(e, rows, vi * xi[h.colIdx])
+        exps.append(e)
+        mats.append(dev_csr(G.HostCSR(h.rowPtr, h.colIdx, vi.astype(np.float32), h.numRows, h.numCols)))
+        xs.append(torch.from_numpy(xi.astype(np.float32)).cuda())
+        ys.append(torch.full((h.numRows,), float("nan"), device="cuda"))
+    bp = batch.BatchPlan(mats, xs, ys, batch.BATCH_SLABS)
+    assert bp.info()["mode"] == "slabs"
+    bp.exec()
+    torch.cuda.synchronize()
+    for m, (e, y) in enumerate(zip(exps, ys)):
+        assert np.array_equal(y.cpu().numpy(), e.astype(np.float32)), m
