@@ -154,7 +154,7 @@ def time_plan_cold(plan: Plan, x, y, reps: int, flusher: Flusher) -> float:
 
 
 LSLAB_MIN_COLS = 1 << 16  # below this x sits in every L2 and the two launches of lslab cannot pay off
-LSLAB_MAX_COLS = 256 * 32768  # 256 slabs of 32768 columns
+LSLAB_MAX_COLS = 256 * 32768  # the selector's lslab domain, as selector.cpp (plans themselves take 512 slabs)
 
 
 def feasible(kernel: str, f: np.ndarray, fp64: bool = False) -> bool:
