@@ -32,7 +32,8 @@ typedef struct muxi_batch_plan muxi_batch_plan;
 
 /* mats[count] with device arrays; xs[count] / ys[count] device vectors of
  * numCols / numRows elements (x may be NULL for a member without nonzeros, y
- * for a member without rows). vtype FLOAT (xtype FLOAT) or DOUBLE (xtype
+ * for a member without rows). Each member's rowPtr runs from 0 to its nnz
+ * (the slab mode checks it: SPMV_ERROR_INVALID_MATRIX). vtype FLOAT (xtype FLOAT) or DOUBLE (xtype
  * DOUBLE). ipt selects the mode and schedule:
  *   0: the slab mode when the batch qualifies -- FLOAT, at least 65,536
  *      columns and 8,388,608 nonzeros in all, at most 16,777,216 columns, rows

@@ -220,6 +220,12 @@ static std::vector<CopySeg> batch_copy_segs(const std::vector<long>& off, const 
     return f;
 }
 
+// A member's row pointers must run from 0 to its nonzero count for its block of the block-diagonal matrix to line
+// up with the column indices and values copied for it.
+__global__ void batch_rowptr_ends_kernel(const int* __restrict__ rp, int R, int nnz, int* __restrict__ bad) {
+    if (rp[0] != 0 || rp[R] != nnz) *bad = 1;
+}
+
 // dst[i] = src[i] + add for i < n (a member's row pointers or column indices moved into the block-diagonal matrix)
 __global__ __launch_bounds__(256) void batch_offset_copy_kernel(const int* __restrict__ src, int* __restrict__ dst,
                                                                 long n, int add) {
@@ -314,15 +320,20 @@ static spmv_status_t batch_slabs_build(muxi_batch_plan* p, const muxi_csr_t* mat
     const long R = roff[count], C = coff[count], N = noff[count];
     p->Rtot = R;
     p->Ctot = C;
-    DevBuf ci, va;
+    DevBuf ci, va, bad;
+    MUXI_HIP(hipMalloc(&bad.p, sizeof(int)));
+    MUXI_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
     MUXI_HIP(hipMalloc(&p->d_bd_rp, sizeof(int) * (size_t)(R + 1)));
     MUXI_HIP(hipMalloc(&ci.p, sizeof(int) * (size_t)std::max(N, 1L)));
     MUXI_HIP(hipMalloc(&va.p, sizeof(float) * (size_t)std::max(N, 1L)));
     for (int m = 0; m < count; ++m) {
         const muxi_csr_t& A = mats[m];
-        if (A.numRows > 0)
+        if (A.numRows > 0) {
+            hipLaunchKernelGGL(batch_rowptr_ends_kernel, dim3(1), dim3(1), 0, s, A.d_rowPtr, A.numRows, A.nnz,
+                               (int*)bad.p);
             hipLaunchKernelGGL(batch_offset_copy_kernel, dim3(grid_for(A.numRows, 256)), dim3(256), 0, s, A.d_rowPtr,
                                p->d_bd_rp + roff[m], (long)A.numRows, (int)noff[m]);
+        }
         if (A.nnz > 0) {
             hipLaunchKernelGGL(batch_offset_copy_kernel, dim3(grid_for(A.nnz, 256)), dim3(256), 0, s, A.d_colIdx,
                                (int*)ci.p + noff[m], (long)A.nnz, (int)coff[m]);
@@ -330,6 +341,13 @@ static spmv_status_t batch_slabs_build(muxi_batch_plan* p, const muxi_csr_t* mat
                                     hipMemcpyDeviceToDevice, s));
         }
         MUXI_LAUNCH_CHECK();
+    }
+    int hbad = 0;
+    MUXI_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    MUXI_HIP(hipStreamSynchronize(s));
+    if (hbad) {
+        set_error("batch: a member's rowPtr must start at 0 and end at its nnz");
+        return SPMV_ERROR_INVALID_MATRIX;
     }
     const int nend = (int)N;
     MUXI_HIP(hipMemcpyAsync(p->d_bd_rp + R, &nend, sizeof(int), hipMemcpyHostToDevice, s));

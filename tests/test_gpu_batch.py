@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 import torch
 
-from muxi_spmv_amd import batch
+from muxi_spmv_amd import api, batch
 from muxi_spmv_amd import generators as G
 import oracle
 from gpu_util import assert_close_fp32, dev_csr, fp32_bound, matrix_zoo
@@ -191,3 +191,18 @@ def test_batch_slabs_degenerate_members():
     torch.cuda.synchronize()
     for m, (e, y) in enumerate(zip(exps, ys)):
         assert np.array_equal(y.cpu().numpy(), e.astype(np.float32)), m
+
+
+def test_batch_slabs_rejects_a_member_whose_row_pointers_do_not_start_at_zero():
+    """Row pointers from 2 to nnz + 2 would shift the member's block of the block-diagonal matrix against its
+    copied columns: the slab mode refuses them."""
+    good = G.poisson_rows(100, 100, 4.0, seed=1)
+    rp = good.rowPtr.astype(np.int32) + 2
+    ci = np.concatenate([np.zeros(2, np.int32), good.colIdx.astype(np.int32)])
+    va = np.concatenate([np.zeros(2, np.float32), good.values.astype(np.float32)])
+    shifted = api.CSRMatrix(torch.from_numpy(rp), torch.from_numpy(ci), torch.from_numpy(va), 100)
+    shifted.c.nnz = good.nnz  # the member's nonzeros, with rowPtr[0] = 2
+    x = torch.ones(100, device="cuda")
+    y = torch.zeros(100, device="cuda")
+    with pytest.raises(api.SpmvError, match="INVALID_MATRIX"):
+        batch.BatchPlan([dev_csr(good), shifted], [x, x], [y, y.clone()], batch.BATCH_SLABS)
