@@ -202,7 +202,8 @@ def main(argv=None) -> int:
 
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the SpMV path here is single-GPU; "
-              f"row-block sharding across GPUs is described in DESIGN.md)", file=sys.stderr)
+              f"a multi-GPU run splits the rows with muxi_shard_rows and plans each share with "
+              f"muxi_plan_create_rows)", file=sys.stderr)
         return 2
 
     import numpy as np

@@ -208,7 +208,7 @@ def main(argv=None) -> int:
     ap.add_argument("--rows-min", type=float, default=1e4)
     ap.add_argument("--rows-max", type=float, default=4e6)
     ap.add_argument("--reps", type=int, default=3)
-    ap.add_argument("--db", default="gpurun_out/selector_db.jsonl")
+    ap.add_argument("--db", default="selector_db.jsonl", help="sweep records to append to (JSON lines)")
     ap.add_argument("--weights", default=None)
     ap.add_argument("--summary", default=None)
     ap.add_argument("--sweep-only", action="store_true")

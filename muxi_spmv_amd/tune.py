@@ -7,7 +7,7 @@
   time is the HIP-event time around the plan's main kernel plus its fix-up.
 * `Selector` is the MLP (HIP forward/backward/Adam kernels) that predicts
   each candidate's log time from the features; its pick is the argmin.
-  Trained on sweep records; the sweep stays the fallback and the judge.
+  Trained on sweep records; the sweep stays the fallback and the measure of the pick's regret.
 """
 from __future__ import annotations
 
