@@ -349,11 +349,14 @@ def main(argv=None) -> int:
         except Exception as e:  # noqa: BLE001
             log(f"vendor comparison skipped: {e!r}")
             out["vendor"] = {"error": repr(e)}
+    out["cpu_baseline"] = None
     if not a.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(h, x)
-    else:
-        out["cpu_baseline"] = None
+        try:
+            out["cpu_baseline"] = cpu_baseline(h, x)
+        except Exception as e:  # noqa: BLE001  (the GPU figures stand without it; say why it is missing)
+            log(f"cpu baseline failed: {e!r}")
+            out["cpu_baseline"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
     return 0
 
